@@ -1,0 +1,193 @@
+/* libcrvhip: the CRV response chain of crvfastsim (CrvSteps -> photoelectrons -> SiPM charges -> digitised
+ * waveforms -> pulses -> coincidence clusters) as HIP kernels for MI355X (gfx950), behind a plain C ABI.
+ *
+ * The reference has no FFI for this path: its operator API is the Python stage functions of crvfastsim/chain.py
+ * (charges :151, digis :234, reco_waveforms :273, clusters :296). Each entry point below replaces one of them (or a
+ * fused run of them), with the reference's tables as device arrays:
+ *   - every call returns 0 or a negative CRV_E* status; crv_last_error() gives the message;
+ *   - the caller owns every input and output buffer (device pointers, e.g. torch tensors); the library never frees
+ *     caller memory and keeps only its own workspace;
+ *   - outputs have a caller-given capacity; when a table would not fit, the call writes nothing, stores the size
+ *     it needs in the `n` field and returns CRV_E_CAPACITY (never a truncated table: the reference's
+ *     response/contract.py:24-29 require_complete rule);
+ *   - one context per host thread; calls are ordered on the given stream and return after it has finished.
+ */
+#ifndef CRVHIP_H
+#define CRVHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRV_OK 0
+#define CRV_E_ARG (-1)
+#define CRV_E_HIP (-2)
+#define CRV_E_CAPACITY (-3)
+#define CRV_E_STATE (-4)
+#define CRV_E_OVERFLOW (-5)
+
+typedef struct crv_ctx crv_ctx;
+typedef void *crv_stream; /* a hipStream_t; NULL is the null stream */
+
+/* Physics and reconstruction settings: the fields of the reference's SiPMConfig (response/sipm.py:26-47),
+ * WaveformConfig (reco/waveform.py:29-44), RecoConfig (reco/pulsefit.py:26-46), FinderConfig (reco/coinc.py:52-61)
+ * and ChainConfig (chain.py:61-84), plus the closed-form optics and the Philox seed. */
+typedef struct crv_params {
+  uint64_t seed;
+  uint32_t evkey_hi; /* event key >> 32, shared by the events of a call */
+  int32_t precision; /* 64: plain FP64 stages; 32: FP32 batched chain */
+  /* optics */
+  double pe_per_mev, ref_distance, w_short, l_short, w_long, l_long, half_length, inv_speed, tau;
+  double photon_window_hi; /* photons after it are dropped (ChainConfig.window[1]) */
+  /* SiPM */
+  int32_t n_pixels_x, n_pixels_y, pixel_lrint, _pad0;
+  double overvoltage, time_constant, capacitance, p1, p2, cross_talk;
+  /* waveforms */
+  double single_pe_reference_charge, single_pe_max_time, single_pe_precision, min_voltage, noise;
+  int32_t noise_on, samples_zs, window_samples, feb_key_roc;
+  double adc_conversion, window_lo, window_hi, feb_phase_width;
+  int32_t pedestal, min_adc, max_adc, _pad1;
+  /* pulse reco */
+  double min_adc_difference, default_beta, min_beta, max_beta, max_time_difference, min_pulse_height_ratio;
+  double max_pulse_height_ratio, le_time_factor, pedestal_undershoot_threshold, time_offset_scale, limit_tolerance;
+  int32_t samples_before, samples_after, use_time_offset_db, ignore_channels, lm_iterations, _pad2;
+  /* coincidence finder */
+  double fiber_signal_speed, max_time_interval, time_offset;
+  int32_t compensate_mask, ignore_secondary_pulses, big_cluster_threshold, add_opposite_side_pulses;
+} crv_params;
+
+/* Context on a device (the library's HIP device id). */
+int crv_ctx_create(int device, crv_ctx **out);
+int crv_ctx_destroy(crv_ctx *ctx);
+int crv_set_params(crv_ctx *ctx, const crv_params *p);
+/* Message of the last failed call on this context (or of the last failed crv_ctx_create when ctx is NULL). */
+int crv_last_error(crv_ctx *ctx, char *buf, int64_t size);
+
+/* Per-channel conditions (host arrays): reference chain.py:29-52 Conditions and readout.py:58-72 effective status;
+ * dev = the photon-yield factor (deviation * variation scale + 1) * yield scale of response/photons.py:348. */
+int crv_upload_conditions(crv_ctx *ctx, int32_t n_channels, const int32_t *status, const int32_t *roc,
+                          const int32_t *feb, const double *dev, const double *pedestal, const double *area,
+                          const double *height, const double *toff);
+/* Resampled single-PE template in volts on the single_pe_precision grid (reco/waveform.py:46-62). */
+int crv_upload_template(crv_ctx *ctx, const double *tmpl, int32_t n);
+/* Photon map [nx * ny] (response/sipm.py:49-91) and the dead-pixel mask [41 * 41] (SiPMConfig.inactive). */
+int crv_upload_photon_map(crv_ctx *ctx, const double *pmap, int32_t nx, int32_t ny, const uint8_t *dead41);
+/* Coincidence-finder geometry (reco/coinc.py:64-122 SectorGeom and SectorCuts): per bar sector, layer,
+ * counter_in_type, x, y, pos[3]; per sector type, length direction, readout-end positions, SiPM sides and the
+ * ten SectorCuts values (has_cuts = 0 for sectors without cuts). */
+int crv_upload_geometry(crv_ctx *ctx, int32_t nbars, const int32_t *sector, const int32_t *layer,
+                        const int32_t *counter_in_type, const double *x, const double *y, const double *pos,
+                        int32_t nsectors, const int32_t *sector_type, const int32_t *length_dir,
+                        const double *pos_side0, const double *pos_side1, const uint8_t *sipms_side0,
+                        const uint8_t *sipms_side1, const uint8_t *has_cuts, const double *cuts);
+
+/* CrvSteps: SoA device arrays sorted by event (32 bytes per step). */
+typedef struct crv_steps {
+  const uint32_t *event;
+  const uint32_t *counter;
+  const double *edep;
+  const double *z;
+  const double *t;
+  int64_t n;
+} crv_steps;
+
+/* SiPM charges (reference chain.charges return dict), sorted by (label, time). */
+typedef struct crv_charges {
+  int64_t *label; /* event key * 32768 + channel */
+  double *time, *charge, *pe;
+  int64_t *photon; /* step row */
+  int64_t capacity, n;
+  int64_t n_photons, capped; /* photoelectrons drawn; Poisson draws that hit the rejection bound */
+} crv_charges;
+
+/* Replaces chain.charges (chain.py:151-213, numpy branch): photons (response/photons.py) + SiPM charges
+ * (response/sipm.py). FP64, one launch per stage. */
+int crv_charges_run(crv_ctx *ctx, const crv_steps *steps, crv_charges *out, crv_stream stream);
+
+/* Zero-suppressed digis (reference chain.digis return dict) and the 128-sample ADC record of every charge cluster
+ * that has a digi. */
+typedef struct crv_digis {
+  int64_t *label;
+  double *start, *tdc0;
+  int64_t *starttdc;
+  int16_t *adc; /* [capacity * samples_zs] */
+  int32_t *record; /* record row of each digi */
+  int32_t *sample; /* first sample of each digi in its record */
+  int64_t capacity, n;
+  uint16_t *record_adc; /* [record_capacity * window_samples] */
+  int64_t *record_label;
+  int32_t *record_tick; /* window start = tick * 12.5 + tdc0 */
+  int64_t record_capacity, n_records;
+  int64_t overlong; /* clusters longer than the window */
+} crv_digis;
+
+/* Replaces chain.digis (chain.py:234-270, numpy branch: tdc0_per_feb + reco/waveform.py synthesize). Input
+ * charges grouped by label and time-ordered within it (as crv_charges_run writes them). */
+int crv_digis_run(crv_ctx *ctx, const int64_t *label, const double *time, const double *charge, int64_t n,
+                  crv_digis *out, crv_stream stream);
+
+/* Reco waveforms (reference chain.reco_waveforms, chain.py:273-286 + reco/pulsefit.py:319-345): continuing
+ * digis of one channel concatenated, sorted by (event, channel, start). Samples are flat, waveform w owning
+ * samples [off[w], off[w+1]). */
+typedef struct crv_waveforms {
+  int64_t *ev;
+  int32_t *chan;
+  int64_t *start; /* start TDC */
+  int32_t *nsamp;
+  int64_t *off;   /* [capacity + 1] */
+  int16_t *adc;   /* [sample_capacity] */
+  int64_t capacity, n, sample_capacity, n_samples;
+} crv_waveforms;
+
+int crv_reco_waveforms_run(crv_ctx *ctx, const int64_t *label, const int64_t *starttdc, const int16_t *adc,
+                           int64_t n, crv_waveforms *out, crv_stream stream);
+
+/* Reco pulses (reference reco/pulsefit.py:236-316 reconstruct_full, then filter_channels and the DB time offset
+ * of chain.py:317-323), sorted by (waveform, sequence). */
+typedef struct crv_pulses {
+  int64_t *wf, *seq;
+  int64_t *ev;   /* event key of the pulse's waveform (optional, NULL) */
+  int32_t *chan; /* channel of the pulse's waveform (optional, NULL) */
+  double *pe, *peph, *time, *height, *beta, *chi2, *letime, *ped, *fit_a, *fit_mu, *fit_b;
+  uint8_t *failed, *zerondf, *pedfromdb;
+  int64_t capacity, n;
+} crv_pulses;
+
+/* Waveforms as flat samples: ev, chan, start TDC, nsamp, off per waveform; calibration per waveform. */
+int crv_pulses_run(crv_ctx *ctx, const int64_t *ev, const int32_t *chan, const int64_t *start, const int32_t *nsamp,
+                   const int64_t *off, const int16_t *adc, int64_t n_wf, const double *ped, const double *area,
+                   const double *height, const double *toff, crv_pulses *out, crv_stream stream);
+
+/* Coincidence clusters (reference chain.clusters detail output, chain.py:325-362 + reco/coinc.py find_event). */
+typedef struct crv_clusters {
+  int64_t *cl_event;
+  int32_t *cl_type, *cl_poscalc, *cl_sidehits; /* sidehits [capacity * 2] */
+  double *cl_pe, *cl_start, *cl_end, *cl_slope, *cl_avgtime, *cl_pos, *cl_sidepe, *cl_sidetime; /* pos [cap*3] */
+  int64_t *cl_mem_off; /* [capacity + 1] */
+  int64_t capacity, n;
+  int64_t *mem_idx; /* pulse index within its event */
+  int64_t mem_capacity, n_mem;
+} crv_clusters;
+
+/* Pulses of several events sorted by (event, waveform, sequence): ev, bar, sipm, time, pe, seq per pulse. */
+int crv_clusters_run(crv_ctx *ctx, const int64_t *ev, const int32_t *bar, const int32_t *sipm, const double *time,
+                     const double *pe, const int64_t *seq, int64_t n, crv_clusters *out, crv_stream stream);
+
+/* Per-stage timings of the last chain call, in milliseconds (HIP events on the chain's stream). */
+typedef struct crv_timing {
+  double charges_ms, digis_ms, waveforms_ms, pulses_ms, clusters_ms, total_ms;
+} crv_timing;
+
+/* The whole chain, one launch per stage: steps -> pulses and clusters (plain FP64, the per-stage functions in
+ * sequence on device-resident tables). Pulse and cluster tables as above; waveform records optional (NULL). */
+int crv_chain_run(crv_ctx *ctx, const crv_steps *steps, crv_pulses *pulses, crv_clusters *clusters,
+                  crv_timing *timing, crv_stream stream);
+/* Copy the pulse and cluster tables of the last crv_chain_run again (after CRV_E_CAPACITY, into larger buffers). */
+int crv_chain_fetch(crv_ctx *ctx, crv_pulses *pulses, crv_clusters *clusters, crv_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,109 @@
+// Internal declarations of libcrvhip: device tables, the context, workspace and the stage launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/crvhip.h"
+#include "crv_math.h"
+
+#define CRV_CHAN_MULT 32768
+#define CRV_NPIX 41
+#define CRV_DEAD_WORDS 53   // ceil(41 * 41 / 32)
+
+namespace crv {
+
+// Read-only device tables uploaded once per context (conditions, template, photon map, finder geometry).
+struct Tables {
+  int32_t n_channels = 0;
+  const int32_t *status = nullptr, *roc = nullptr, *feb = nullptr;
+  const double *dev = nullptr, *pedestal = nullptr, *area = nullptr, *height = nullptr, *toff = nullptr;
+  const double *tmpl = nullptr;
+  int32_t ntmpl = 0;
+  double vmax = 0.0;
+  const double *cdf = nullptr;
+  int32_t ncdf = 0, map_ny = 0;
+  double cdf_total = 0.0;
+  const uint32_t *dead = nullptr;  // CRV_DEAD_WORDS words, bit pix = x * 41 + y
+  // finder geometry
+  int32_t nbars = 0, nsectors = 0, ntypes = 0;
+  const int32_t *sector = nullptr, *layer = nullptr, *cit = nullptr;
+  const double *bx = nullptr, *by = nullptr, *bpos = nullptr;
+  const int32_t *stype = nullptr, *ldir = nullptr;
+  const double *ps0 = nullptr, *ps1 = nullptr;
+  const uint8_t *sside0 = nullptr, *sside1 = nullptr, *hascuts = nullptr;
+  const double *cuts = nullptr;  // [nsectors * 10] SectorCuts fields in declaration order
+  // per sector type (sorted type ids): p0, p1, half, max_dt, side0, side1
+  const int32_t *type_id = nullptr;
+  const double *tp0 = nullptr, *tp1 = nullptr, *thalf = nullptr, *tmaxdt = nullptr;
+  const uint8_t *tside0 = nullptr, *tside1 = nullptr;
+};
+
+// Derived scalars of the parameters (computed on the host with the same portable functions as the device).
+struct Derived {
+  double yscale;  // pe_per_mev / A(ref_distance)
+  double q_ct;    // per-neighbour cross-talk candidate probability
+  int32_t nreach;
+};
+
+struct Ctx {
+  int device = 0;
+  crv_params p{};
+  bool have_params = false;
+  Derived d{};
+  Tables t;
+  std::vector<void *> owned;  // device allocations of the tables
+  std::vector<void *> ws;     // workspace slots
+  std::vector<size_t> ws_size;
+  int *flags = nullptr;       // device error / diagnostic counters
+  std::string err;
+  hipEvent_t ev[8];
+  crv_pulses last_pulses{};      // tables of the last chained run (workspace)
+  crv_clusters last_clusters{};
+};
+
+// workspace slot of at least `bytes` bytes (grown, never shrunk)
+void *ws(Ctx *c, int slot, size_t bytes);
+
+// device flag words
+enum Flag { F_UNSORTED = 0, F_COUNTER = 1, F_CAPPED = 2, F_OVERFLOW = 3, F_CASCADE = 4, F_OVERLONG = 5, F_BADCHAN = 6,
+            F_NFLAGS = 8 };
+
+int set_error(Ctx *c, int code, const std::string &msg);
+#define CRV_HIP(c, call)                                                                  \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) return set_error((c), CRV_E_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- stage launchers (host functions; all asynchronous on `s`) ----
+struct StepGroups {  // hit counters: steps grouped by (event, counter)
+  int64_t n_steps = 0, T = 0;
+  const uint32_t *ev = nullptr, *counter = nullptr;
+  const double *edep = nullptr, *z = nullptr, *t = nullptr;
+  uint64_t *task_key = nullptr;  // (event << 13) | counter
+  int32_t *task_off = nullptr;   // [T + 1]
+  int32_t *perm = nullptr;       // step rows in task order
+  int32_t *evfirst = nullptr;    // first row of each step's event
+};
+int group_steps(Ctx *c, const crv_steps *st, StepGroups &g, hipStream_t s);
+int charges_stage(Ctx *c, StepGroups &g, crv_charges *out, hipStream_t s);
+int digis_stage(Ctx *c, const int64_t *label, const double *time, const double *charge, int64_t n, crv_digis *out,
+                hipStream_t s);
+int waveforms_stage(Ctx *c, const int64_t *label, const int64_t *starttdc, const int16_t *adc, int64_t n,
+                    crv_waveforms *out, hipStream_t s);
+int pulses_stage(Ctx *c, const int64_t *ev, const int32_t *chan, const int64_t *start, const int32_t *nsamp,
+                 const int64_t *off, const int16_t *adc, int64_t n_wf, const double *ped, const double *area,
+                 const double *height, const double *toff, crv_pulses *out, hipStream_t s);
+int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t *sipm, const double *time,
+                   const double *pe, const int64_t *seq, int64_t n, crv_clusters *out, hipStream_t s);
+
+// device-side exclusive scan of int64 counts (n + 1 outputs, last = total) with hipcub
+int scan_i64(Ctx *c, const int64_t *in, int64_t *out, int64_t n, int slot, hipStream_t s);
+int scan_i32_to_i64(Ctx *c, const int32_t *in, int64_t *out, int64_t n, int slot, hipStream_t s);
+int read_i64(Ctx *c, const int64_t *dev, int64_t *host, hipStream_t s);
+
+inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace crv

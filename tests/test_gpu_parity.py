@@ -1,0 +1,156 @@
+"""HIP kernels (through the C-ABI) against the oracle on the same seeded inputs: each stage, then the whole chain.
+
+Integer results (PE counts, charge rows, digi ADC words, cluster membership) must be bit-identical. The FP64
+kernels evaluate the oracle's operation sequence, so pulse heights and times are compared at 1e-12 relative
+(the bar is 1e-9).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from crvhip import chain as CH, config as CF, detector as DT, synth as SY, template as TP  # noqa: E402
+from oracle import chain as OC, pulsefit as OPF  # noqa: E402
+
+RTOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def setup():
+    cfg = CF.ChainConfig()
+    cond = DT.nominal_conditions()
+    G = DT.sector_geom()
+    tmpl = TP.load(cfg.waveform)
+    S = SY.cosmic_events(300, seed=0)
+    Qo = OC.charges(S, cond, cfg, 5)
+    return cfg, cond, G, tmpl, S, Qo
+
+
+def test_charges_bit_exact(setup):
+    cfg, cond, G, tmpl, S, Qo = setup
+    timers = {}
+    Q = CH.charges(S, None, G, cond, cfg, None, key=5, timers=timers)
+    assert timers["photons"] == Qo["n_photons"]
+    for k in ("chan", "time", "photon", "pe", "charge"):
+        np.testing.assert_array_equal(Q[k], Qo[k], err_msg=k)
+
+
+def test_digis_and_records_bit_exact(setup):
+    cfg, cond, G, tmpl, S, Qo = setup
+    D = CH.digis(Qo, cond, cfg, None, key=5)
+    Do = OC.digis(Qo, cond, cfg, 5, tmpl)
+    assert len(D["chan"]) == len(Do["chan"]) > 1000
+    for k in ("chan", "starttdc", "adc", "start", "tdc0"):
+        np.testing.assert_array_equal(D[k], Do[k], err_msg=k)
+    np.testing.assert_array_equal(D["sample"], Do["digi_sample"])
+    np.testing.assert_array_equal(D["record_adc"], Do["record_adc"].astype(np.uint16))
+    assert D["overlong"] == Do["overlong"]
+
+
+def test_reco_waveforms_match(setup):
+    cfg, cond, G, tmpl, S, Qo = setup
+    Do = OC.digis(Qo, cond, cfg, 5, tmpl)
+    W = CH.reco_waveforms(Do, cond)
+    Wo = OC.reco_waveforms(Do, cond)
+    for k in ("ev", "bar", "sipm", "chan", "n", "start", "adc", "ped", "area", "height", "toff"):
+        np.testing.assert_array_equal(W[k], Wo[k], err_msg=k)
+
+
+def _pulse_compare(P, Po):
+    for k in ("wf", "seq", "failed", "zerondf", "pedfromdb"):
+        np.testing.assert_array_equal(P[k], Po[k], err_msg=k)
+    for k in ("pe", "peph", "time", "height", "beta", "chi2", "letime", "ped", "fit_A", "fit_mu", "fit_b"):
+        np.testing.assert_allclose(P[k], Po[k], rtol=RTOL, atol=0, err_msg=k)
+
+
+def test_pulses_match(setup):
+    cfg, cond, G, tmpl, S, Qo = setup
+    Wo = OC.reco_waveforms(OC.digis(Qo, cond, cfg, 5, tmpl), cond)
+    P = CH.pulses(Wo, cond, cfg, G)
+    _pulse_compare(P, OC.pulses(Wo, cond, cfg))
+
+
+def test_pulses_match_reference_golden(golden):
+    cfg = CF.ChainConfig()
+    cond = DT.nominal_conditions()
+    for name in ("synth12", "edge36", "double36", "train252", "chain"):
+        args = [golden[f"pf_{name}_in_{k}"] for k in ("adc", "n", "start", "ped", "area", "height")]
+        n = len(args[1])
+        W = dict(ev=np.zeros(n, np.int64), bar=np.zeros(n, np.int64), sipm=np.zeros(n, np.int64), adc=args[0],
+                 n=args[1], start=args[2], ped=args[3], area=args[4], height=args[5], toff=np.zeros(n))
+        P = CH.pulses(W, cond, cfg)
+        _pulse_compare(P, OPF.reconstruct_full(*args))
+        g = lambda k: golden[f"pf_{name}_{k}"]
+        for k in ("wf", "seq", "failed"):
+            np.testing.assert_array_equal(P[k], g(k))
+        np.testing.assert_allclose(P["pe"], g("pe"), rtol=2e-8)
+        np.testing.assert_allclose(P["time"], g("time"), atol=1e-6)
+
+
+def _cluster_compare(out, ref):
+    for k in ("cl_event", "cl_type", "cl_poscalc", "mem_cl", "mem_idx", "mem_event"):
+        np.testing.assert_array_equal(np.asarray(out[k]).astype(np.int64), np.asarray(ref[k]).astype(np.int64),
+                                      err_msg=k)
+    for k in ("cl_pe", "cl_start", "cl_end", "cl_avgtime", "cl_pos"):
+        np.testing.assert_allclose(out[k], ref[k], rtol=RTOL, atol=1e-9, err_msg=k)
+    np.testing.assert_allclose(out["cl_slope"], ref["cl_slope"], rtol=1e-9, atol=1e-9)
+
+
+def test_clusters_match(setup):
+    cfg, cond, G, tmpl, S, Qo = setup
+    Wo = OC.reco_waveforms(OC.digis(Qo, cond, cfg, 5, tmpl), cond)
+    out, _ = CH.clusters(Wo, cond, G, None, cfg, detail=True)
+    ref, _ = OC.clusters(Wo, cond, G, cfg)
+    assert len(ref["cl_event"]) > 100
+    _cluster_compare(out[0], ref)
+
+
+def test_clusters_match_reference_golden(golden):
+    cfg = CF.ChainConfig()
+    cond = DT.nominal_conditions()
+    G = DT.sector_geom()
+    for name in ("synth", "chain"):
+        g = lambda k: golden[f"co_{name}_{k}"]
+        status = g("status").astype(np.int64)
+        cond_s = DT.Conditions(cond.pedestal, cond.pulsearea, cond.pulseheight, cond.timeoffset, status, cond.roc,
+                               cond.feb, cond.deviation)
+        ev = g("in_ev")
+        o = np.lexsort((g("in_idx"), ev))
+        assert np.array_equal(o, np.arange(len(o)))
+        out = CH.find_clusters(ev, g("in_bar"), g("in_sipm"), g("in_time"), g("in_pe"), g("in_seq"), cfg, cond_s, G)
+        np.testing.assert_array_equal(out["cl_event"], g("cl_event"))
+        np.testing.assert_array_equal(out["mem_idx"], g("members"))
+        np.testing.assert_allclose(out["cl_pe"], g("cl_pe"), rtol=1e-12)
+
+
+def test_chain_edge_cases():
+    cfg = CF.ChainConfig(sipm=CF.SiPMConfig(cross_talk=0.2, pixel_id="truncate"),
+                         photon_map=np.random.default_rng(3).random((40, 40)) ** 2)
+    cond = DT.nominal_conditions()
+    cond.status[[0, 9, 130]] = [1, 4, 2]          # bit 0 / 2: no SiPM; bit 1: pulses ignored
+    G = DT.sector_geom()
+    tmpl = TP.load(cfg.waveform)
+    S = SY.uneven_events(60, seed=4)
+    Q = CH.charges(S, None, G, cond, cfg, None, key=9)
+    Qo = OC.charges(S, cond, cfg, 9)
+    for k in ("chan", "time", "photon", "pe"):
+        np.testing.assert_array_equal(Q[k], Qo[k], err_msg=k)
+    D = CH.digis(Qo, cond, cfg, None, key=9)
+    Do = OC.digis(Qo, cond, cfg, 9, tmpl)
+    np.testing.assert_array_equal(D["adc"], Do["adc"])
+    Wo = OC.reco_waveforms(Do, cond)
+    _pulse_compare(CH.pulses(Wo, cond, cfg, G), OC.pulses(Wo, cond, cfg))
+
+
+def test_empty_inputs():
+    cfg = CF.ChainConfig()
+    cond = DT.nominal_conditions()
+    G = DT.sector_geom()
+    S = SY.cosmic_events(0, seed=0)
+    Q = CH.charges(S, None, G, cond, cfg, None, key=1)
+    assert len(Q["chan"]) == 0
+    D = CH.digis(Q, cond, cfg, None, key=1)
+    assert len(D["chan"]) == 0
+    W = CH.reco_waveforms(D, cond)
+    out, timers = CH.clusters(W, cond, G, None, cfg, detail=True)
+    assert len(out[0]["cl_event"]) == 0 and timers["pulses"] == 0
