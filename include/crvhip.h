@@ -175,17 +175,24 @@ typedef struct crv_clusters {
 int crv_clusters_run(crv_ctx *ctx, const int64_t *ev, const int32_t *bar, const int32_t *sipm, const double *time,
                      const double *pe, const int64_t *seq, int64_t n, crv_clusters *out, crv_stream stream);
 
-/* Per-stage timings of the last chain call, in milliseconds (HIP events on the chain's stream). */
+/* Per-stage and per-kernel timings of the last chain call, in milliseconds (HIP events on the chain's stream), and
+ * the table sizes of the call. */
 typedef struct crv_timing {
   double charges_ms, digis_ms, waveforms_ms, pulses_ms, clusters_ms, total_ms;
+  double k_sipm_ms, k_digi_ms, k_pulse_ms, k_coinc_ms; /* the main kernel of each stage */
+  int64_t n_steps, n_hit_counters, n_photons, n_charges, n_digis, n_records, n_waveforms, n_pulses, n_clusters,
+      n_members;
 } crv_timing;
 
-/* The whole chain, one launch per stage: steps -> pulses and clusters (plain FP64, the per-stage functions in
- * sequence on device-resident tables). Pulse and cluster tables as above; waveform records optional (NULL). */
+/* The whole chain, one launch per stage: steps -> ADC records, pulses and clusters (plain FP64, the per-stage
+ * functions in sequence on device-resident tables). The 128-sample records stay in the library's workspace
+ * (crv_chain_records gives their device address); pulse and cluster tables are copied to the caller's buffers. */
 int crv_chain_run(crv_ctx *ctx, const crv_steps *steps, crv_pulses *pulses, crv_clusters *clusters,
                   crv_timing *timing, crv_stream stream);
 /* Copy the pulse and cluster tables of the last crv_chain_run again (after CRV_E_CAPACITY, into larger buffers). */
 int crv_chain_fetch(crv_ctx *ctx, crv_pulses *pulses, crv_clusters *clusters, crv_stream stream);
+/* Device addresses of the last chain run's waveform records: adc [n * window_samples] u16, label [n] i64. */
+int crv_chain_records(crv_ctx *ctx, const uint16_t **adc, const int64_t **label, int64_t *n);
 
 #ifdef __cplusplus
 }

@@ -76,13 +76,20 @@ class Clusters(C.Structure):
                                          ("mem_capacity", i64), ("n_mem", i64)]
 
 
+TIMING_COUNTS = ("n_steps", "n_hit_counters", "n_photons", "n_charges", "n_digis", "n_records", "n_waveforms",
+                 "n_pulses", "n_clusters", "n_members")
+
+
 class Timing(C.Structure):
-    _fields_ = [(k, f64) for k in ("charges_ms", "digis_ms", "waveforms_ms", "pulses_ms", "clusters_ms", "total_ms")]
+    _fields_ = [(k, f64) for k in ("charges_ms", "digis_ms", "waveforms_ms", "pulses_ms", "clusters_ms", "total_ms",
+                                   "k_sipm_ms", "k_digi_ms", "k_pulse_ms", "k_coinc_ms")] + \
+        [(k, i64) for k in TIMING_COUNTS]
 
 
 EXPORTS = ("crv_ctx_create", "crv_ctx_destroy", "crv_set_params", "crv_last_error", "crv_upload_conditions",
            "crv_upload_template", "crv_upload_photon_map", "crv_upload_geometry", "crv_charges_run", "crv_digis_run",
-           "crv_reco_waveforms_run", "crv_pulses_run", "crv_clusters_run", "crv_chain_run", "crv_chain_fetch")
+           "crv_reco_waveforms_run", "crv_pulses_run", "crv_clusters_run", "crv_chain_run", "crv_chain_fetch",
+           "crv_chain_records")
 
 _lib = None
 
@@ -110,6 +117,7 @@ def load(path=LIB_PATH):
     lib.crv_clusters_run.argtypes = [P] + [P] * 6 + [i64, C.POINTER(Clusters), P]
     lib.crv_chain_run.argtypes = [P, C.POINTER(Steps), C.POINTER(Pulses), C.POINTER(Clusters), C.POINTER(Timing), P]
     lib.crv_chain_fetch.argtypes = [P, C.POINTER(Pulses), C.POINTER(Clusters), P]
+    lib.crv_chain_records.argtypes = [P, C.POINTER(P), C.POINTER(P), C.POINTER(i64)]
     for name in EXPORTS:
         getattr(lib, name).restype = C.c_int
     _lib = lib

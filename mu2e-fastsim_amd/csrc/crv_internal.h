@@ -61,6 +61,9 @@ struct Ctx {
   hipEvent_t ev[8];
   crv_pulses last_pulses{};      // tables of the last chained run (workspace)
   crv_clusters last_clusters{};
+  crv_digis last_digis{};
+  hipEvent_t kev[8];             // start / end of the main kernel of each stage (sipm, digi, pulse, coinc)
+  bool kev_on = false;
 };
 
 // workspace slot of at least `bytes` bytes (grown, never shrunk)
@@ -105,5 +108,10 @@ int scan_i32_to_i64(Ctx *c, const int32_t *in, int64_t *out, int64_t n, int slot
 int read_i64(Ctx *c, const int64_t *dev, int64_t *host, hipStream_t s);
 
 inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// HIP events around the main kernel `k` (0 sipm, 1 digi, 2 pulse, 3 coinc) when the chained run asks for them
+inline void kmark(Ctx *c, int k, int end, hipStream_t s) {
+  if (c->kev_on) (void)hipEventRecord(c->kev[2 * k + end], s);
+}
 
 }  // namespace crv

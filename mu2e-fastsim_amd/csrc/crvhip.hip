@@ -231,6 +231,7 @@ int crv_ctx_create(int device, crv_ctx **out) {
     return set_error(nullptr, CRV_E_HIP, "hipMalloc failed");
   }
   for (auto &e : ctx->c.ev) (void)hipEventCreate(&e);
+  for (auto &e : ctx->c.kev) (void)hipEventCreate(&e);
   *out = ctx;
   return CRV_OK;
 }
@@ -244,6 +245,7 @@ int crv_ctx_destroy(crv_ctx *ctx) {
   for (void *p : ctx->c.owned) (void)hipFree(p);
   if (ctx->c.flags) (void)hipFree(ctx->c.flags);
   for (auto &e : ctx->c.ev) (void)hipEventDestroy(e);
+  for (auto &e : ctx->c.kev) (void)hipEventDestroy(e);
   delete ctx;
   return CRV_OK;
 }
@@ -453,6 +455,11 @@ int crv_chain_run(crv_ctx *ctx, const crv_steps *steps, crv_pulses *pulses, crv_
   if (int rc = ready(c, true)) return rc;
   hipStream_t s = (hipStream_t)stream;
   hipEvent_t *ev = c->ev;
+  c->kev_on = true;
+  struct Off {
+    Ctx *c;
+    ~Off() { c->kev_on = false; }
+  } off_{c};
   CRV_HIP(c, hipEventRecord(ev[0], s));
   StepGroups g;
   int rc = group_steps(c, steps, g, s);
@@ -504,10 +511,38 @@ int crv_chain_run(crv_ctx *ctx, const crv_steps *steps, crv_pulses *pulses, crv_
     float tot = 0;
     (void)hipEventElapsedTime(&tot, ev[0], ev[5]);
     timing->total_ms = tot;
+    float km[4] = {0, 0, 0, 0};
+    bool ran[4] = {g.T > 0, q.n > 0, w.n > 0, P.n > 0};
+    for (int k = 0; k < 4; ++k)
+      if (ran[k]) (void)hipEventElapsedTime(&km[k], c->kev[2 * k], c->kev[2 * k + 1]);
+    timing->k_sipm_ms = km[0];
+    timing->k_digi_ms = km[1];
+    timing->k_pulse_ms = km[2];
+    timing->k_coinc_ms = km[3];
+    timing->n_steps = steps->n;
+    timing->n_hit_counters = g.T;
+    timing->n_photons = q.n_photons;
+    timing->n_charges = q.n;
+    timing->n_digis = d.n;
+    timing->n_records = d.n_records;
+    timing->n_waveforms = w.n;
+    timing->n_pulses = P.n;
+    timing->n_clusters = C.n;
+    timing->n_members = C.n_mem;
   }
+  c->last_digis = d;
   c->last_pulses = P;
   c->last_clusters = C;
   return crv_chain_fetch(ctx, pulses, clusters, stream);
+}
+
+int crv_chain_records(crv_ctx *ctx, const uint16_t **adc, const int64_t **label, int64_t *n) {
+  if (int rc = guard(ctx)) return rc;
+  if (!adc || !label || !n) return set_error(&ctx->c, CRV_E_ARG, "NULL argument");
+  *adc = ctx->c.last_digis.record_adc;
+  *label = ctx->c.last_digis.record_label;
+  *n = ctx->c.last_digis.n_records;
+  return CRV_OK;
 }
 
 int crv_chain_fetch(crv_ctx *ctx, crv_pulses *pulses, crv_clusters *clusters, crv_stream stream) {

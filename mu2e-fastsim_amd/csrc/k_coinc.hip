@@ -536,9 +536,11 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     return set_error(c, CRV_E_HIP, "workspace allocation failed");
   Ev e{bar, sipm, seq};
   CRV_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int) * F_NFLAGS, s));
+  kmark(c, 3, 0, s);
   k_coinc<<<nblocks(nev, 64), 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, sc, ncl, nmem, cl_ev, cl_type,
                                           cl_pc, cl_sh, cl_d, cl_moff, mem, c->flags);
   CRV_HIP(c, hipGetLastError());
+  kmark(c, 3, 1, s);
   if ((rc = scan_i32_to_i64(c, ncl, clo, nev, 124, s))) return rc;
   if ((rc = scan_i32_to_i64(c, nmem, mo, nev, 125, s))) return rc;
   int64_t nc = 0, nmt = 0;

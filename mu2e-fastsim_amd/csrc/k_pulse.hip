@@ -427,9 +427,11 @@ int pulses_stage(Ctx *c, const int64_t *ev, const int32_t *chan, const int64_t *
   sl.zerondf = (uint8_t *)ws(c, 79, ncap + 8);
   sl.pedfromdb = (uint8_t *)ws(c, 80, ncap + 8);
   if (!sl.seq || !sl.failed || !sl.zerondf || !sl.pedfromdb) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  kmark(c, 2, 0, s);
   k_pulses<<<nblocks(n_wf, 64), 64, 0, s>>>(c->p, c->t, n_wf, chan, start, nsamp, off, adc, ped, area, height, toff,
                                             yscr, poff, sl, npul);
   CRV_HIP(c, hipGetLastError());
+  kmark(c, 2, 1, s);
   if ((rc = scan_i32_to_i64(c, npul, pout, n_wf, 81, s))) return rc;
   int64_t np = 0;
   if ((rc = read_i64(c, pout + n_wf, &np, s))) return rc;

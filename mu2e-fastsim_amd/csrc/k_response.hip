@@ -413,9 +413,11 @@ int charges_stage(Ctx *c, StepGroups &g, crv_charges *out, hipStream_t s) {
     int16_t *c_pix = (int16_t *)ws(c, 25, total * 2 + 8);
     if (!c_time || !c_v || !c_row || !c_pix) return set_error(c, CRV_E_HIP, "charge workspace allocation failed");
     CRV_HIP(c, hipMemsetAsync(c->flags + F_OVERFLOW, 0, sizeof(int) * 2, s));
+    kmark(c, 0, 0, s);
     k_sipm<<<nblocks(L, 64), 64, 0, s>>>(c->p, c->t, c->d, st, T, g.task_key, g.task_off, g.perm, g.evfirst, nph,
                                          coff, gen, c_time, c_v, c_row, c_pix, ccount, c->flags);
     CRV_HIP(c, hipGetLastError());
+    kmark(c, 0, 1, s);
     int hf[F_NFLAGS];
     CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
     CRV_HIP(c, hipStreamSynchronize(s));

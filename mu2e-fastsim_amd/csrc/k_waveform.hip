@@ -291,9 +291,11 @@ int digis_stage(Ctx *c, const int64_t *label, const double *time, const double *
       !rflag || !rout)
     return set_error(c, CRV_E_HIP, "workspace allocation failed");
   CRV_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int) * F_NFLAGS, s));
+  kmark(c, 1, 0, s);
   k_wf_synth<<<nblocks(nseg, 64), 64, 0, s>>>(p, c->t, c->d.nreach, nseg, seg_label, soff, time, charge, roff, doff,
                                               rec_adc, rec_has, rec_tick, rec_tdc0, ds, ndig, c->flags);
   CRV_HIP(c, hipGetLastError());
+  kmark(c, 1, 1, s);
   k_rec_flags<<<nblocks(nrec, 256), 256, 0, s>>>(nrec, rec_has, rflag);
   if ((rc = scan_i64(c, rflag, rout, nrec, 54, s))) return rc;
   if ((rc = scan_i32_to_i64(c, ndig, dout, nseg, 55, s))) return rc;
@@ -308,7 +310,7 @@ int digis_stage(Ctx *c, const int64_t *label, const double *time, const double *
   out->overlong = hf[F_OVERLONG];
   out->n = nd;
   out->n_records = nr;
-  if (out->capacity < 0) {  // internal outputs of the chained run (no records)
+  if (out->capacity < 0) {  // internal outputs of the chained run (records in the workspace too)
     out->label = (int64_t *)ws(c, 305, nd * 8 + 8);
     out->start = (double *)ws(c, 306, nd * 8 + 8);
     out->tdc0 = (double *)ws(c, 307, nd * 8 + 8);
@@ -316,12 +318,14 @@ int digis_stage(Ctx *c, const int64_t *label, const double *time, const double *
     out->adc = (int16_t *)ws(c, 309, nd * p.samples_zs * 2 + 8);
     out->record = nullptr;
     out->sample = nullptr;
-    out->record_adc = nullptr;
-    out->record_label = nullptr;
-    out->record_tick = nullptr;
-    if (!out->label || !out->start || !out->tdc0 || !out->starttdc || !out->adc)
+    out->record_adc = (uint16_t *)ws(c, 360, nr * p.window_samples * 2 + 8);
+    out->record_label = (int64_t *)ws(c, 361, nr * 8 + 8);
+    out->record_tick = (int32_t *)ws(c, 362, nr * 4 + 8);
+    if (!out->label || !out->start || !out->tdc0 || !out->starttdc || !out->adc || !out->record_adc ||
+        !out->record_label || !out->record_tick)
       return set_error(c, CRV_E_HIP, "digi table allocation failed");
     out->capacity = nd;
+    out->record_capacity = nr;
   }
   if (nd > out->capacity || (out->record_adc && nr > out->record_capacity))
     return set_error(c, CRV_E_CAPACITY, "digis: output capacity too small");
