@@ -1,14 +1,17 @@
 #!/usr/bin/env python3
 """CRV response chain benchmark on one MI355X: events/s from CrvSteps to coincidence clusters.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--events N] [--workload cosmic|uneven|overlay]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--events N] [--fp64-events M] [--workload cosmic|uneven|overlay]
 
-A step is one pass of the chain over the whole synthetic event batch (inputs resident in HBM before timing). The JSON
-line carries the dominant kernel's roofline (algorithmic bytes per launch over its HIP-event duration), the chain's
-byte-contract roofline (32 S + 268 W + 32 P + 56 C + 4 M bytes per event), and the oracle timed on this host's
-cores over event shards (a reported baseline).
+A step is one pass of the chain over the whole synthetic event batch, inputs resident in HBM before timing, outputs
+(pulse and cluster tables) copied into caller buffers inside the timed region. The headline is the FP32 batched chain
+(crv_chain_run_batched) on N events; the FP64 plain chain (crv_chain_run, one launch per stage, bit-exact to the
+oracle) is measured beside it on M events. The JSON line carries the dominant kernel's roofline (algorithmic bytes
+per launch over its HIP-event duration on the chain's stream), the chain's byte contract per event, and the oracle
+timed on this host's cores over event shards (a reported baseline).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -25,12 +28,19 @@ HBM_PEAK_GBS = 8000.0
 
 # algorithmic bytes per table row (inputs read once, outputs written once)
 STEP_B, RECORD_B, PULSE_B, CLUSTER_B, MEMBER_B = 32, 268, 32, 56, 4
+# FP64 plain chain kernels
 CHARGE_OUT_B = 22      # time f64, voltage f64, step row i32, pixel i16 written by the SiPM kernel
 CHARGE_IN_B = 16       # time and charge read by the digitisation kernel
 DIGI_SLOT_B = 16       # record, sample, start of a digi
 SAMPLE_B = 2           # one ADC word read by the pulse kernel
 WAVEFORM_META_B = 40   # start, length, offset, calibration of a reco waveform
 PULSE_IN_B = 40        # event, bar, sipm, time, pe, seq read by the finder
+# FP32 batched chain kernels
+SIPM_STEP_B = 20       # per (step, SiPM) read by kb_sipm: photon count i32, z f64, t f64
+FIRE_B = 14            # fired-pixel log entry written by kb_sipm: time f64, overvoltage f32, pixel i16
+FIRE_IN_B = 12         # log entry read by kb_wave: time, overvoltage
+RUN_B = 32             # reco-waveform descriptor: record, first sample, samples, run number i32; SiPM, start TDC i64
+SLOT_B = 48            # pulse slot written by kb_fit: time, location f64; 7 x f32; flags u32
 
 
 def generate(workload, n, seed):
@@ -73,7 +83,105 @@ def cpu_baseline(S, n_events, seed, per_worker, workers):
     n = len(shards) * per_worker
     return {"value": n / dt, "unit": "events/s", "cores": len(shards), "kind": "port",
             "sample": f"{n} events ({len(shards)} shards of {per_worker} contiguous events of the same synthetic "
-                      f"batch), oracle chain steps->clusters, {dt:.1f} s wall"}
+                      f"batch), oracle chain steps->clusters (FP64), {dt:.1f} s wall"}
+
+
+class Outputs:
+    """Caller-side pulse and cluster buffers (torch device tensors) sized for n events."""
+
+    def __init__(self, L, torch, n_ev):
+        cap_p, cap_c = 90 * n_ev + 1024, 12 * n_ev + 1024
+        b = {k: torch.empty(cap_p, dtype=torch.int64, device="cuda") for k in ("wf", "seq", "ev")}
+        b["chan"] = torch.empty(cap_p, dtype=torch.int32, device="cuda")
+        for k in L.PULSE_F64:
+            b[k] = torch.empty(cap_p, dtype=torch.float64, device="cuda")
+        for k in L.PULSE_U8:
+            b[k] = torch.empty(cap_p, dtype=torch.uint8, device="cuda")
+        c = {"cl_event": torch.empty(cap_c, dtype=torch.int64, device="cuda"),
+             "cl_mem_off": torch.empty(cap_c + 1, dtype=torch.int64, device="cuda"),
+             "mem_idx": torch.empty(cap_p, dtype=torch.int64, device="cuda")}
+        for k in ("cl_type", "cl_poscalc"):
+            c[k] = torch.empty(cap_c, dtype=torch.int32, device="cuda")
+        c["cl_sidehits"] = torch.empty(2 * cap_c, dtype=torch.int32, device="cuda")
+        for k in L.CLUSTER_F64:
+            m = 3 if k == "cl_pos" else 2 if k in ("cl_sidepe", "cl_sidetime") else 1
+            c[k] = torch.empty(m * cap_c, dtype=torch.float64, device="cuda")
+        self.keep = (b, c)
+        self.P = L.Pulses(*[L.ptr(b[k]) for k in ("wf", "seq", "ev", "chan") + L.PULSE_F64 + L.PULSE_U8], cap_p, 0)
+        self.C = L.Clusters(L.ptr(c["cl_event"]), L.ptr(c["cl_type"]), L.ptr(c["cl_poscalc"]),
+                            L.ptr(c["cl_sidehits"]), *[L.ptr(c[k]) for k in L.CLUSTER_F64], L.ptr(c["cl_mem_off"]),
+                            cap_c, 0, L.ptr(c["mem_idx"]), cap_p, 0)
+
+
+def measure(precision, S, n_ev, seed, steps, warmup, batch):
+    """Time `steps` chain passes over the batch; returns (events/s, ms/step, mean timing fields, last counts)."""
+    import torch
+    from crvhip import _lib as L, config as CF, detector as DT, template as TP
+    from crvhip.chain import DeviceSteps
+    cfg = CF.ChainConfig(precision=precision)
+    ctx = L.Context(cfg, DT.nominal_conditions(), DT.sector_geom(), TP.load(cfg.waveform), 0)
+    ds = DeviceSteps(S["evkey"], S["counter"], S["edep"], S["z"], S["t"])
+    ctx.set_params(cfg, seed, ds.evkey_hi, 32 if precision == "fp32" else 64)
+    out = Outputs(L, torch, n_ev)
+    st = ds.struct()
+    tm = L.Timing()
+    opts = L.BatchOpts(**batch)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def step():
+        if precision == "fp32":
+            rc = ctx.lib.crv_chain_run_batched(ctx.h, ctypes.byref(st), ctypes.byref(opts), ctypes.byref(out.P),
+                                               ctypes.byref(out.C), ctypes.byref(tm), stream)
+        else:
+            rc = ctx.lib.crv_chain_run(ctx.h, ctypes.byref(st), ctypes.byref(out.P), ctypes.byref(out.C),
+                                       ctypes.byref(tm), stream)
+        ctx.check(rc)
+        return {f: getattr(tm, f) for f, _ in L.Timing._fields_}
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    runs = [step() for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    avg = {k: float(np.mean([r[k] for r in runs])) for k in runs[0]}
+    extra = {"n_waveforms_batched": opts.n_waveforms, "pulse_slots_used": opts.pulse_slots_used} \
+        if precision == "fp32" else {}
+    ctx.close()
+    del out, ds
+    torch.cuda.empty_cache()
+    return n_ev * steps / dt, dt / steps * 1e3, avg, runs[-1], extra
+
+
+def kernels_fp64(avg, cnt):
+    S_, W_, P_, C_, M_ = (cnt["n_steps"], cnt["n_records"], cnt["n_pulses"], cnt["n_clusters"], cnt["n_members"])
+    return {
+        "k_sipm": (avg["k_sipm_ms"], STEP_B * S_ + CHARGE_OUT_B * cnt["n_charges"]),
+        "k_wf_synth": (avg["k_digi_ms"], CHARGE_IN_B * cnt["n_charges"] + RECORD_B * W_ + DIGI_SLOT_B * cnt["n_digis"]),
+        "k_pulses": (avg["k_pulse_ms"], SAMPLE_B * 12 * cnt["n_digis"] + WAVEFORM_META_B * cnt["n_waveforms"]
+                     + PULSE_B * P_),
+        "k_coinc": (avg["k_coinc_ms"], PULSE_IN_B * P_ + CLUSTER_B * C_ + MEMBER_B * M_),
+    }
+
+
+def kernels_fp32(avg, cnt, fires):
+    S_, W_, R_, P_, C_, M_ = (cnt["n_steps"], cnt["n_records"], cnt["n_waveforms"], cnt["n_pulses"],
+                              cnt["n_clusters"], cnt["n_members"])
+    return {
+        "kb_sipm": (avg["k_sipm_ms"], SIPM_STEP_B * S_ + FIRE_B * fires),
+        "kb_wave": (avg["k_digi_ms"], FIRE_IN_B * fires + RECORD_B * W_ + RUN_B * R_),
+        "kb_fit": (avg["k_pulse_ms"], RUN_B * R_ + SAMPLE_B * 24 * R_ + SLOT_B * P_),
+        "k_coinc": (avg["k_coinc_ms"], PULSE_IN_B * P_ + CLUSTER_B * C_ + MEMBER_B * M_),
+    }
+
+
+def roofline(kernels):
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    kms, kbytes = kernels[dom]
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": kbytes, "ms_per_launch": kms}
 
 
 def main():
@@ -81,10 +189,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--events", type=int, default=100_000)
+    ap.add_argument("--events", type=int, default=1_000_000, help="events of the FP32 batched chain (headline)")
+    ap.add_argument("--fp64-events", type=int, default=100_000, help="events of the FP64 plain chain (0: skip)")
     ap.add_argument("--workload", default="cosmic")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-per-worker", type=int, default=40)
+    ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--sort", type=int, default=1)
+    ap.add_argument("--cpu-per-worker", type=int, default=150)
     ap.add_argument("--cpu-workers", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -93,10 +205,7 @@ def main():
               "event ranges, one process per GPU). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    from crvhip import _lib as L, config as CF, detector as DT, template as TP
-    from crvhip.chain import DeviceSteps
     S = generate(args.workload, args.events, args.seed)
-
     cpu = None
     if not args.no_cpu:
         workers = args.cpu_workers or min(len(os.sched_getaffinity(0)), 16)
@@ -104,82 +213,43 @@ def main():
 
     import torch
     assert torch.cuda.is_available(), "bench needs a GPU"
-    cfg = CF.ChainConfig()
-    cond = DT.nominal_conditions()
-    G = DT.sector_geom()
-    ctx = L.Context(cfg, cond, G, TP.load(cfg.waveform), 0)
-    ds = DeviceSteps(S["evkey"], S["counter"], S["edep"], S["z"], S["t"])
-    ctx.set_params(cfg, args.seed + 1, ds.evkey_hi)
-    n_ev = args.events
-    cap_p, cap_c = 90 * n_ev + 1024, 12 * n_ev + 1024
-    bufs = {k: torch.empty(cap_p, dtype=torch.int64, device="cuda") for k in ("wf", "seq", "ev")}
-    bufs["chan"] = torch.empty(cap_p, dtype=torch.int32, device="cuda")
-    for k in L.PULSE_F64:
-        bufs[k] = torch.empty(cap_p, dtype=torch.float64, device="cuda")
-    for k in L.PULSE_U8:
-        bufs[k] = torch.empty(cap_p, dtype=torch.uint8, device="cuda")
-    cb = {"cl_event": torch.empty(cap_c, dtype=torch.int64, device="cuda"),
-          "cl_mem_off": torch.empty(cap_c + 1, dtype=torch.int64, device="cuda"),
-          "mem_idx": torch.empty(cap_p, dtype=torch.int64, device="cuda")}
-    for k in ("cl_type", "cl_poscalc"):
-        cb[k] = torch.empty(cap_c, dtype=torch.int32, device="cuda")
-    cb["cl_sidehits"] = torch.empty(2 * cap_c, dtype=torch.int32, device="cuda")
-    for k in L.CLUSTER_F64:
-        m = 3 if k == "cl_pos" else 2 if k in ("cl_sidepe", "cl_sidetime") else 1
-        cb[k] = torch.empty(m * cap_c, dtype=torch.float64, device="cuda")
-    P = L.Pulses(*[L.ptr(bufs[k]) for k in ("wf", "seq", "ev", "chan") + L.PULSE_F64 + L.PULSE_U8], cap_p, 0)
-    C = L.Clusters(L.ptr(cb["cl_event"]), L.ptr(cb["cl_type"]), L.ptr(cb["cl_poscalc"]), L.ptr(cb["cl_sidehits"]),
-                   *[L.ptr(cb[k]) for k in L.CLUSTER_F64], L.ptr(cb["cl_mem_off"]), cap_c, 0, L.ptr(cb["mem_idx"]),
-                   cap_p, 0)
-    import ctypes
-    st = ds.struct()
-    tm = L.Timing()
-    stream = torch.cuda.current_stream().cuda_stream
-
-    def step():
-        ctx.check(ctx.lib.crv_chain_run(ctx.h, ctypes.byref(st), ctypes.byref(P), ctypes.byref(C), ctypes.byref(tm),
-                                        ctypes.c_void_p(stream)))
-        return {f: getattr(tm, f) for f, _ in L.Timing._fields_}
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    runs = [step() for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    value = n_ev * args.steps / dt
-    avg = {k: float(np.mean([r[k] for r in runs])) for k in runs[0]}
-    cnt = runs[-1]
-    S_, W_, P_, C_, M_ = (cnt["n_steps"], cnt["n_records"], cnt["n_pulses"], cnt["n_clusters"], cnt["n_members"])
-    b_event = (STEP_B * S_ + RECORD_B * W_ + PULSE_B * P_ + CLUSTER_B * C_ + MEMBER_B * M_) / n_ev
-    kernels = {
-        "k_sipm": (avg["k_sipm_ms"], STEP_B * S_ + CHARGE_OUT_B * cnt["n_charges"]),
-        "k_wf_synth": (avg["k_digi_ms"], CHARGE_IN_B * cnt["n_charges"] + RECORD_B * W_ + DIGI_SLOT_B * cnt["n_digis"]),
-        "k_pulses": (avg["k_pulse_ms"], SAMPLE_B * 12 * cnt["n_digis"] + WAVEFORM_META_B * cnt["n_waveforms"]
-                     + PULSE_B * P_),
-        "k_coinc": (avg["k_coinc_ms"], PULSE_IN_B * P_ + CLUSTER_B * C_ + MEMBER_B * M_),
-    }
-    dom = max(kernels, key=lambda k: kernels[k][0])
-    kms, kbytes = kernels[dom]
-    achieved = kbytes / (kms * 1e-3) / 1e9
+    batch = {"grid": args.grid, "tile": args.tile, "sort_tasks": args.sort}
+    v32, ms32, avg32, cnt32, ex32 = measure("fp32", S, args.events, args.seed + 1, args.steps, args.warmup, batch)
+    k32 = kernels_fp32(avg32, cnt32, cnt32["n_charges"])
+    fp64 = None
+    if args.fp64_events:
+        n64 = min(args.fp64_events, args.events)
+        ev = S["evkey"]
+        cut = int(np.searchsorted(ev, n64))
+        S64 = {k: v[:cut] for k, v in S.items()}
+        v64, ms64, avg64, cnt64, _ = measure("fp64", S64, n64, args.seed + 1, args.steps, args.warmup, batch)
+        k64 = kernels_fp64(avg64, cnt64)
+        fp64 = {"value": v64, "unit": "events/s", "events": n64, "ms_per_step": ms64, "dtype": "f64",
+                "speedup_fp32_over_fp64": v32 / v64, "roofline": roofline(k64),
+                "kernel_ms": {k: v[0] for k, v in k64.items()},
+                "stage_ms": {k: avg64[k] for k in ("charges_ms", "digis_ms", "waveforms_ms", "pulses_ms",
+                                                   "clusters_ms", "total_ms")},
+                "per_event": {k[2:]: cnt64[k] / n64 for k in cnt64 if k.startswith("n_")}}
+    S_, W_, P_, C_, M_ = (cnt32["n_steps"], cnt32["n_records"], cnt32["n_pulses"], cnt32["n_clusters"],
+                          cnt32["n_members"])
+    b_event = (STEP_B * S_ + RECORD_B * W_ + PULSE_B * P_ + CLUSTER_B * C_ + MEMBER_B * M_) / args.events
     line = {
-        "metric": METRIC, "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.workload} events, FP64 plain chain, one launch per stage", "events": n_ev,
-                   "seed": args.seed, "chain": "steps->photons->SiPM->waveforms->pulses->clusters",
-                   "precision": "fp64"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": kbytes,
-                     "ms_per_launch": kms},
-        "chain_roofline": {"bytes_per_event": b_event, "achieved": b_event * value / 1e9, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": b_event * value / 1e9 / HBM_PEAK_GBS},
+        "metric": METRIC, "value": v32, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.workload} events, FP32 batched chain (3 fused kernels + finder)",
+                   "events": args.events, "seed": args.seed, "precision": "fp32 (fp64 times and 3x3 solve)",
+                   "chain": "steps->photons->SiPM->waveforms->pulses->clusters", "grid": args.grid,
+                   "tile": args.tile or 64, "sort_tasks": args.sort},
+        "roofline": roofline(k32),
+        "chain_roofline": {"bytes_per_event": b_event, "achieved": b_event * v32 / 1e9, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": b_event * v32 / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
-        "stage_ms": {k: avg[k] for k in ("charges_ms", "digis_ms", "waveforms_ms", "pulses_ms", "clusters_ms",
-                                         "total_ms")},
-        "kernel_ms": {k: v[0] for k, v in kernels.items()},
-        "per_event": {k[2:]: cnt[k] / n_ev for k in L.TIMING_COUNTS},
+        "fp64_plain": fp64,
+        "stage_ms": {k: avg32[k] for k in ("charges_ms", "digis_ms", "pulses_ms", "clusters_ms", "total_ms")},
+        "kernel_ms": {k: v[0] for k, v in k32.items()},
+        "per_event": {k[2:]: cnt32[k] / args.events for k in cnt32 if k.startswith("n_")},
+        "batched": ex32,
     }
     print(json.dumps(line))
 

@@ -194,6 +194,30 @@ int crv_chain_fetch(crv_ctx *ctx, crv_pulses *pulses, crv_clusters *clusters, cr
 /* Device addresses of the last chain run's waveform records: adc [n * window_samples] u16, label [n] i64. */
 int crv_chain_records(crv_ctx *ctx, const uint16_t **adc, const int64_t **label, int64_t *n);
 
+/* Options and diagnostics of the FP32 batched chain. */
+typedef struct crv_batch_opts {
+  int32_t grid;        /* persistent workgroups of 64 lanes of the SiPM kernels (0: all that fit on the device) */
+  int32_t tile;        /* SiPMs per work-queue grab, a multiple of 64 (0: 64) */
+  int32_t pulse_slots; /* initial pulse slots per reco waveform (0: 4); grown and re-run when one has more */
+  int32_t sort_tasks;  /* 1: run the SiPMs longest first (photon count), so the lanes of a wave carry similar work */
+  /* written by the call: the window_samples-long ADC record of every cluster with a digi, in no particular order
+   * (label -1: unused slot), in the library's workspace until the next call on the context */
+  const uint16_t *record_adc;
+  const int64_t *record_label;
+  const int32_t *record_tick;
+  int64_t n_records, n_waveforms;
+  int32_t overlong, capped, pulse_slots_used, _pad;
+} crv_batch_opts;
+
+/* The whole chain as the FP32 batched pipeline: steps -> pulses in one persistent fused kernel (photons, SiPM,
+ * waveform, pulse fit per SiPM lane; no charge or digi tables), then the coincidence finder. FP32 arithmetic; FP64
+ * for absolute times and the 3 x 3 normal-equation solve of the fit. Same outputs and conventions as crv_chain_run
+ * (pulse `wf` is the SiPM index of the call), equal to the FP64 chain statistically (same Philox streams, FP32
+ * rounding). The reference runs this chain as chain.charges -> digis -> reco_waveforms -> clusters
+ * (chain.py:151-362); this entry point has no single counterpart there. */
+int crv_chain_run_batched(crv_ctx *ctx, const crv_steps *steps, crv_batch_opts *opts, crv_pulses *pulses,
+                          crv_clusters *clusters, crv_timing *timing, crv_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,10 +86,16 @@ class Timing(C.Structure):
         [(k, i64) for k in TIMING_COUNTS]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("grid", i32), ("tile", i32), ("pulse_slots", i32), ("sort_tasks", i32), ("record_adc", P),
+                ("record_label", P), ("record_tick", P), ("n_records", i64), ("n_waveforms", i64), ("overlong", i32),
+                ("capped", i32), ("pulse_slots_used", i32), ("_pad", i32)]
+
+
 EXPORTS = ("crv_ctx_create", "crv_ctx_destroy", "crv_set_params", "crv_last_error", "crv_upload_conditions",
            "crv_upload_template", "crv_upload_photon_map", "crv_upload_geometry", "crv_charges_run", "crv_digis_run",
            "crv_reco_waveforms_run", "crv_pulses_run", "crv_clusters_run", "crv_chain_run", "crv_chain_fetch",
-           "crv_chain_records")
+           "crv_chain_records", "crv_chain_run_batched")
 
 _lib = None
 
@@ -118,6 +124,8 @@ def load(path=LIB_PATH):
     lib.crv_chain_run.argtypes = [P, C.POINTER(Steps), C.POINTER(Pulses), C.POINTER(Clusters), C.POINTER(Timing), P]
     lib.crv_chain_fetch.argtypes = [P, C.POINTER(Pulses), C.POINTER(Clusters), P]
     lib.crv_chain_records.argtypes = [P, C.POINTER(P), C.POINTER(P), C.POINTER(i64)]
+    lib.crv_chain_run_batched.argtypes = [P, C.POINTER(Steps), C.POINTER(BatchOpts), C.POINTER(Pulses),
+                                          C.POINTER(Clusters), C.POINTER(Timing), P]
     for name in EXPORTS:
         getattr(lib, name).restype = C.c_int
     _lib = lib

@@ -338,6 +338,103 @@ def find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G):
     return cluster_table(b, out.n, out.n_mem)
 
 
+def _pulse_bufs(n):
+    torch = _torch()
+    b = {k: _empty(n, torch.int64) for k in ("wf", "seq", "ev")}
+    b["chan"] = _empty(n, torch.int32)
+    for k in L.PULSE_F64:
+        b[k] = _empty(n, torch.float64)
+    for k in L.PULSE_U8:
+        b[k] = _empty(n, torch.uint8)
+    s = L.Pulses(*[L.ptr(b[k]) for k in ("wf", "seq", "ev", "chan") + L.PULSE_F64 + L.PULSE_U8], n, 0)
+    return b, s
+
+
+def _cluster_bufs(nc, nm):
+    torch = _torch()
+    b = {"cl_event": _empty(nc, torch.int64), "cl_type": _empty(nc, torch.int32), "cl_poscalc": _empty(nc, torch.int32),
+         "cl_sidehits": _empty(2 * nc, torch.int32), "cl_mem_off": _empty(nc + 1, torch.int64),
+         "mem_idx": _empty(nm, torch.int64)}
+    for k in L.CLUSTER_F64:
+        b[k] = _empty(3 * nc if k == "cl_pos" else 2 * nc if k in ("cl_sidepe", "cl_sidetime") else nc, torch.float64)
+    s = L.Clusters(L.ptr(b["cl_event"]), L.ptr(b["cl_type"]), L.ptr(b["cl_poscalc"]), L.ptr(b["cl_sidehits"]),
+                   *[L.ptr(b[k]) for k in L.CLUSTER_F64], L.ptr(b["cl_mem_off"]), nc, 0, L.ptr(b["mem_idx"]), nm, 0)
+    return b, s
+
+
+def _fetch(addr, n, dtype):
+    """n elements at a device address of the library's workspace, copied to a new host array (hipMemcpy of the HIP
+    runtime the process already runs; the soname resolves to the loaded copy)."""
+    import ctypes
+    out = np.empty(max(int(n), 0), dtype)
+    if n:
+        hip = ctypes.CDLL("libamdhip64.so.7")
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        rc = hip.hipMemcpy(out.ctypes.data, addr, out.nbytes, 2)  # hipMemcpyDeviceToHost
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed ({rc})")
+    return out
+
+
+def run_chain(steps, cond, cfg, key=0, geom=None, batch=None, timers=None, records=False):
+    """CrvSteps -> pulses and coincidence clusters in one device call (the reference drives the same stages one by one,
+    analyses/runchain.py:517-530 -> chain.py:151-362).
+
+    cfg.precision "fp64": the plain FP64 chain (one launch per stage, equal to the oracle bit for bit); "fp32": the
+    batched FP32 chain (three fused kernels, FP64 only for absolute times and the fit's 3 x 3 solve), equal to the
+    FP64 chain statistically. `batch` holds crv_batch_opts fields for the FP32 chain (grid, tile, pulse_slots,
+    sort_tasks). Returns {"pulses": {...}, "clusters": cluster table, "timing": {...}} with pulses in (event, channel,
+    time) order and `wf` the pulse's reco-waveform number; with records=True (FP32 chain) also "records", the ADC
+    records of the clusters with digis sorted by (label, FEB tick)."""
+    import ctypes
+    from . import detector as DT
+    G = geom if geom is not None else DT.sector_geom()
+    ev, counter, edep, z, t, _ = step_table(steps)
+    ds = DeviceSteps(ev, counter, edep, z, t)
+    ctx = context(cfg, cond, G)
+    ctx.set_params(cfg, _seed(key, None), ds.evkey_hi, 32 if cfg.precision == "fp32" else 64)
+    st = ds.struct()
+    tm = L.Timing()
+    guess = max(1024, 4 * ds.n)
+    pb, ps = _pulse_bufs(guess)
+    cb, cs = _cluster_bufs(guess, guess)
+    opts = L.BatchOpts()
+    for k, v in (batch or {}).items():
+        setattr(opts, k, v)
+    if cfg.precision == "fp32":
+        rc = ctx.lib.crv_chain_run_batched(ctx.h, ctypes.byref(st), ctypes.byref(opts), ctypes.byref(ps),
+                                           ctypes.byref(cs), ctypes.byref(tm), None)
+    elif cfg.precision == "fp64":
+        rc = ctx.lib.crv_chain_run(ctx.h, ctypes.byref(st), ctypes.byref(ps), ctypes.byref(cs), ctypes.byref(tm), None)
+    else:
+        raise ValueError(f"precision must be fp64 or fp32, not {cfg.precision!r}")
+    if rc == -3:  # CRV_E_CAPACITY: the sizes are in n; fetch again into larger buffers
+        pb, ps = _pulse_bufs(ps.n)
+        cb, cs = _cluster_bufs(cs.n, cs.n_mem)
+        rc = ctx.lib.crv_chain_fetch(ctx.h, ctypes.byref(ps), ctypes.byref(cs), None)
+    ctx.check(rc)
+    n = ps.n
+    P = {k: _host(pb[k], n) for k in ("wf", "seq", "ev", "chan") + L.PULSE_F64 + L.PULSE_U8}
+    for k in L.PULSE_U8:
+        P[k] = P[k].astype(bool)
+    P["bar"], P["sipm"] = P["chan"] // 4, P["chan"] % 4
+    timing = {f: getattr(tm, f) for f, _ in L.Timing._fields_}
+    if cfg.precision == "fp32":
+        timing.update(n_records=opts.n_records, n_waveforms=opts.n_waveforms, overlong=opts.overlong,
+                      capped=opts.capped, pulse_slots_used=opts.pulse_slots_used)
+    if timers is not None:
+        timers.update(timing)
+    out = {"pulses": P, "clusters": cluster_table(cb, cs.n, cs.n_mem), "timing": timing}
+    if records and cfg.precision == "fp32":  # the records of the clusters with digis, by (label, tick)
+        nr, Wn = opts.n_records, cfg.window_samples
+        lab, tick = _fetch(opts.record_label, nr, np.int64), _fetch(opts.record_tick, nr, np.int32)
+        adc = _fetch(opts.record_adc, nr * Wn, np.uint16).reshape(nr, Wn)
+        keep = np.flatnonzero(lab >= 0)
+        o = keep[np.lexsort((tick[keep], lab[keep]))]
+        out["records"] = {"label": lab[o], "tick": tick[o], "adc": adc[o]}
+    return out
+
+
 def cluster_table(b, nc, nm):
     mo = _host(b["cl_mem_off"], nc + 1)
     cnt = np.diff(mo) if nc else np.zeros(0, np.int64)

@@ -20,10 +20,12 @@ struct Tables {
   const int32_t *status = nullptr, *roc = nullptr, *feb = nullptr;
   const double *dev = nullptr, *pedestal = nullptr, *area = nullptr, *height = nullptr, *toff = nullptr;
   const double *tmpl = nullptr;
+  const float *tmpl_f = nullptr;  // FP32 copy
   int32_t ntmpl = 0;
   double vmax = 0.0;
   const double *cdf = nullptr;
   int32_t ncdf = 0, map_ny = 0;
+  int32_t map_uniform = 0;  // every map bin is 1: the CDF lookup is floor(x)
   double cdf_total = 0.0;
   const uint32_t *dead = nullptr;  // CRV_DEAD_WORDS words, bit pix = x * 41 + y
   // finder geometry
@@ -64,6 +66,7 @@ struct Ctx {
   crv_digis last_digis{};
   hipEvent_t kev[8];             // start / end of the main kernel of each stage (sipm, digi, pulse, coinc)
   bool kev_on = false;
+  int64_t b_rec_cap = 0, b_run_cap = 0;  // pool sizes of the batched chain (grown on demand)
 };
 
 // workspace slot of at least `bytes` bytes (grown, never shrunk)
@@ -101,6 +104,8 @@ int pulses_stage(Ctx *c, const int64_t *ev, const int32_t *chan, const int64_t *
                  const double *height, const double *toff, crv_pulses *out, hipStream_t s);
 int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t *sipm, const double *time,
                    const double *pe, const int64_t *seq, int64_t n, crv_clusters *out, hipStream_t s);
+// the FP32 batched chain; pulses and clusters end in last_pulses / last_clusters
+int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timing *timing, hipStream_t s);
 
 // device-side exclusive scan of int64 counts (n + 1 outputs, last = total) with hipcub
 int scan_i64(Ctx *c, const int64_t *in, int64_t *out, int64_t n, int slot, hipStream_t s);

@@ -293,6 +293,8 @@ int crv_upload_template(crv_ctx *ctx, const double *tmpl, int32_t n) {
   if (n <= 0 || !tmpl) return set_error(c, CRV_E_ARG, "template: bad arguments");
   int rc = upload(c, tmpl, n, &c->t.tmpl);
   if (rc) return rc;
+  std::vector<float> tf(tmpl, tmpl + n);  // the FP32 batched chain's copy
+  if ((rc = upload(c, tf.data(), n, &c->t.tmpl_f))) return rc;
   c->t.ntmpl = n;
   double m = tmpl[0];
   for (int i = 1; i < n; ++i) m = std::max(m, tmpl[i]);
@@ -320,6 +322,9 @@ int crv_upload_photon_map(crv_ctx *ctx, const double *pmap, int32_t nx, int32_t 
     return rc;
   c->t.ncdf = nx * ny;
   c->t.map_ny = ny;
+  c->t.map_uniform = 1;
+  for (int i = 0; i < nx * ny; ++i)
+    if (pmap[i] != 1.0) c->t.map_uniform = 0;
   c->t.cdf_total = acc;
   return CRV_OK;
 }
@@ -534,6 +539,24 @@ int crv_chain_run(crv_ctx *ctx, const crv_steps *steps, crv_pulses *pulses, crv_
   c->last_pulses = P;
   c->last_clusters = C;
   return crv_chain_fetch(ctx, pulses, clusters, stream);
+}
+
+int crv_chain_run_batched(crv_ctx *ctx, const crv_steps *steps, crv_batch_opts *opts, crv_pulses *pulses,
+                          crv_clusters *clusters, crv_timing *timing, crv_stream stream) {
+  if (int rc = guard(ctx)) return rc;
+  Ctx *c = &ctx->c;
+  if (!steps || !opts || !pulses || !clusters) return set_error(c, CRV_E_ARG, "NULL argument");
+  if (int rc = ready(c, true)) return rc;
+  c->kev_on = true;
+  struct Off {
+    Ctx *c;
+    ~Off() { c->kev_on = false; }
+  } off_{c};
+  c->last_digis = crv_digis{};
+  int rc = batched_stage(c, steps, opts, timing, (hipStream_t)stream);
+  if (rc && rc != CRV_E_CAPACITY) return rc;
+  int rf = crv_chain_fetch(ctx, pulses, clusters, stream);
+  return rf ? rf : rc;
 }
 
 int crv_chain_records(crv_ctx *ctx, const uint16_t **adc, const int64_t **label, int64_t *n) {

@@ -1,0 +1,1244 @@
+// The FP32 batched chain: steps -> pulses in three persistent / data-parallel kernels, then the coincidence finder.
+//
+//  kb_sipm  one lane per SiPM: the photon streams of the counter's steps (ordered exponential delays) -> pixel,
+//           recovery, firing, cross-talk cascade (reference response/sipm.py:141-198) -> the SiPM's fired-pixel log
+//           (time, overvoltage); no photon or charge table.
+//  kb_wave  one lane per SiPM: its fired pixels in time order -> charge clusters -> a 16-sample register window of
+//           template sums, noise, ADC words and zero suppression (reference reco/waveform.py:96-172) -> the ADC record
+//           of each cluster with digis (record pool) and its reco waveforms, the runs of continuing digis (run pool).
+//  kb_fit   one lane per reco waveform: peak finding and the Gumbel fit (reference reco/pulsefit.py:236-316).
+// The SiPM kernels take tiles of SiPMs from an atomic work queue (persistent grid), optionally longest first (photon
+// count), so that the lanes of a wave carry similar work. A run map (SiPM, run number) -> pool slot keeps every
+// output in (event, channel, time) order whatever order the lanes fill the pools in.
+//
+// Arithmetic is FP32 with the hardware exp / log / sin / cos; absolute times are FP64 (photon arrival = step time +
+// propagation, window starts, pulse times = TDC grid + fitted offset) and the 3 x 3 normal equations of the fit are
+// solved in FP64. The draws are the Philox streams of the FP64 chain (same keys and counters), so both chains
+// simulate the same photons up to rounding; their tables agree statistically, not bit for bit.
+#include <hipcub/hipcub.hpp>
+
+#include "crv_internal.h"
+
+namespace crv {
+
+namespace {
+
+constexpr int CMAXB = 16;   // cascade candidates per generation and lane
+constexpr int MAXGEN = 31;  // cascade generations
+constexpr int RINGB = 16;   // open samples of the waveform window
+constexpr int PSLOTS = 4;   // pulse slots per reco waveform (more: the fits run again with more)
+constexpr int DWORDS = 6;   // digi-start mask words: window_samples <= 192
+constexpr float PERIODF = 12.5f;
+constexpr double PERIODD = 12.5;
+constexpr int F_PSLOT = 7;  // a reco waveform produced more pulses than its slots
+
+__device__ __forceinline__ float u32f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * 0x1p-24f; }
+
+__device__ __forceinline__ float lgam_int_f(float x) {  // log((x - 1)!) for integer x >= 1
+  if (x < 10.f) {
+    float f = 1.f;
+    for (int i = 2; i < (int)x; ++i) f *= (float)i;
+    return __logf(f);
+  }
+  const float z = 1.f / x, z2 = z * z;
+  const float s = z * (0.0833333333f + z2 * (-0.00277777778f + z2 * (0.000793650794f + z2 * -0.000595238095f)));
+  return (x - 0.5f) * __logf(x) - x + 0.918938533f + s;
+}
+
+// Poisson(mu) on the KIND_COUNT draws of the FP64 chain: inversion below 10, PTRS (Hoermann 1993) above
+__device__ __forceinline__ int poisson_f(float mu, const Draw &dr, uint32_t ev, uint32_t ch, uint32_t j, int *capped) {
+  if (mu < 10.f) {
+    const float u = u32f(dr(KIND_COUNT, ev, ch, j, 0).x);
+    float p = __expf(-mu), cdf = p;
+    int k = 0;
+    while (u > cdf && k < 200) {
+      k += 1;
+      p = p * mu / (float)k;
+      cdf += p;
+    }
+    return k;
+  }
+  const float slam = sqrtf(mu), loglam = __logf(mu);
+  const float b = 0.931f + 2.53f * slam, a = -0.059f + 0.02483f * b;
+  const float invalpha = 1.1239f + 1.1328f / (b - 3.4f), vr = 0.9277f - 3.6224f / (b - 2.f);
+  U4 cache = dr(KIND_COUNT, ev, ch, j, 0);
+  for (int trial = 0; trial < 64; ++trial) {
+    if (trial && (trial & 1) == 0) cache = dr(KIND_COUNT, ev, ch, j, (uint32_t)(trial >> 1));
+    const float U = u32f((trial & 1) ? cache.z : cache.x) - 0.5f;
+    const float V = u32f((trial & 1) ? cache.w : cache.y);
+    const float us = 0.5f - fabsf(U);
+    const float kf = floorf((2.f * a / us + b) * U + mu + 0.43f);
+    if (us >= 0.07f && V <= vr) return (int)kf;
+    if (kf < 0.f || (us < 0.013f && V > us)) continue;
+    if (__logf(V * invalpha / (a / (us * us) + b)) <= -mu + kf * loglam - lgam_int_f(kf + 1.f)) return (int)kf;
+  }
+  atomicAdd(capped, 1);
+  return (int)rintf(mu);
+}
+
+struct BStep {  // the steps of the batch (device SoA)
+  const double *edep, *z, *t;
+};
+
+// photoelectrons per (step, SiPM), and per SiPM in total
+__global__ void kb_count(crv_params p, Tables t, Derived dv, BStep st, const int32_t *__restrict__ Tdev,
+                         const uint64_t *__restrict__ task_key, const int32_t *__restrict__ task_off,
+                         const int32_t *__restrict__ perm, const int32_t *__restrict__ evfirst,
+                         int32_t *__restrict__ nph, int64_t *__restrict__ sipm_nph, uint32_t *__restrict__ sort_key,
+                         int32_t *__restrict__ sort_idx, int *__restrict__ flags) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t L4 = (int64_t)(*Tdev) * 4;
+  if (g >= L4) return;
+  const int64_t tk = g >> 2;
+  const int s = (int)(g & 3);
+  const uint64_t key = task_key[tk];
+  const uint32_t ev = (uint32_t)(key >> 13);
+  const int ch = (int)(key & 8191u) * 4 + s;
+  const int a = task_off[tk], b = task_off[tk + 1];
+  const bool live = ch < t.n_channels && (t.status[ch] & 0b101) == 0;
+  const Draw dr{(uint32_t)p.seed, p.evkey_hi};
+  const float ys = live ? (float)(dv.yscale * t.dev[ch]) : 0.f;
+  const float H = (float)p.half_length;
+  const float ws_ = (float)p.w_short, wl = (float)p.w_long, ils = 1.f / (float)p.l_short, ill = 1.f / (float)p.l_long;
+  int64_t total = 0;
+  for (int k = a; k < b; ++k) {
+    int n = 0;
+    if (live) {
+      const int32_t i = perm[k];
+      const uint32_t j = (uint32_t)(i - evfirst[i]);
+      const float z = (float)st.z[i];
+      const float d = (s & 1) == 0 ? z + H : H - z;
+      const float att = ws_ * __expf(-d * ils) + wl * __expf(-d * ill);
+      const float mu = (float)st.edep[i] * ys * att;
+      n = mu > 0.f ? poisson_f(mu, dr, ev, (uint32_t)ch, j, &flags[F_CAPPED]) : 0;
+      if (n > 65535) {  // the photon rank is 16 bits of the Philox counter
+        atomicAdd(&flags[F_COUNTER], 1);
+        n = 65535;
+      }
+    }
+    nph[(int64_t)k * 4 + s] = n;
+    total += n;
+  }
+  sipm_nph[g] = total;
+  // longest first: descending photon count, ties in SiPM order (stable radix sort of the complement)
+  sort_key[g] = 0xFFFFFFFFu - (uint32_t)(total < 0xFFFFFFFFll ? total : 0xFFFFFFFFll);
+  sort_idx[g] = (int32_t)g;
+}
+
+__device__ __forceinline__ int adc_f(const crv_params &p, float v) {
+  float x = v * (float)p.adc_conversion + (float)p.pedestal + 0.5f;
+  x = fminf(fmaxf(x, -1.f), (float)p.max_adc + 1.f);
+  const float a = truncf(x);
+  return (int)fminf(fmaxf(a, (float)p.min_adc), (float)p.max_adc);
+}
+
+// ---- pulse fit: Gumbel A exp(-z - e^-z), z = (t - mu) / beta, in sine-mapped coordinates ----
+struct FSum {
+  float chi2, jtj[6], jtr[3];
+};
+
+__device__ __forceinline__ void fparams(const float lo[3], const float hi[3], const float u[3], float par[3],
+                                        float dpdu[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float span = hi[i] - lo[i];
+    float sn, cs;
+    __sincosf(u[i], &sn, &cs);
+    par[i] = lo[i] + span * (sn + 1.f) * 0.5f;
+    dpdu[i] = span * 0.5f * cs;
+  }
+}
+
+__device__ __forceinline__ void feval(const float *y, int64_t ys, int fs, int fe, const float lo[3],
+                                      const float hi[3], const float u[3], FSum &s) {
+  float par[3], dpdu[3];
+  fparams(lo, hi, u, par, dpdu);
+  const float A = par[0], mu = par[1], ib = 1.f / par[2];
+  s.chi2 = 0.f;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) s.jtj[i] = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) s.jtr[i] = 0.f;
+  for (int q = fs; q <= fe; ++q) {
+    const float z = ((float)q * PERIODF - mu) * ib;
+    const float ez = __expf(-z);
+    const float g = __expf(-z - ez);
+    const float fv = A * g;
+    const float r = y[q * ys] - fv;
+    const float dfdz = fv * (ez - 1.f);
+    const float J0 = g * dpdu[0], J1 = -dfdz * ib * dpdu[1], J2 = -dfdz * z * ib * dpdu[2];
+    s.chi2 += r * r;
+    s.jtr[0] += J0 * r;
+    s.jtr[1] += J1 * r;
+    s.jtr[2] += J2 * r;
+    s.jtj[0] += J0 * J0;
+    s.jtj[1] += J0 * J1;
+    s.jtj[2] += J0 * J2;
+    s.jtj[3] += J1 * J1;
+    s.jtj[4] += J1 * J2;
+    s.jtj[5] += J2 * J2;
+  }
+}
+
+// (S + diag(add)) x = b in FP64 with partial pivoting; false on a zero pivot
+__device__ __forceinline__ bool fsolve(const float jtj[6], const double add[3], const float bv[3], double x[3]) {
+  double a[3][3] = {{(double)jtj[0] + add[0], (double)jtj[1], (double)jtj[2]},
+                    {(double)jtj[1], (double)jtj[3] + add[1], (double)jtj[4]},
+                    {(double)jtj[2], (double)jtj[4], (double)jtj[5] + add[2]}};
+  double v[3] = {bv[0], bv[1], bv[2]};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    int piv = c;
+    double best = fabs(a[c][c]);
+#pragma unroll
+    for (int r = c + 1; r < 3; ++r)
+      if (fabs(a[r][c]) > best) {
+        piv = r;
+        best = fabs(a[r][c]);
+      }
+    if (piv != c) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double tmp = a[c][k];
+        a[c][k] = a[piv][k];
+        a[piv][k] = tmp;
+      }
+      const double tv = v[c];
+      v[c] = v[piv];
+      v[piv] = tv;
+    }
+    if (a[c][c] == 0.0) return false;
+#pragma unroll
+    for (int r = c + 1; r < 3; ++r) {
+      const double l = a[r][c] / a[c][c];
+#pragma unroll
+      for (int cc = c + 1; cc < 3; ++cc) a[r][cc] -= l * a[c][cc];
+      v[r] -= l * v[c];
+    }
+  }
+  x[2] = v[2] / a[2][2];
+  x[1] = (v[1] - a[1][2] * x[2]) / a[1][1];
+  x[0] = (v[0] - a[0][1] * x[1] - a[0][2] * x[2]) / a[0][0];
+  return true;
+}
+
+// extreme eigenvalues of the symmetric 3 x 3 normal matrix: six cyclic Jacobi sweeps
+__device__ __forceinline__ void feig(const float jtj[6], float &emin, float &emax) {
+  float a[3][3] = {{jtj[0], jtj[1], jtj[2]}, {jtj[1], jtj[3], jtj[4]}, {jtj[2], jtj[4], jtj[5]}};
+  const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+  for (int sweep = 0; sweep < 6; ++sweep) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int p = P[k], q = Q[k], r = 3 - p - q;
+      const float apq = a[p][q];
+      if (apq == 0.f) continue;
+      const float th = (a[q][q] - a[p][p]) / (2.f * apq);
+      const float tt = (th >= 0.f ? 1.f : -1.f) / (fabsf(th) + sqrtf(th * th + 1.f));
+      const float c = 1.f / sqrtf(tt * tt + 1.f), s = tt * c;
+      const float arp = a[r][p], arq = a[r][q];
+      a[p][p] -= tt * apq;
+      a[q][q] += tt * apq;
+      a[p][q] = a[q][p] = 0.f;
+      a[r][p] = a[p][r] = c * arp - s * arq;
+      a[r][q] = a[q][r] = s * arp + c * arq;
+    }
+  }
+  emin = fminf(fminf(a[0][0], a[1][1]), a[2][2]);
+  emax = fmaxf(fmaxf(a[0][0], a[1][1]), a[2][2]);
+}
+
+
+struct Out {
+  // fired-pixel log per SiPM (offsets loff)
+  const int64_t *loff;
+  double *log_t;
+  float *log_v;
+  int16_t *log_pix;
+  // photon streams of the (step, SiPM) pairs of counters with several steps
+  float *gS;
+  double *gHead, *gTb;
+  int32_t *gR;
+  // cascade lists per persistent lane [slot][grid lane]
+  uint16_t *xcur, *xcand;
+  uint8_t *xmult;
+  int64_t xstride;
+  // record pool: window_samples ADC words, label and FEB tick of a cluster with digis (label -1: unused slot)
+  uint16_t *rec_adc;
+  int64_t *rec_label;
+  int32_t *rec_tick;
+  int32_t *rec_count;
+  int64_t rec_cap;
+  // run pool: the reco waveforms (record, first sample, samples, SiPM, run number in the SiPM, start TDC)
+  int32_t *run_rec, *run_i0, *run_n, *run_k;
+  int64_t *run_g, *run_tdc;
+  int32_t *run_count;
+  int64_t run_cap;
+  int32_t *nrun;  // per SiPM
+  // pulse slots per pool run, and per final run the pulse count and the pool run
+  int pcap;
+  double *tp, *mu;  // pulse time without the DB offset, fitted Gumbel location (absolute, ns)
+  float *pe, *height, *beta, *chi2, *ped, *fa, *fb;
+  uint32_t *sf;  // seq | failed << 16 | zerondf << 17 | pedfromdb << 18
+  int32_t *npr, *rmap;
+  float *yscr;  // fit samples [sample][pool run]
+};
+
+// work queue of a persistent grid: a workgroup takes `tile` SiPMs (in `order`) per grab
+struct Queue {
+  int *ctr;
+  int tile;
+  int64_t n;
+  const int32_t *order;
+};
+
+__device__ __forceinline__ bool grab(const Queue &qu, int *tile_base, int64_t &base) {
+  if (threadIdx.x == 0) *tile_base = atomicAdd(qu.ctr, qu.tile);
+  __syncthreads();
+  base = *tile_base;
+  __syncthreads();
+  return base < qu.n;
+}
+
+__device__ __forceinline__ bool bget(const uint32_t *bm, int lane, int pix) {
+  return (bm[(pix >> 5) * 64 + lane] >> (pix & 31)) & 1u;
+}
+__device__ __forceinline__ void bset(uint32_t *bm, int lane, int pix) { bm[(pix >> 5) * 64 + lane] |= 1u << (pix & 31); }
+
+// overvoltage of a pixel at time tph: from its last firing in the lane's log (the bitmap says whether there is one)
+__device__ __forceinline__ float pixel_v(const uint32_t *bm, int lane, int pix, const Out &o, int64_t lb, int64_t nlog,
+                                         double tph, float Vov, float inv_tau) {
+  if (!bget(bm, lane, pix)) return Vov;
+  for (int64_t q = nlog - 1; q >= 0; --q)
+    if (o.log_pix[lb + q] == pix) return Vov * (1.f - __expf(-(float)(tph - o.log_t[lb + q]) * inv_tau));
+  return Vov;
+}
+// kb_sipm, one lane per SiPM: photons of the counter's steps in time order -> pixel, recovery, firing, cross-talk
+// cascade -> the SiPM's fired-pixel log (time, overvoltage, pixel), in firing order.
+__global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv, BStep st,
+                                              const uint64_t *__restrict__ task_key,
+                                              const int32_t *__restrict__ task_off, const int32_t *__restrict__ perm,
+                                              const int32_t *__restrict__ evfirst, const int32_t *__restrict__ nph,
+                                              Out o, Queue qu, int32_t *__restrict__ nfired, int *__restrict__ flags) {
+  __shared__ uint32_t bm[CRV_DEAD_WORDS * 64];
+  __shared__ int tile_base;
+  const int lane = threadIdx.x;
+  const float Vov = (float)p.overvoltage, p1 = (float)p.p1;
+  const float inv_tau = 1.f / (float)p.time_constant, inv_p2 = 1.f / (float)p.p2;
+  const float pf_full = p1 * (1.f - __expf(-Vov * inv_p2));
+  const float qct = (float)dv.q_ct, tauf = (float)p.tau;
+  const Draw dr{(uint32_t)p.seed, p.evkey_hi};
+  const int64_t xb = (int64_t)blockIdx.x * 64 + lane;
+  uint16_t *xcur = o.xcur + xb, *xcand = o.xcand + xb;
+  uint8_t *xmult = o.xmult + xb;
+  const int64_t xs = o.xstride;
+  int64_t base = 0;
+  while (grab(qu, &tile_base, base)) {
+    for (int64_t gq = base + lane; gq < base + qu.tile && gq < qu.n; gq += 64) {
+      const int64_t g = qu.order ? (int64_t)qu.order[gq] : gq;
+      const int64_t tk = g >> 2;
+      const int s = (int)(g & 3);
+      const uint64_t key = task_key[tk];
+      const uint32_t ev = (uint32_t)(key >> 13);
+      const int ch = (int)(key & 8191u) * 4 + s;
+      if (ch >= t.n_channels || (t.status[ch] & 0b101) != 0) {
+        nfired[g] = 0;
+        continue;
+      }
+      const int a = task_off[tk], b = task_off[tk + 1];
+      const int nstr = b - a;
+      for (int w = 0; w < CRV_DEAD_WORDS; ++w) bm[w * 64 + lane] = 0u;
+      const int64_t lb = o.loff[g], lcap = o.loff[g + 1] - lb;
+      int64_t nlog = 0;
+      // photon streams: in registers for a counter with one step, in global memory otherwise
+      float S1 = 0.f;
+      double head1 = 0.0, tb1 = 0.0;
+      int r1 = 0, N1 = 0;
+      uint32_t j1 = 0;
+      for (int k = a; k < b; ++k) {
+        const int64_t gi = (int64_t)k * 4 + s;
+        const int N = nph[gi];
+        if (nstr > 1) o.gR[gi] = 0;
+        if (N <= 0) continue;
+        const int32_t i = perm[k];
+        const uint32_t j = (uint32_t)(i - evfirst[i]);
+        const double z = st.z[i];
+        const double d = (s & 1) == 0 ? z + p.half_length : p.half_length - z;
+        const double tb = st.t[i] + d * p.inv_speed;
+        const float S = -__logf(u32f(dr(KIND_DELAY, ev, (uint32_t)ch, j, 0u).x)) / (float)N;
+        if (nstr == 1) {
+          N1 = N;
+          j1 = j;
+          tb1 = tb;
+          S1 = S;
+          head1 = tb + (double)(tauf * S);
+        } else {
+          o.gS[gi] = S;
+          o.gTb[gi] = tb;
+          o.gHead[gi] = tb + (double)(tauf * S);
+        }
+      }
+      bool bad = false;
+      for (;;) {
+        // next photon: the earliest stream head (lowest step first on ties)
+        int kb = -1;
+        double tph = 0.0;
+        if (nstr == 1) {
+          if (r1 < N1) {
+            kb = a;
+            tph = head1;
+          }
+        } else {
+          for (int k = a; k < b; ++k) {
+            const int64_t gi = (int64_t)k * 4 + s;
+            if (o.gR[gi] >= nph[gi]) continue;
+            const double h = o.gHead[gi];
+            if (kb < 0 || h < tph) {
+              kb = k;
+              tph = h;
+            }
+          }
+        }
+        if (kb < 0 || tph > p.photon_window_hi) break;  // later photons of every stream are later still
+        // advance the stream
+        uint32_t j;
+        int r;
+        if (nstr == 1) {
+          j = j1;
+          r = r1;
+          if (r1 + 1 < N1) {
+            const int rn = r1 + 1;
+            const U4 uu = dr(KIND_DELAY, ev, (uint32_t)ch, j1, (uint32_t)(rn >> 2));
+            const uint32_t x = (rn & 3) == 0 ? uu.x : (rn & 3) == 1 ? uu.y : (rn & 3) == 2 ? uu.z : uu.w;
+            S1 += (-__logf(u32f(x))) / (float)(N1 - rn);
+            head1 = tb1 + (double)(tauf * S1);
+          }
+          r1 += 1;
+        } else {
+          const int64_t gi = (int64_t)kb * 4 + s;
+          const int32_t row = perm[kb];
+          j = (uint32_t)(row - evfirst[row]);
+          r = o.gR[gi];
+          const int N = nph[gi];
+          if (r + 1 < N) {
+            const int rn = r + 1;
+            const U4 uu = dr(KIND_DELAY, ev, (uint32_t)ch, j, (uint32_t)(rn >> 2));
+            const uint32_t x = (rn & 3) == 0 ? uu.x : (rn & 3) == 1 ? uu.y : (rn & 3) == 2 ? uu.z : uu.w;
+            const float S = o.gS[gi] + (-__logf(u32f(x))) / (float)(N - rn);
+            o.gS[gi] = S;
+            o.gHead[gi] = o.gTb[gi] + (double)(tauf * S);
+          }
+          o.gR[gi] = r + 1;
+        }
+        // pixel of the photon (photon-map CDF), firing with the recovered overvoltage
+        const U4 u = dr(KIND_PIXEL, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, 0u, 0u));
+        const double xc = unif(u.x) * t.cdf_total;
+        int bin;
+        if (t.map_uniform) {
+          bin = min((int)xc, t.ncdf - 1);
+        } else {
+          int lo = 0, hi = t.ncdf;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (t.cdf[mid] <= xc) lo = mid + 1;
+            else hi = mid;
+          }
+          bin = min(lo, t.ncdf - 1);
+        }
+        int px = bin / t.map_ny, py = bin % t.map_ny;
+        if (p.pixel_lrint) {
+          px += u32f(u.y) >= 0.5f ? 1 : 0;
+          py += u32f(u.z) >= 0.5f ? 1 : 0;
+        }
+        const int pix = px * CRV_NPIX + py;
+        if ((t.dead[pix >> 5] >> (pix & 31)) & 1u) continue;
+        const float v = pixel_v(bm, lane, pix, o, lb, nlog, tph, Vov, inv_tau);
+        const float pf = v == Vov ? pf_full : p1 * (1.f - __expf(-v * inv_p2));
+        if (!(u32f(u.w) < pf)) continue;
+        if (nlog >= lcap) {
+          bad = true;
+          break;
+        }
+        o.log_t[lb + nlog] = tph;
+        o.log_v[lb + nlog] = v;
+        o.log_pix[lb + nlog] = (int16_t)pix;
+        ++nlog;
+        bset(bm, lane, pix);
+        // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order
+        int ncur = 1, gnum = 0;
+        xcur[0] = (uint16_t)pix;
+        while (ncur > 0 && !bad) {
+          int ncand = 0;
+          for (int ai = 0; ai < ncur; ++ai) {
+            const int f = xcur[ai * xs];
+            const int fx = f / CRV_NPIX, fy = f % CRV_NPIX;
+            const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)f));
+            const uint32_t uw[4] = {ux.x, ux.y, ux.z, ux.w};
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+              const int nx = nb == 0 ? fx - 1 : nb == 1 ? fx + 1 : fx;
+              const int ny = nb == 2 ? fy - 1 : nb == 3 ? fy + 1 : fy;
+              const bool ok =
+                  nb == 0 ? fx > 0 : nb == 1 ? fx + 1 < p.n_pixels_x : nb == 2 ? fy > 0 : fy + 1 < p.n_pixels_y;
+              if (!ok || !(u32f(uw[nb]) < qct)) continue;
+              const int c = nx * CRV_NPIX + ny;
+              int at = 0;
+              while (at < ncand && xcand[at * xs] < c) ++at;
+              if (at < ncand && xcand[at * xs] == c) {
+                xmult[at * xs] += 1;
+              } else if (ncand < CMAXB) {
+                for (int q = ncand; q > at; --q) {
+                  xcand[q * xs] = xcand[(q - 1) * xs];
+                  xmult[q * xs] = xmult[(q - 1) * xs];
+                }
+                xcand[at * xs] = (uint16_t)c;
+                xmult[at * xs] = 1;
+                ++ncand;
+              } else {
+                atomicAdd(&flags[F_CASCADE], 1);
+                bad = true;
+              }
+            }
+          }
+          if (ncand == 0 || bad) break;
+          if (++gnum > MAXGEN) {
+            atomicAdd(&flags[F_CASCADE], 1);
+            bad = true;
+            break;
+          }
+          ncur = 0;
+          for (int ai = 0; ai < ncand; ++ai) {
+            const int c = xcand[ai * xs];
+            const int m = xmult[ai * xs];
+            if ((t.dead[c >> 5] >> (c & 31)) & 1u) continue;
+            const float vc = pixel_v(bm, lane, c, o, lb, nlog, tph, Vov, inv_tau);
+            const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
+            float qm = q1;
+            for (int mm = 2; mm <= m; ++mm) qm *= q1;
+            const float uc =
+                u32f(dr(KIND_XTFIRE, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)c)).x);
+            if (!(uc < 1.f - qm)) continue;
+            if (nlog >= lcap) {
+              bad = true;
+              break;
+            }
+            o.log_t[lb + nlog] = tph;
+            o.log_v[lb + nlog] = vc;
+            o.log_pix[lb + nlog] = (int16_t)c;
+            ++nlog;
+            bset(bm, lane, c);
+            xcur[ncur * xs] = (uint16_t)c;
+            ++ncur;
+          }
+        }
+        if (bad) break;
+      }
+      if (bad) atomicAdd(&flags[F_OVERFLOW], 1);  // the batch is run again with larger logs
+      nfired[g] = (int32_t)nlog;
+    }
+  }
+}
+
+
+// ---- waveform machine (kb_wave) ----
+struct Wave {
+  float win[RINGB];        // open samples E .. E + RINGB - 1
+  U4 nz;                   // noise draw of the sample pair holding E - 1
+  double t_start, t_first, t_last;
+  int32_t kf, nq, used, E, next_i, nd;
+  uint32_t above;          // bit k: sample E - 1 - k above threshold
+  uint32_t dmask[DWORDS];  // bit i: a digi starts at sample i
+  int32_t rec;             // reserved record slot (-1: none)
+  bool open;
+  int32_t overlong;
+};
+
+struct Lane {
+  uint32_t ev;
+  int ch;
+  int64_t evkey;
+  double td;  // TDC0 of the SiPM's FEB
+  int64_t task;
+  int nrun;   // reco waveforms of the SiPM so far
+  int W;
+  const float *tmpl;
+};
+
+// finalise sample E: noise, ADC word, zero-suppression decision for sample E - 2
+__device__ __forceinline__ void wave_step(const crv_params &p, const Lane &L, const Out &o, Wave &w, bool at_close,
+                                          int length) {
+  const int J = w.E++;
+  float v = w.win[0];
+#pragma unroll
+  for (int k = 0; k < RINGB - 1; ++k) w.win[k] = w.win[k + 1];
+  w.win[RINGB - 1] = 0.f;
+  const bool valid = !at_close || J < length;
+  if (valid && p.noise_on) {
+    if ((J & 1) == 0)
+      w.nz = Draw{(uint32_t)p.seed, p.evkey_hi}(KIND_NOISE, L.ev, (uint32_t)L.ch, (uint32_t)w.kf, (uint32_t)(J >> 1));
+    const float u1 = (J & 1) ? u32f(w.nz.z) : u32f(w.nz.x), u2 = (J & 1) ? u32f(w.nz.w) : u32f(w.nz.y);
+    v += (float)p.noise * sqrtf(-2.f * __logf(u1)) * __cosf(6.28318531f * u2);
+  }
+  if (J < L.W && w.rec >= 0) o.rec_adc[(int64_t)w.rec * L.W + J] = (uint16_t)adc_f(p, valid ? v : 0.f);
+  const bool ab = valid && v > (float)p.min_voltage;
+  w.above = (w.above << 1) | (ab ? 1u : 0u);
+  const int i = J - 2;  // zero suppression of sample i needs samples i - 2 .. i + 2
+  const bool i_valid = i >= 0 && (!at_close || i < length);
+  if (i_valid && (w.above & 0x1Fu) != 0u && w.next_i <= i) {
+    const double tstart = w.t_start + (double)i * PERIODD;
+    if (tstart >= p.window_lo && tstart <= p.window_hi) {  // i < length <= W - samples_zs < 32 * DWORDS
+#pragma unroll
+      for (int q = 0; q < DWORDS; ++q)
+        if (q == (i >> 5)) w.dmask[q] |= 1u << (i & 31);
+      ++w.nd;
+      w.next_i = i + p.samples_zs;
+    }
+  }
+}
+
+__device__ __forceinline__ void wave_open(const Lane &L, const Out &o, Wave &w, double t) {
+  w.open = true;
+  w.t_first = t;
+  w.t_last = t;
+  w.kf = (int32_t)ceil((t - PERIODD - L.td) / PERIODD);
+  w.t_start = (double)w.kf * PERIODD + L.td;
+  w.nq = 0;
+  w.used = 0;
+  w.E = 0;
+  w.above = 0u;
+  w.next_i = 0;
+  w.nd = 0;
+#pragma unroll
+  for (int k = 0; k < RINGB; ++k) w.win[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < DWORDS; ++k) w.dmask[k] = 0u;
+  if (w.rec < 0) {  // a record slot; a cluster without digis leaves it to the next cluster
+    const int r = atomicAdd(o.rec_count, 1);
+    w.rec = r < o.rec_cap ? r : -1;
+  }
+}
+
+// nf charges at time tq with summed overvoltage vsum (a fired pixel and its cross-talk): template sums
+__device__ __forceinline__ void wave_add(const crv_params &p, const Tables &t, const Lane &L, const Out &o, Wave &w,
+                                         double tq, float vsum, int nf, int nreach) {
+  const int lmax = L.W - p.samples_zs;
+  w.nq += nf;
+  w.t_last = tq;
+  const float dt = (float)(tq - w.t_start);
+  const int j0 = (int)fmaxf(ceilf(dt / PERIODF), 0.f);
+  while (w.E < j0 && w.E < lmax) wave_step(p, L, o, w, false, 0);
+  if (j0 >= lmax) {  // every sample the charges reach is past the window
+    w.overlong = 1;
+    return;
+  }
+  // here E == j0: j0 never decreases within a cluster and E only advances to it
+  const float q = (float)p.capacitance * vsum / (float)p.single_pe_reference_charge;
+  const float inv_prec = 1.f / (float)p.single_pe_precision;
+  const int nt = t.ntmpl;
+#pragma unroll
+  for (int r = 0; r < RINGB; ++r) {
+    if (r < nreach) {
+      const int J = j0 + r;
+      const float stt = (float)J * PERIODF - dt;
+      const float ti = rintf(stt * inv_prec);
+      if (stt >= 0.f && ti < (float)nt) {
+        if (J >= lmax) {
+          w.overlong = 1;
+        } else {
+          w.win[r] += L.tmpl[(int)ti] * q;
+          w.used = max(w.used, J + 1);
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t dword_at(const uint32_t m[DWORDS], int q) {
+  uint32_t v = 0u;
+#pragma unroll
+  for (int k = 0; k < DWORDS; ++k) v = q == k ? m[k] : v;
+  return v;
+}
+
+// first set bit >= pos of the digi mask, or -1
+__device__ __forceinline__ int next_digi(const uint32_t m[DWORDS], int pos) {
+  for (int q = pos >> 5; q < DWORDS; ++q) {
+    uint32_t word = dword_at(m, q);
+    if (q == (pos >> 5)) word &= ~0u << (pos & 31);
+    if (word) return q * 32 + __ffs(word) - 1;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ bool has_digi(const uint32_t m[DWORDS], int i) {
+  return i < 32 * DWORDS && ((dword_at(m, i >> 5) >> (i & 31)) & 1u);
+}
+
+__device__ __forceinline__ void wave_close(const crv_params &p, const Tables &t, Lane &L, const Out &o, Wave &w,
+                                           int *flags) {
+  const int W = L.W, Sz = p.samples_zs, lmax = W - Sz;
+  const bool keep = (float)w.nq * (float)t.vmax >= (float)p.min_voltage;
+  w.open = false;
+  if (!keep) return;
+  const int est = (int)floor((w.t_last - w.t_first + p.single_pe_max_time) / PERIODD);
+  int length = est > w.used ? est : w.used;
+  if (length > lmax) {
+    length = lmax;
+    w.overlong = 1;
+  }
+  while (w.E < length + 2) wave_step(p, L, o, w, true, length);
+  if (w.nd == 0 || w.rec < 0) return;  // no digis (the slot stays reserved), or the record pool is full
+  const int pedw = adc_f(p, 0.f);
+  for (int J = w.E; J < W; ++J) o.rec_adc[(int64_t)w.rec * W + J] = (uint16_t)pedw;
+  o.rec_label[w.rec] = L.evkey * CRV_CHAN_MULT + L.ch;
+  o.rec_tick[w.rec] = w.kf;
+  // reco waveforms: runs of continuing digis (start TDC = window tick + first sample), in start order
+  const int64_t tdc_base = (int64_t)llrint((w.t_start - L.td) / PERIODD);
+  int pos = 0;
+  for (;;) {
+    const int i0 = next_digi(w.dmask, pos);
+    if (i0 < 0) break;
+    int last = i0;
+    while (has_digi(w.dmask, last + Sz)) last += Sz;
+    pos = last + 1;
+    const int r = atomicAdd(o.run_count, 1);
+    if (r < o.run_cap) {
+      o.run_rec[r] = w.rec;
+      o.run_i0[r] = i0;
+      o.run_n[r] = last + Sz - i0;  // <= W: every digi starts below W - Sz
+      o.run_k[r] = L.nrun;
+      o.run_g[r] = L.task;
+      o.run_tdc[r] = tdc_base + i0;
+    }
+    ++L.nrun;
+  }
+  w.rec = -1;
+}
+
+// kb_wave, one lane per SiPM: its fired pixels in time order -> charge clusters -> waveform window (template sums,
+// noise, ADC, zero suppression) -> ADC records and reco waveforms of the clusters with digis.
+__global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv, const uint64_t *__restrict__ task_key,
+                                              const int32_t *__restrict__ nfired, Out o, Queue qu,
+                                              int *__restrict__ flags) {
+  __shared__ int tile_base;
+  const int lane = threadIdx.x;
+  const double gap = p.single_pe_max_time + 4.0 * PERIODD;
+  const Draw dr{(uint32_t)p.seed, p.evkey_hi};
+  Lane L;
+  L.W = p.window_samples;
+  L.tmpl = t.tmpl_f;
+  int64_t base = 0;
+  while (grab(qu, &tile_base, base)) {
+    for (int64_t gq = base + lane; gq < base + qu.tile && gq < qu.n; gq += 64) {
+      const int64_t g = qu.order ? (int64_t)qu.order[gq] : gq;
+      const uint64_t key = task_key[g >> 2];
+      L.ev = (uint32_t)(key >> 13);
+      L.evkey = ((int64_t)p.evkey_hi << 32) | (int64_t)L.ev;
+      L.ch = (int)(key & 8191u) * 4 + (int)(g & 3);
+      L.task = g;
+      L.nrun = 0;
+      const int nf = nfired[g];
+      const int64_t lb = o.loff[g];
+      L.td = 0.0;
+      if (nf > 0) {
+        const uint32_t fk = p.feb_key_roc ? (uint32_t)(t.roc[L.ch] * 1000 + t.feb[L.ch]) : (uint32_t)t.feb[L.ch];
+        L.td = (double)u32f(dr(KIND_TDC0, L.ev, fk, 0u, 0u).x) * p.feb_phase_width;
+      }
+      Wave wv;
+      wv.open = false;
+      wv.overlong = 0;
+      wv.rec = -1;
+      int q = 0;
+      while (q < nf) {
+        // the charges of one time (a fired pixel and its cross-talk)
+        const double tq = o.log_t[lb + q];
+        float vsum = o.log_v[lb + q];
+        int n = 1;
+        while (q + n < nf && o.log_t[lb + q + n] == tq) {
+          vsum += o.log_v[lb + q + n];
+          ++n;
+        }
+        q += n;
+        if (wv.open && tq - wv.t_last > gap) wave_close(p, t, L, o, wv, flags);
+        if (!wv.open) wave_open(L, o, wv, tq);
+        wave_add(p, t, L, o, wv, tq, vsum, n, dv.nreach);
+      }
+      if (wv.open) wave_close(p, t, L, o, wv, flags);
+      if (wv.rec >= 0) o.rec_label[wv.rec] = -1;  // reserved, never used
+      if (wv.overlong) atomicAdd(&flags[F_OVERLONG], 1);
+      o.nrun[g] = L.nrun;
+    }
+  }
+}
+
+// kb_fit, one lane per reco waveform of the run pool: pedestal, peaks, Gumbel fits, pulses -> the run's slots; the
+// run's place in the output (SiPM's first run + run number) gets its pulse count and pool slot.
+__global__ __launch_bounds__(64) void kb_fit(crv_params p, Tables t, const uint64_t *__restrict__ task_key,
+                                             const int64_t *__restrict__ roff, Out o, int64_t nruns,
+                                             int *__restrict__ flags) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  const int64_t g = o.run_g[r];
+  const int ch = (int)(task_key[g >> 2] & 8191u) * 4 + (int)(g & 3);
+  const int n = o.run_n[r];
+  const int64_t st = o.run_tdc[r];
+  const int W = p.window_samples;
+  const uint16_t *src = o.rec_adc + (int64_t)o.run_rec[r] * W + o.run_i0[r];
+  float *y = o.yscr + r;
+  const int64_t ys = o.run_cap;
+  for (int q = 0; q < n; ++q) y[q * ys] = (float)src[q];
+  const bool drop = p.ignore_channels && (t.status[ch] & 0b010) != 0;
+  const float ped_db = (float)t.pedestal[ch];
+  const bool und = ped_db - y[0] > (float)p.pedestal_undershoot_threshold;
+  const float ped = und ? y[0] : ped_db;
+  for (int i = 0; i < n; ++i) y[i * ys] -= ped;
+  int np = 0;
+  int startb = 0;
+  bool failed_before = false;
+  for (int seq = 0;; ++seq) {
+    bool found = false, rising = false;
+    int ps = 0, pe = 0;
+    for (int b = startb + 1; b < n; ++b) {
+      const float prev = y[(b - 1) * ys], cur = y[b * ys];
+      if (prev < cur) {
+        rising = true;
+        ps = pe = b;
+      } else if (prev == cur && rising) {
+        pe = b;
+      } else if (prev > cur && rising) {
+        if (y[pe * ys] > (float)p.min_adc_difference) {
+          found = true;
+          break;
+        }
+        rising = false;
+      }
+    }
+    if (!found) break;
+    int fs = ps - 1, fe = pe + 1;
+    bool go = true;
+    for (int k = 1; k <= p.samples_before && go; ++k) {
+      const int i = ps - k;
+      const bool cond = i >= 0 && y[i * ys] <= y[(i + 1) * ys];
+      if (cond) fs = i;
+      go = cond && i != 0;
+    }
+    go = true;
+    for (int k = 1; k <= p.samples_after && go; ++k) {
+      const int i = pe + k;
+      const bool cond = i < n && y[(i - 1) * ys] >= y[i * ys];
+      if (cond) fe = i;
+      go = cond;
+    }
+    if (ps - fs > 1) fs += 1;
+    if (fe - pe > 1) fe -= 1;
+    const float yps = y[ps * ys];
+    const float t_ps = (float)ps * PERIODF, t_pe = (float)pe * PERIODF;  // relative to the waveform start
+    const float t_peak = 0.5f * (t_ps + t_pe);
+    const float ye = yps * 2.71828183f;
+    const float lo[3] = {ye * (float)p.min_pulse_height_ratio, t_ps - (float)p.max_time_difference, (float)p.min_beta};
+    const float hi[3] = {ye * (float)p.max_pulse_height_ratio, t_pe + (float)p.max_time_difference, (float)p.max_beta};
+    const float p0[3] = {ye, t_peak, (float)p.default_beta};
+    float u[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) u[i] = asinf(fminf(fmaxf(2.f * (p0[i] - lo[i]) / (hi[i] - lo[i]) - 1.f, -1.f), 1.f));
+    float lam = 1e-3f;
+    FSum s, sn;
+    feval(y, ys, fs, fe, lo, hi, u, s);
+    for (int it = 0; it < p.lm_iterations; ++it) {
+      const double add[3] = {(double)lam * ((double)s.jtj[0] + 1e-12), (double)lam * ((double)s.jtj[3] + 1e-12),
+                             (double)lam * ((double)s.jtj[5] + 1e-12)};
+      double du[3];
+      if (!fsolve(s.jtj, add, s.jtr, du)) du[0] = du[1] = du[2] = 0.0;
+      const float un[3] = {u[0] + (float)du[0], u[1] + (float)du[1], u[2] + (float)du[2]};
+      feval(y, ys, fs, fe, lo, hi, un, sn);
+      const bool better = sn.chi2 < s.chi2;
+      if (better) {
+        u[0] = un[0];
+        u[1] = un[1];
+        u[2] = un[2];
+        s = sn;
+      }
+      lam = fminf(fmaxf(better ? lam * 0.3f : lam * 10.f, 1e-9f), 1e9f);
+    }
+    const double tiny[3] = {1e-12, 1e-12, 1e-12};
+    for (int it = 0; it < 8; ++it) {
+      double du[3];
+      if (!fsolve(s.jtj, tiny, s.jtr, du)) break;
+      bool moved = false;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        du[i] = isfinite(du[i]) ? du[i] : 0.0;
+        moved |= du[i] != 0.0;
+      }
+      const float sc[4] = {1.f, 0.5f, 0.25f, 0.125f};
+      for (int k = 0; k < 4 && moved; ++k) {
+        const float un[3] = {u[0] + sc[k] * (float)du[0], u[1] + sc[k] * (float)du[1], u[2] + sc[k] * (float)du[2]};
+        feval(y, ys, fs, fe, lo, hi, un, sn);
+        if (sn.chi2 < s.chi2) {
+          u[0] = un[0];
+          u[1] = un[1];
+          u[2] = un[2];
+          s = sn;
+          moved = false;
+        }
+      }
+    }
+    double step[3];
+    const bool ok = fsolve(s.jtj, tiny, s.jtr, step);
+    const double edm = 0.5 * ((double)s.jtr[0] * step[0] + (double)s.jtr[1] * step[1] + (double)s.jtr[2] * step[2]);
+    float emin, emax;
+    feig(s.jtj, emin, emax);
+    const bool hess_ok = ok && emin > 1e-12f * fmaxf(emax, 1e-30f);
+    const bool conv = ok && isfinite(edm) && edm <= 1e-3 * fmax((double)s.chi2, 1.0);
+    float par[3], dp[3];
+    fparams(lo, hi, u, par, dp);
+    bool near = false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float span = hi[i] - lo[i];
+      near |= (par[i] - lo[i]) / span < (float)p.limit_tolerance || (hi[i] - par[i]) / span < (float)p.limit_tolerance;
+    }
+    const bool failed = !conv || !hess_ok || near;
+    const float A = par[0], mu = par[1], bb = par[2];
+    const int ndf = (fe - fs + 1) - 3;
+    const bool fseq = failed || failed_before;
+    if (!drop) {
+      if (np < o.pcap) {
+        const int64_t q = r * o.pcap + np;
+        const double t0 = (double)st * PERIODD;
+        o.tp[q] = t0 + (double)(failed ? t_peak : mu);
+        o.mu[q] = t0 + (double)mu;
+        o.pe[q] = (failed ? ye * (float)p.default_beta : A * bb) / (float)t.area[ch];
+        o.height[q] = failed ? yps : A / 2.71828183f;
+        o.beta[q] = failed ? (float)p.default_beta : bb;
+        o.chi2[q] = (failed || ndf <= 0) ? -1.f : s.chi2 / (float)max(ndf, 1);
+        o.ped[q] = ped;
+        o.fa[q] = A;
+        o.fb[q] = bb;
+        o.sf[q] = (uint32_t)seq | (fseq ? 1u << 16 : 0u) | (ndf <= 0 ? 1u << 17 : 0u) | (!und ? 1u << 18 : 0u);
+      }
+      ++np;
+    }
+    failed_before = fseq;
+    const float ib = 1.f / bb;
+    for (int i = 0; i < n; ++i) {
+      const float z = ((float)i * PERIODF - mu) * ib;
+      const float sub = A * __expf(-z - __expf(-z));
+      y[i * ys] -= sub >= 0.f ? floorf(sub + 0.5f) : -floorf(-sub + 0.5f);
+    }
+    startb = fe;
+  }
+  if (np > o.pcap) atomicAdd(&flags[F_PSLOT], 1);
+  const int64_t fr = roff[g] + o.run_k[r];
+  o.npr[fr] = min(np, o.pcap);
+  o.rmap[fr] = (int32_t)r;
+}
+
+// fired-pixel log capacity per SiPM: photons + a cross-talk margin
+__global__ void kb_caps(const int64_t *__restrict__ nph, const int32_t *__restrict__ Tdev, int64_t *__restrict__ cap,
+                        int64_t n, int factor) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int64_t v = nph[g];
+  cap[g] = g < (int64_t)(*Tdev) * 4 ? v + v * factor / 4 + 8 : 0;
+}
+
+__global__ void kb_zero(int *__restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void kb_widen(const int32_t *__restrict__ in, int64_t *__restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+// the pulse table in run order (= (event, channel, time) order), one lane per run
+__global__ void kb_pulses_out(crv_params p, Tables t, int64_t nruns, const uint64_t *__restrict__ task_key,
+                              const int64_t *__restrict__ poff, Out o, crv_pulses P, int32_t *__restrict__ pbar,
+                              int32_t *__restrict__ psipm) {
+  const int64_t fr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (fr >= nruns) return;
+  const int64_t dst = poff[fr], n = poff[fr + 1] - dst;
+  if (n == 0) return;
+  const int64_t r = o.rmap[fr];
+  const int64_t g = o.run_g[r];
+  const uint64_t key = task_key[g >> 2];
+  const int ch = (int)(key & 8191u) * 4 + (int)(g & 3);
+  const int64_t evkey = ((int64_t)p.evkey_hi << 32) | (int64_t)(key >> 13);
+  const double toff = p.use_time_offset_db ? t.toff[ch] * p.time_offset_scale : 0.0;
+  const double hdb = t.height[ch];
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t a = r * o.pcap + k, b = dst + k;
+    const uint32_t sf = o.sf[a];
+    const double tp = o.tp[a], beta = o.beta[a], height = o.height[a];
+    P.wf[b] = fr;
+    P.seq[b] = sf & 0xFFFFu;
+    P.ev[b] = evkey;
+    P.chan[b] = ch;
+    P.pe[b] = o.pe[a];
+    P.peph[b] = height / hdb;
+    P.time[b] = tp + toff;
+    P.height[b] = height;
+    P.beta[b] = beta;
+    P.chi2[b] = o.chi2[a];
+    P.letime[b] = tp - p.le_time_factor * beta;
+    P.ped[b] = o.ped[a];
+    P.fit_a[b] = o.fa[a];
+    P.fit_mu[b] = o.mu[a];
+    P.fit_b[b] = o.fb[a];
+    P.failed[b] = (sf >> 16) & 1u;
+    P.zerondf[b] = (sf >> 17) & 1u;
+    P.pedfromdb[b] = (sf >> 18) & 1u;
+    pbar[b] = ch / 4;
+    psipm[b] = ch % 4;
+  }
+}
+
+int resident_blocks(Ctx *c, const void *kernel) {
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
+  return per_cu * cus;
+}
+
+}  // namespace
+
+// ---- host side ----
+int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timing *timing, hipStream_t s) {
+  const crv_params &p = c->p;
+  const int W = p.window_samples, Sz = p.samples_zs;
+  if (W > 32 * DWORDS || Sz < 1 || W <= Sz)
+    return set_error(c, CRV_E_ARG, "batched chain: window_samples must be in (samples_zs, 192]");
+  if (!c->t.tmpl_f) return set_error(c, CRV_E_STATE, "template not uploaded");
+  if (c->d.nreach > RINGB) return set_error(c, CRV_E_ARG, "batched chain: single-PE template reaches over 16 samples");
+  const int tile = opt->tile > 0 ? opt->tile : 64;
+  if (tile % 64) return set_error(c, CRV_E_ARG, "batched chain: tile must be a multiple of 64");
+  const int grid_s = opt->grid > 0 ? opt->grid : resident_blocks(c, (const void *)kb_sipm);
+  const int grid_w = opt->grid > 0 ? opt->grid : resident_blocks(c, (const void *)kb_wave);
+  hipEvent_t *ev = c->ev, *kev = c->kev;
+  CRV_HIP(c, hipEventRecord(ev[0], s));
+  StepGroups g;
+  int rc = group_steps(c, steps, g, s);
+  if (rc) return rc;
+  const int64_t T = g.T, L4 = T * 4, n = g.n_steps;
+  if (L4 > INT32_MAX / 2) return set_error(c, CRV_E_ARG, "batched chain: too many hit counters in one call");
+  int32_t *Tdev = (int32_t *)ws(c, 500, 8);
+  int32_t *nph = (int32_t *)ws(c, 501, n * 16 + 16);
+  int64_t *sipm_nph = (int64_t *)ws(c, 502, (L4 + 1) * 8);
+  int64_t *cap = (int64_t *)ws(c, 503, (L4 + 1) * 8);
+  int64_t *loff = (int64_t *)ws(c, 504, (L4 + 2) * 8);
+  uint32_t *skey = (uint32_t *)ws(c, 505, L4 * 4 + 8), *skey2 = (uint32_t *)ws(c, 506, L4 * 4 + 8);
+  int32_t *sidx = (int32_t *)ws(c, 507, L4 * 4 + 8), *order = (int32_t *)ws(c, 508, L4 * 4 + 8);
+  int32_t *nfired = (int32_t *)ws(c, 512, L4 * 4 + 8);
+  int *counters = (int *)ws(c, 540, 64);  // queues of kb_sipm and kb_wave, record / run / digi counts
+  Out o{};
+  o.xstride = (int64_t)grid_s * 64;
+  o.xcur = (uint16_t *)ws(c, 543, (size_t)o.xstride * CMAXB * 2);
+  o.xcand = (uint16_t *)ws(c, 544, (size_t)o.xstride * CMAXB * 2);
+  o.xmult = (uint8_t *)ws(c, 545, (size_t)o.xstride * CMAXB);
+  o.gS = (float *)ws(c, 534, n * 16 + 16);
+  o.gHead = (double *)ws(c, 535, n * 32 + 16);
+  o.gTb = (double *)ws(c, 536, n * 32 + 16);
+  o.gR = (int32_t *)ws(c, 537, n * 16 + 16);
+  o.nrun = (int32_t *)ws(c, 513, L4 * 4 + 8);
+  if (!Tdev || !nph || !sipm_nph || !cap || !loff || !skey || !skey2 || !sidx || !order || !nfired || !counters ||
+      !o.xcur || !o.xcand || !o.xmult || !o.gS || !o.gHead || !o.gTb || !o.gR || !o.nrun)
+    return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  o.rec_count = counters + 2;
+  o.run_count = counters + 3;
+  const int32_t T32 = (int32_t)T;
+  CRV_HIP(c, hipMemcpyAsync(Tdev, &T32, 4, hipMemcpyHostToDevice, s));
+  kb_zero<<<1, 64, 0, s>>>(c->flags, F_NFLAGS);
+  const BStep st{g.edep, g.z, g.t};
+  if (L4)
+    kb_count<<<nblocks(L4, 256), 256, 0, s>>>(p, c->t, c->d, st, Tdev, g.task_key, g.task_off, g.perm, g.evfirst,
+                                              nph, sipm_nph, skey, sidx, c->flags);
+  CRV_HIP(c, hipGetLastError());
+  const bool sorted = opt->sort_tasks != 0 && L4 > 0;
+  if (sorted) {
+    size_t tb = 0;
+    CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, skey, skey2, sidx, order, (int)L4, 0, 32, s));
+    void *tmp = ws(c, 509, tb);
+    if (!tmp) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, skey, skey2, sidx, order, (int)L4, 0, 32, s));
+  }
+  const Queue q_sipm{counters, tile, L4, sorted ? order : nullptr}, q_wave{counters + 1, tile, L4, sorted ? order : nullptr};
+  CRV_HIP(c, hipEventRecord(ev[1], s));
+  int hf[F_NFLAGS];
+  // phase 1: photons -> fired-pixel logs (a log that overflows is grown and the phase run again)
+  int64_t nlog = 0;
+  for (int attempt = 0, factor = 1;; ++attempt, factor *= 4) {
+    if (L4) kb_caps<<<nblocks(L4, 256), 256, 0, s>>>(sipm_nph, Tdev, cap, L4, factor);
+    if ((rc = scan_i64(c, cap, loff, L4, 510, s))) return rc;
+    if ((rc = read_i64(c, loff + L4, &nlog, s))) return rc;
+    o.loff = loff;
+    o.log_t = (double *)ws(c, 538, nlog * 8 + 8);
+    o.log_v = (float *)ws(c, 546, nlog * 4 + 8);
+    o.log_pix = (int16_t *)ws(c, 539, nlog * 2 + 8);
+    if (!o.log_t || !o.log_v || !o.log_pix) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    kb_zero<<<1, 64, 0, s>>>(counters, 1);
+    kb_zero<<<1, 64, 0, s>>>(c->flags + F_OVERFLOW, 1);
+    CRV_HIP(c, hipEventRecord(kev[0], s));
+    if (L4)
+      kb_sipm<<<grid_s, 64, 0, s>>>(p, c->t, c->d, st, g.task_key, g.task_off, g.perm, g.evfirst, nph, o, q_sipm, nfired,
+                                    c->flags);
+    CRV_HIP(c, hipGetLastError());
+    CRV_HIP(c, hipEventRecord(kev[1], s));
+    CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
+    CRV_HIP(c, hipStreamSynchronize(s));
+    if (hf[F_CASCADE])
+      return set_error(c, CRV_E_OVERFLOW,
+                       "batched chain: cross-talk cascade wider than 16 pixels or deeper than 31 generations");
+    if (hf[F_COUNTER]) return set_error(c, CRV_E_OVERFLOW, "batched chain: more than 65535 photoelectrons from one step");
+    if (!hf[F_OVERFLOW]) break;
+    if (attempt >= 5) return set_error(c, CRV_E_OVERFLOW, "batched chain: fired-pixel log still too short");
+  }
+  // photons drawn and pixels fired (diagnostics)
+  int64_t *sums = (int64_t *)ws(c, 515, 16);
+  int64_t *fired64 = (int64_t *)ws(c, 516, L4 * 8 + 8);
+  if (!sums || !fired64) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  int64_t hsum[2] = {0, 0};
+  if (L4) {
+    kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(nfired, fired64, L4);
+    size_t tb = 0;
+    CRV_HIP(c, hipcub::DeviceReduce::Sum(nullptr, tb, fired64, sums, (int)L4, s));
+    void *tmp = ws(c, 517, tb);
+    if (!tmp) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, fired64, sums, (int)L4, s));
+    CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, sipm_nph, sums + 1, (int)L4, s));
+    CRV_HIP(c, hipMemcpyAsync(hsum, sums, 16, hipMemcpyDeviceToHost, s));
+  }
+  // phase 2: fired pixels -> records and reco waveforms (pools grown and the phase run again when they overflow)
+  int32_t cnt[2] = {0, 0};
+  for (int attempt = 0;; ++attempt) {
+    o.rec_cap = std::max<int64_t>(c->b_rec_cap, L4 / 8 + 1024);
+    o.run_cap = std::max<int64_t>(c->b_run_cap, L4 / 8 + 1024);
+    o.rec_adc = (uint16_t *)ws(c, 570, o.rec_cap * W * 2);
+    o.rec_label = (int64_t *)ws(c, 571, o.rec_cap * 8);
+    o.rec_tick = (int32_t *)ws(c, 572, o.rec_cap * 4);
+    o.run_rec = (int32_t *)ws(c, 573, o.run_cap * 4);
+    o.run_i0 = (int32_t *)ws(c, 574, o.run_cap * 4);
+    o.run_n = (int32_t *)ws(c, 575, o.run_cap * 4);
+    o.run_k = (int32_t *)ws(c, 576, o.run_cap * 4);
+    o.run_g = (int64_t *)ws(c, 577, o.run_cap * 8);
+    o.run_tdc = (int64_t *)ws(c, 578, o.run_cap * 8);
+    if (!o.rec_adc || !o.rec_label || !o.rec_tick || !o.run_rec || !o.run_i0 || !o.run_n || !o.run_k || !o.run_g ||
+        !o.run_tdc)
+      return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    kb_zero<<<1, 64, 0, s>>>(counters + 1, 3);
+    CRV_HIP(c, hipEventRecord(kev[2], s));
+    if (L4) kb_wave<<<grid_w, 64, 0, s>>>(p, c->t, c->d, g.task_key, nfired, o, q_wave, c->flags);
+    CRV_HIP(c, hipGetLastError());
+    CRV_HIP(c, hipEventRecord(kev[3], s));
+    CRV_HIP(c, hipMemcpyAsync(cnt, counters + 2, 8, hipMemcpyDeviceToHost, s));
+    CRV_HIP(c, hipStreamSynchronize(s));
+    if (cnt[0] <= o.rec_cap && cnt[1] <= o.run_cap) break;
+    if (attempt >= 3) return set_error(c, CRV_E_OVERFLOW, "batched chain: record or run pool still too small");
+    c->b_rec_cap = std::max<int64_t>(o.rec_cap, (int64_t)cnt[0] + cnt[0] / 4 + 1024);
+    c->b_run_cap = std::max<int64_t>(o.run_cap, (int64_t)cnt[1] + cnt[1] / 4 + 1024);
+  }
+  const int64_t nrec = cnt[0], nruns = cnt[1];
+  // the runs' places in the output: each SiPM's runs in run-number order
+  int64_t *nrun64 = (int64_t *)ws(c, 579, (L4 + 1) * 8), *roff = (int64_t *)ws(c, 580, (L4 + 2) * 8);
+  if (!nrun64 || !roff) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (L4) kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(o.nrun, nrun64, L4);
+  if ((rc = scan_i64(c, nrun64, roff, L4, 514, s))) return rc;
+  // phase 3: pulse fits (slots grown and the fits run again when a waveform has more pulses)
+  o.yscr = (float *)ws(c, 581, o.run_cap * W * 4 + 8);
+  o.npr = (int32_t *)ws(c, 582, nruns * 4 + 8);
+  o.rmap = (int32_t *)ws(c, 583, nruns * 4 + 8);
+  if (!o.yscr || !o.npr || !o.rmap) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  o.pcap = opt->pulse_slots > 0 ? opt->pulse_slots : PSLOTS;
+  for (int attempt = 0;; ++attempt) {
+    const int64_t ns = nruns * o.pcap + 8;
+    o.tp = (double *)ws(c, 520, ns * 8);
+    o.mu = (double *)ws(c, 521, ns * 8);
+    float **fp[] = {&o.pe, &o.height, &o.beta, &o.chi2, &o.ped, &o.fa, &o.fb};
+    bool okp = o.tp && o.mu;
+    for (int i = 0; i < 7; ++i) okp = okp && (*fp[i] = (float *)ws(c, 522 + i, ns * 4));
+    o.sf = (uint32_t *)ws(c, 529, ns * 4);
+    if (!okp || !o.sf) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    kb_zero<<<1, 64, 0, s>>>(c->flags + F_PSLOT, 1);
+    CRV_HIP(c, hipEventRecord(kev[4], s));
+    if (nruns) kb_fit<<<nblocks(nruns, 64), 64, 0, s>>>(p, c->t, g.task_key, roff, o, nruns, c->flags);
+    CRV_HIP(c, hipGetLastError());
+    CRV_HIP(c, hipEventRecord(kev[5], s));
+    CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
+    CRV_HIP(c, hipStreamSynchronize(s));
+    if (!hf[F_PSLOT]) break;
+    if (attempt >= 5) return set_error(c, CRV_E_OVERFLOW, "batched chain: pulse slots still too few");
+    o.pcap *= 2;
+  }
+  CRV_HIP(c, hipEventRecord(ev[2], s));
+  // pulse table
+  int64_t *npr64 = (int64_t *)ws(c, 541, (nruns + 1) * 8), *poff = (int64_t *)ws(c, 542, (nruns + 2) * 8);
+  if (!npr64 || !poff) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (nruns) kb_widen<<<nblocks(nruns, 256), 256, 0, s>>>(o.npr, npr64, nruns);
+  if ((rc = scan_i64(c, npr64, poff, nruns, 511, s))) return rc;
+  int64_t np = 0;
+  if ((rc = read_i64(c, poff + nruns, &np, s))) return rc;
+  crv_pulses P{};
+  const int64_t m = np + 1;
+  int64_t **ip[] = {&P.wf, &P.seq, &P.ev};
+  for (int i = 0; i < 3; ++i) *ip[i] = (int64_t *)ws(c, 550 + i, m * 8);
+  P.chan = (int32_t *)ws(c, 553, m * 4);
+  double **dp[] = {&P.pe, &P.peph, &P.time, &P.height, &P.beta, &P.chi2, &P.letime, &P.ped, &P.fit_a, &P.fit_mu,
+                   &P.fit_b};
+  for (int i = 0; i < 11; ++i) *dp[i] = (double *)ws(c, 554 + i, m * 8);
+  uint8_t **bp[] = {&P.failed, &P.zerondf, &P.pedfromdb};
+  for (int i = 0; i < 3; ++i) *bp[i] = (uint8_t *)ws(c, 565 + i, m + 8);
+  int32_t *pbar = (int32_t *)ws(c, 568, m * 4), *psipm = (int32_t *)ws(c, 569, m * 4);
+  bool okp = P.chan && pbar && psipm;
+  for (int i = 0; i < 3; ++i) okp = okp && *ip[i] && *bp[i];
+  for (int i = 0; i < 11; ++i) okp = okp && *dp[i];
+  if (!okp) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  P.capacity = np;
+  P.n = np;
+  if (nruns) kb_pulses_out<<<nblocks(nruns, 256), 256, 0, s>>>(p, c->t, nruns, g.task_key, poff, o, P, pbar, psipm);
+  CRV_HIP(c, hipGetLastError());
+  CRV_HIP(c, hipEventRecord(ev[3], s));
+  crv_clusters C{};
+  C.capacity = -1;
+  if ((rc = clusters_stage(c, P.ev, pbar, psipm, P.time, P.pe, P.seq, np, &C, s))) return rc;
+  CRV_HIP(c, hipEventRecord(ev[4], s));
+  CRV_HIP(c, hipEventSynchronize(ev[4]));
+  c->last_pulses = P;
+  c->last_clusters = C;
+  opt->record_adc = o.rec_adc;
+  opt->record_label = o.rec_label;
+  opt->record_tick = o.rec_tick;
+  opt->n_records = nrec;
+  opt->n_waveforms = nruns;
+  opt->overlong = hf[F_OVERLONG];
+  opt->capped = hf[F_CAPPED];
+  opt->pulse_slots_used = o.pcap;
+  if (timing) {
+    float ms[4], km[3] = {0.f, 0.f, 0.f}, kc = 0.f, tot = 0.f;
+    for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+    for (int i = 0; i < 3; ++i) (void)hipEventElapsedTime(&km[i], kev[2 * i], kev[2 * i + 1]);
+    if (np && c->kev_on) (void)hipEventElapsedTime(&kc, kev[6], kev[7]);
+    (void)hipEventElapsedTime(&tot, ev[0], ev[4]);
+    *timing = crv_timing{};
+    timing->charges_ms = ms[0];   // grouping, photon counts, SiPM order
+    timing->digis_ms = ms[1];     // the three phases with their sizing and re-runs
+    timing->pulses_ms = ms[2];    // pulse table
+    timing->clusters_ms = ms[3];
+    timing->total_ms = tot;
+    timing->k_sipm_ms = km[0];    // kb_sipm (last run)
+    timing->k_digi_ms = km[1];    // kb_wave (last run)
+    timing->k_pulse_ms = km[2];   // kb_fit (last run)
+    timing->k_coinc_ms = kc;
+    timing->n_steps = n;
+    timing->n_hit_counters = T;
+    timing->n_photons = hsum[1];
+    timing->n_charges = hsum[0];  // fired pixels (the charges, not materialised as a table)
+    timing->n_records = nrec;
+    timing->n_waveforms = nruns;
+    timing->n_pulses = np;
+    timing->n_clusters = C.n;
+    timing->n_members = C.n_mem;
+  }
+  return CRV_OK;
+}
+
+}  // namespace crv

@@ -494,9 +494,19 @@ __global__ void k_coinc_out(int64_t nev, const int64_t *__restrict__ eoff, const
 int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t *sipm, const double *time,
                    const double *pe, const int64_t *seq, int64_t n, crv_clusters *out, hipStream_t s) {
   if (!c->t.cuts) return set_error(c, CRV_E_STATE, "clusters: geometry not uploaded");
-  if (n == 0) {
+  if (n == 0) {  // no clusters: the member offsets are the single 0
     out->n = 0;
     out->n_mem = 0;
+    if (out->capacity < 0) {
+      out->cl_mem_off = (int64_t *)ws(c, 352, 8);
+      if (!out->cl_mem_off) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+      out->capacity = 0;
+      out->mem_capacity = 0;
+    }
+    if (out->cl_mem_off) {
+      CRV_HIP(c, hipMemsetAsync(out->cl_mem_off, 0, 8, s));
+      CRV_HIP(c, hipStreamSynchronize(s));
+    }
     return CRV_OK;
   }
   int64_t *evk = (int64_t *)ws(c, 90, n * 8);

@@ -72,3 +72,22 @@ def test_generated_constants_equal_the_oracle():
         assert vals["FM_" + k + "_INIT"] == c[k], k
     assert vals["FM_STIRLING_INIT"] == R.STIRLING and vals["FM_FACT_INIT"] == R.FACT
     assert vals["FM_LN_SQRT_2PI"] == R.LN_SQRT_2PI and vals["FM_E"] == np.e
+
+
+def _header_fields(struct):
+    text = open(HEADER).read()
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct, struct), text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = " ".join(decl.split())
+        m = re.match(r"(?:const )?(?:u?int\d+_t|double)\s*(.*)", decl)
+        if m:
+            fields += [f.strip().lstrip("*").strip() for f in m.group(1).split(",")]
+    return fields
+
+
+@pytest.mark.parametrize("struct,cls", [("crv_batch_opts", "BatchOpts"), ("crv_timing", "Timing"),
+                                        ("crv_pulses", "Pulses"), ("crv_clusters", "Clusters")])
+def test_ctypes_structs_follow_the_header(struct, cls):
+    assert [f for f, _ in getattr(L, cls)._fields_] == _header_fields(struct)

@@ -1,0 +1,154 @@
+"""The FP32 batched chain (crv_chain_run_batched) against the FP64 chain and the oracle, through the C-ABI.
+
+The batched chain draws the same Philox streams as the FP64 chain (which equals the oracle bit for bit) but computes
+in FP32, so its tables agree with the FP64 ones statistically; the tolerances are written below. It must also be
+bit-reproducible and independent of the launch shape (grid, tile, SiPM order, pulse-slot growth): each SiPM and each
+reco waveform is computed by one lane, whichever lane that is.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from crvhip import chain as CH, config as CF, detector as DT, synth as SY  # noqa: E402
+from oracle import chain as OC  # noqa: E402
+from crvhip import template as TP  # noqa: E402
+
+COUNT_RTOL = 3e-3   # pulse count, PE sum, cluster count, cluster members: FP32 chain vs FP64 chain
+MATCH_DT = 1.0      # ns: a FP32 pulse matches the FP64 pulse of its channel nearest in time within this
+MATCH_FRAC = 0.99   # fraction of pulses with a match
+TIME_MEDIAN = 0.05  # ns: median |dt| of matched pulses
+PE_MEDIAN = 2e-3    # median relative |dPE| of matched pulses
+
+
+def _cfgs(**kw):
+    c64 = dataclasses.replace(CF.ChainConfig(), **kw)
+    return c64, dataclasses.replace(c64, precision="fp32")
+
+
+def _close(a, b, rtol, slack=3):
+    assert abs(a - b) <= max(slack, rtol * max(abs(a), abs(b))), (a, b)
+
+
+def _compare(r32, r64, rtol=COUNT_RTOL):
+    P32, P64 = r32["pulses"], r64["pulses"]
+    C32, C64 = r32["clusters"], r64["clusters"]
+    assert len(P64["ev"]) > 0
+    _close(len(P32["ev"]), len(P64["ev"]), rtol)
+    _close(P32["pe"].sum(), P64["pe"].sum(), rtol)
+    _close(len(C32["cl_event"]), len(C64["cl_event"]), rtol)
+    _close(len(C32["mem_idx"]), len(C64["mem_idx"]), rtol)
+    # matched pulses: same (event, channel), nearest time
+    k32 = P32["ev"] * 32768 + P32["chan"]
+    k64 = P64["ev"] * 32768 + P64["chan"]
+    o = np.lexsort((P64["time"], k64))
+    k64s, t64s, pe64s = k64[o], P64["time"][o], P64["pe"][o]
+    lo, hi = np.searchsorted(k64s, k32, "left"), np.searchsorted(k64s, k32, "right")
+    dt = np.full(len(k32), np.inf)
+    dpe = np.full(len(k32), np.inf)
+    for i in np.flatnonzero(hi > lo):
+        d = np.abs(t64s[lo[i]:hi[i]] - P32["time"][i])
+        j = int(np.argmin(d))
+        dt[i] = d[j]
+        dpe[i] = abs(pe64s[lo[i] + j] - P32["pe"][i]) / max(abs(pe64s[lo[i] + j]), 1e-9)
+    ok = dt < MATCH_DT
+    assert ok.mean() >= MATCH_FRAC, ok.mean()
+    assert np.median(dt[ok]) < TIME_MEDIAN, np.median(dt[ok])
+    assert np.median(dpe[ok]) < PE_MEDIAN, np.median(dpe[ok])
+
+
+@pytest.fixture(scope="module")
+def cosmic():
+    c64, c32 = _cfgs()
+    cond, G = DT.nominal_conditions(), DT.sector_geom()
+    S = SY.cosmic_events(3000, seed=11)
+    r64 = CH.run_chain(S, cond, c64, key=7, geom=G)
+    r32 = CH.run_chain(S, cond, c32, key=7, geom=G, records=True)
+    return S, cond, G, c64, c32, r64, r32
+
+
+def test_fp32_chain_matches_fp64_chain(cosmic):
+    S, cond, G, c64, c32, r64, r32 = cosmic
+    _compare(r32, r64)
+    assert len(r32["pulses"]["ev"]) > 50_000
+
+
+def test_fp32_chain_matches_oracle():
+    c64, c32 = _cfgs()
+    cond, G = DT.nominal_conditions(), DT.sector_geom()
+    S = SY.cosmic_events(150, seed=3)
+    ro = OC.run(S, cond, G, c64, 9, TP.load(c64.waveform))
+    r32 = CH.run_chain(S, cond, c32, key=9, geom=G)
+    Po, Co = ro["P"], ro["clusters"]
+    _close(len(r32["pulses"]["ev"]), len(Po["pe"]), COUNT_RTOL)
+    _close(r32["pulses"]["pe"].sum(), Po["pe"].sum(), COUNT_RTOL)
+    _close(len(r32["clusters"]["cl_event"]), len(Co["cl_event"]), COUNT_RTOL)
+
+
+def test_pulses_ordered_by_event_channel_time(cosmic):
+    P = cosmic[6]["pulses"]
+    k = P["ev"] * 32768 + P["chan"]
+    assert np.all(np.diff(k) >= 0)
+    same = np.diff(k) == 0
+    assert np.all(np.diff(P["wf"])[same] >= 0)
+    assert np.all(np.diff(P["wf"]) >= 0)
+
+
+def test_launch_shape_does_not_change_results(cosmic):
+    S, cond, G, c64, c32, r64, r32 = cosmic
+    for batch in ({"grid": 96, "tile": 128}, {"sort_tasks": 1, "tile": 64}, {"pulse_slots": 1, "grid": 300}):
+        r = CH.run_chain(S, cond, c32, key=7, geom=G, batch=batch)
+        for k, v in r32["pulses"].items():
+            np.testing.assert_array_equal(r["pulses"][k], v, err_msg=f"{batch} {k}")
+        for k, v in r32["clusters"].items():
+            np.testing.assert_array_equal(r["clusters"][k], v, err_msg=f"{batch} {k}")
+    assert r["timing"]["pulse_slots_used"] >= 2  # the one-slot run had to grow
+
+
+def test_records_match_fp64_records(cosmic):
+    S, cond, G, c64, c32, r64, r32 = cosmic
+    R = r32["records"]
+    assert len(R["label"]) > 10_000
+    assert np.all(R["adc"] <= 4095)
+    Q = CH.charges(S, None, G, cond, c64, None, key=7)
+    D = CH.digis(Q, cond, c64, None, key=7)
+    k64 = D["record_label"] * 2 ** 20 + D["record_tick"]
+    k32 = R["label"] * 2 ** 20 + R["tick"]
+    _close(len(k32), len(k64), 0.01, slack=20)
+    both, i32, i64 = np.intersect1d(k32, k64, return_indices=True)
+    assert len(both) >= 0.99 * len(k64)
+    same = R["adc"][i32].astype(np.int32) == D["record_adc"][i64].astype(np.int32)
+    assert same.mean() > 0.99, same.mean()  # same noise draws; FP32 sums move a word by one now and then
+
+
+def test_empty_and_tiny_inputs():
+    c64, c32 = _cfgs()
+    cond, G = DT.nominal_conditions(), DT.sector_geom()
+    S = SY.cosmic_events(1, seed=0)
+    empty = {k: v[:0] for k, v in S.items()}
+    r = CH.run_chain(empty, cond, c32, key=1, geom=G)
+    assert len(r["pulses"]["ev"]) == 0 and len(r["clusters"]["cl_event"]) == 0
+    S = SY.cosmic_events(5, seed=1)
+    _r64, _r32 = CH.run_chain(S, cond, c64, key=2, geom=G), CH.run_chain(S, cond, c32, key=2, geom=G)
+    _close(len(_r32["pulses"]["ev"]), len(_r64["pulses"]["ev"]), 0.05)
+
+
+def test_heavy_cross_talk_and_dead_channels():
+    c64, c32 = _cfgs(sipm=dataclasses.replace(CF.SiPMConfig(), cross_talk=0.2))
+    cond = DT.nominal_conditions()
+    st = np.array(cond.status)
+    rng = np.random.default_rng(4)
+    st[rng.choice(len(st), 300, replace=False)] = rng.choice([1, 2, 4], 300)
+    cond = dataclasses.replace(cond, status=st)
+    G = DT.sector_geom()
+    S = SY.cosmic_events(1500, seed=21)
+    _compare(CH.run_chain(S, cond, c32, key=5, geom=G), CH.run_chain(S, cond, c64, key=5, geom=G), rtol=5e-3)
+
+
+def test_uneven_showers():
+    c64, c32 = _cfgs()
+    cond, G = DT.nominal_conditions(), DT.sector_geom()
+    S = SY.uneven_events(300, seed=8)
+    _compare(CH.run_chain(S, cond, c32, key=3, geom=G), CH.run_chain(S, cond, c64, key=3, geom=G), rtol=5e-3)
