@@ -105,6 +105,21 @@ def test_clusters_match(setup):
     _cluster_compare(out[0], ref)
 
 
+def test_clusters_match_in_every_event_size_class(setup):
+    """Events of up to 128 and up to 256 pulses take the two wavefront finders, larger ones the lane finder: merge
+    groups of eight events (about 380 pulses) and of four (about 190), keep the rest single; all must equal the
+    oracle."""
+    cfg, cond, G, tmpl, S, Qo = setup
+    Wo = dict(OC.reco_waveforms(OC.digis(Qo, cond, cfg, 5, tmpl), cond))
+    ev = np.asarray(Wo["ev"])
+    Wo["ev"] = np.where(ev < 80, ev // 8, np.where(ev < 160, 100 + ev // 4, ev + 1000))
+    out, _ = CH.clusters(Wo, cond, G, None, cfg, detail=True)
+    ref, _ = OC.clusters(Wo, cond, G, cfg)
+    npl = np.bincount(np.unique(ref["pulses"]["ev"], return_inverse=True)[1])
+    assert (npl > 256).any() and ((npl > 128) & (npl <= 256)).any() and (npl <= 128).any()
+    _cluster_compare(out[0], ref)
+
+
 def test_clusters_match_reference_golden(golden):
     cfg = CF.ChainConfig()
     cond = DT.nominal_conditions()
