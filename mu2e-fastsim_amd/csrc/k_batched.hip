@@ -249,8 +249,9 @@ __device__ __forceinline__ void feig(const float jtj[6], float &emin, float &ema
 
 
 struct Out {
-  // fired-pixel log per SiPM (offsets loff)
-  const int64_t *loff;
+  // fired-pixel log: per group of 64 SiPMs in queue order, entry k of the group's lane l at goff + 64 k + l (the
+  // lanes of a wavefront write and read neighbouring words); a group holds gcap entries per lane
+  const int64_t *goff, *gcap;
   double *log_t;
   float *log_v;
   int16_t *log_pix;
@@ -309,7 +310,7 @@ __device__ __forceinline__ float pixel_v(const uint32_t *bm, int lane, int pix, 
                                          double tph, float Vov, float inv_tau) {
   if (!bget(bm, lane, pix)) return Vov;
   for (int64_t q = nlog - 1; q >= 0; --q)
-    if (o.log_pix[lb + q] == pix) return Vov * (1.f - __expf(-(float)(tph - o.log_t[lb + q]) * inv_tau));
+    if (o.log_pix[lb + q * 64] == pix) return Vov * (1.f - __expf(-(float)(tph - o.log_t[lb + q * 64]) * inv_tau));
   return Vov;
 }
 // kb_sipm, one lane per SiPM: photons of the counter's steps in time order -> pixel, recovery, firing, cross-talk
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
       const int a = task_off[tk], b = task_off[tk + 1];
       const int nstr = b - a;
       for (int w = 0; w < CRV_DEAD_WORDS; ++w) bm[w * 64 + lane] = 0u;
-      const int64_t lb = o.loff[g], lcap = o.loff[g + 1] - lb;
+      const int64_t lb = o.goff[gq >> 6] + (gq & 63), lcap = o.gcap[gq >> 6];
       int64_t nlog = 0;
       // photon streams: in registers for a counter with one step, in global memory otherwise
       float S1 = 0.f;
@@ -458,9 +459,9 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           bad = true;
           break;
         }
-        o.log_t[lb + nlog] = tph;
-        o.log_v[lb + nlog] = v;
-        o.log_pix[lb + nlog] = (int16_t)pix;
+        o.log_t[lb + nlog * 64] = tph;
+        o.log_v[lb + nlog * 64] = v;
+        o.log_pix[lb + nlog * 64] = (int16_t)pix;
         ++nlog;
         bset(bm, lane, pix);
         // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order
@@ -521,9 +522,9 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
               bad = true;
               break;
             }
-            o.log_t[lb + nlog] = tph;
-            o.log_v[lb + nlog] = vc;
-            o.log_pix[lb + nlog] = (int16_t)c;
+            o.log_t[lb + nlog * 64] = tph;
+            o.log_v[lb + nlog * 64] = vc;
+            o.log_pix[lb + nlog * 64] = (int16_t)c;
             ++nlog;
             bset(bm, lane, c);
             xcur[ncur * xs] = (uint16_t)c;
@@ -737,7 +738,7 @@ __global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv
       L.task = g;
       L.nrun = 0;
       const int nf = nfired[g];
-      const int64_t lb = o.loff[g];
+      const int64_t lb = o.goff[gq >> 6] + (gq & 63);
       L.td = 0.0;
       if (nf > 0) {
         const uint32_t fk = p.feb_key_roc ? (uint32_t)(t.roc[L.ch] * 1000 + t.feb[L.ch]) : (uint32_t)t.feb[L.ch];
@@ -750,11 +751,11 @@ __global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv
       int q = 0;
       while (q < nf) {
         // the charges of one time (a fired pixel and its cross-talk)
-        const double tq = o.log_t[lb + q];
-        float vsum = o.log_v[lb + q];
+        const double tq = o.log_t[lb + (int64_t)q * 64];
+        float vsum = o.log_v[lb + (int64_t)q * 64];
         int n = 1;
-        while (q + n < nf && o.log_t[lb + q + n] == tq) {
-          vsum += o.log_v[lb + q + n];
+        while (q + n < nf && o.log_t[lb + (int64_t)(q + n) * 64] == tq) {
+          vsum += o.log_v[lb + (int64_t)(q + n) * 64];
           ++n;
         }
         q += n;
@@ -933,13 +934,19 @@ __global__ __launch_bounds__(64) void kb_fit(crv_params p, Tables t, const uint6
   o.rmap[fr] = (int32_t)r;
 }
 
-// fired-pixel log capacity per SiPM: photons + a cross-talk margin
-__global__ void kb_caps(const int64_t *__restrict__ nph, const int32_t *__restrict__ Tdev, int64_t *__restrict__ cap,
-                        int64_t n, int factor) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  const int64_t v = nph[g];
-  cap[g] = g < (int64_t)(*Tdev) * 4 ? v + v * factor / 4 + 8 : 0;
+// fired-pixel log capacity per lane of each group of 64 SiPMs in queue order: the largest of the group's photon
+// counts + a cross-talk margin; the group takes 64 times that
+__global__ void kb_gcaps(const int64_t *__restrict__ nph, const int32_t *__restrict__ order, int64_t L4,
+                         int64_t *__restrict__ gcap, int64_t *__restrict__ gsize, int factor) {
+  const int64_t grp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (grp >= (L4 + 63) / 64) return;
+  int64_t m = 0;
+  for (int64_t gq = grp * 64; gq < L4 && gq < grp * 64 + 64; ++gq) {
+    const int64_t v = nph[order ? order[gq] : gq];
+    m = max(m, v + v * factor / 4 + 8);
+  }
+  gcap[grp] = m;
+  gsize[grp] = m * 64;
 }
 
 __global__ void kb_zero(int *__restrict__ p, int n) {
@@ -1025,8 +1032,9 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   int32_t *Tdev = (int32_t *)ws(c, 500, 8);
   int32_t *nph = (int32_t *)ws(c, 501, n * 16 + 16);
   int64_t *sipm_nph = (int64_t *)ws(c, 502, (L4 + 1) * 8);
-  int64_t *cap = (int64_t *)ws(c, 503, (L4 + 1) * 8);
-  int64_t *loff = (int64_t *)ws(c, 504, (L4 + 2) * 8);
+  const int64_t ngrp = (L4 + 63) / 64;
+  int64_t *gcap = (int64_t *)ws(c, 503, (ngrp + 1) * 8), *gsize = (int64_t *)ws(c, 518, (ngrp + 1) * 8);
+  int64_t *goff = (int64_t *)ws(c, 504, (ngrp + 2) * 8);
   uint32_t *skey = (uint32_t *)ws(c, 505, L4 * 4 + 8), *skey2 = (uint32_t *)ws(c, 506, L4 * 4 + 8);
   int32_t *sidx = (int32_t *)ws(c, 507, L4 * 4 + 8), *order = (int32_t *)ws(c, 508, L4 * 4 + 8);
   int32_t *nfired = (int32_t *)ws(c, 512, L4 * 4 + 8);
@@ -1041,7 +1049,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   o.gTb = (double *)ws(c, 536, n * 32 + 16);
   o.gR = (int32_t *)ws(c, 537, n * 16 + 16);
   o.nrun = (int32_t *)ws(c, 513, L4 * 4 + 8);
-  if (!Tdev || !nph || !sipm_nph || !cap || !loff || !skey || !skey2 || !sidx || !order || !nfired || !counters ||
+  if (!Tdev || !nph || !sipm_nph || !gcap || !gsize || !goff || !skey || !skey2 || !sidx || !order || !nfired || !counters ||
       !o.xcur || !o.xcand || !o.xmult || !o.gS || !o.gHead || !o.gTb || !o.gR || !o.nrun)
     return set_error(c, CRV_E_HIP, "workspace allocation failed");
   o.rec_count = counters + 2;
@@ -1068,10 +1076,11 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   // phase 1: photons -> fired-pixel logs (a log that overflows is grown and the phase run again)
   int64_t nlog = 0;
   for (int attempt = 0, factor = 1;; ++attempt, factor *= 4) {
-    if (L4) kb_caps<<<nblocks(L4, 256), 256, 0, s>>>(sipm_nph, Tdev, cap, L4, factor);
-    if ((rc = scan_i64(c, cap, loff, L4, 510, s))) return rc;
-    if ((rc = read_i64(c, loff + L4, &nlog, s))) return rc;
-    o.loff = loff;
+    if (L4) kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, sorted ? order : nullptr, L4, gcap, gsize, factor);
+    if ((rc = scan_i64(c, gsize, goff, ngrp, 510, s))) return rc;
+    if ((rc = read_i64(c, goff + ngrp, &nlog, s))) return rc;
+    o.goff = goff;
+    o.gcap = gcap;
     o.log_t = (double *)ws(c, 538, nlog * 8 + 8);
     o.log_v = (float *)ws(c, 546, nlog * 4 + 8);
     o.log_pix = (int16_t *)ws(c, 539, nlog * 2 + 8);
@@ -1111,8 +1120,8 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   // phase 2: fired pixels -> records and reco waveforms (pools grown and the phase run again when they overflow)
   int32_t cnt[2] = {0, 0};
   for (int attempt = 0;; ++attempt) {
-    o.rec_cap = std::max<int64_t>(c->b_rec_cap, L4 / 8 + 1024);
-    o.run_cap = std::max<int64_t>(c->b_run_cap, L4 / 8 + 1024);
+    o.rec_cap = std::max<int64_t>(c->b_rec_cap, L4 + 1024);
+    o.run_cap = std::max<int64_t>(c->b_run_cap, L4 + 1024);
     o.rec_adc = (uint16_t *)ws(c, 570, o.rec_cap * W * 2);
     o.rec_label = (int64_t *)ws(c, 571, o.rec_cap * 8);
     o.rec_tick = (int32_t *)ws(c, 572, o.rec_cap * 4);

@@ -490,7 +490,6 @@ struct WS {
   double cmaxd[WMAX];  // widened max distance, by coinc position
   double wq[WMAX];     // widened max distance, by by-layer position
   double cdt[WMAX];    // attachment dt, by candidate position
-  uint64_t link[WMAX * WNW];
   uint64_t kmask[WNW], mmask[WNW];
   int32_t bar[WMAX];
   int16_t sec[WMAX], cit[WMAX], chk[WMAX], off[WMAX + 1], off2[WMAX + 1];
@@ -498,6 +497,8 @@ struct WS {
   uint8_t sc[WMAX];    // sorted attachments: candidate position
   uint8_t layer[WMAX], side[WMAX], sipm[WMAX], seq0[WMAX];
   uint8_t hl[WMAX], filt[WMAX], items[WMAX], coinc[WMAX], byl[WMAX], kept[WMAX], cl2[WMAX];
+  uint8_t sl[WMAX], fpass[WMAX];  // hits grouped by (side, layer) in hit order; filterHits result by hit position
+  int16_t gof[9];
   int32_t cl_mc[WMAX];  // members of a second-level cluster (0: dropped)
 };
 
@@ -531,62 +532,97 @@ __device__ __forceinline__ void lds_fmax(double *addr, double v) {
   } while (old != assumed);
 }
 
-// greedy findClusters of the pulses list[0..m): link matrix across the lanes, then the serial growth on bit masks
-// (each pass visits the remaining hits in list order; a hit joins through any member). Members go to out[] in join
-// order, cluster starts to off[]; returns the number of clusters.
+// greedy findClusters of the pulses list[0..m) (reco/coinc.py:128-152): each pass visits the remaining hits in list
+// order and a hit joins through any member, including one that joined earlier in the same pass; passes repeat while
+// the cluster grows, and the first remaining hit seeds the next cluster. One pass is the least fixpoint of
+//   J = { p remaining : link(p) meets M or { q in J : q < p } }      (M = the members before the pass),
+// reached from J = {} by repeated evaluation: each lane holds the link rows of its list positions and a round is a
+// ballot. Members go to out[] in join order (pass by pass, ascending within a pass), cluster starts to off[].
 template <int WMAX>
 __device__ int wave_clusters(const Tables &t, WS<WMAX> &S, const uint8_t *list, int m, const double *md, double max_dt,
                              bool opposite, double dt_opp, uint8_t *out, int16_t *off) {
   constexpr int WNW = WMAX / 64;
   const int lane = threadIdx.x;
-  for (int i = lane; i < m; i += 64) {
-    const int h = list[i];
-    for (int w = 0; w < WNW; ++w) {
-      uint64_t bits = 0;
-      for (int j = w * 64; j < m && j < w * 64 + 64; ++j) {
+  const uint64_t below = lanes_below();
+  uint64_t row[WNW][WNW];  // link rows of list positions w * 64 + lane
+#pragma unroll
+  for (int w = 0; w < WNW; ++w) {
+#pragma unroll
+    for (int v = 0; v < WNW; ++v) row[w][v] = 0ull;
+    const int i = w * 64 + lane;
+    if (i < m) {
+      const int h = list[i];
+      const double th = S.t[h], xh = S.x[h], mh = md[h];
+      const int sh = S.side[h];
+      for (int j = 0; j < m; ++j) {
         const int c = list[j];
-        const double mdd = fmax(md[h], md[c]);
-        const double mt = max_dt + ((opposite && S.side[h] != S.side[c]) ? dt_opp : 0.0);
-        if (fabs(S.x[h] - S.x[c]) <= mdd && fabs(S.t[h] - S.t[c]) < mt) bits |= 1ull << (j - w * 64);
+        const double mdd = fmax(mh, md[c]);
+        const double mt = max_dt + ((opposite && sh != S.side[c]) ? dt_opp : 0.0);
+        const bool l = fabs(xh - S.x[c]) <= mdd && fabs(th - S.t[c]) < mt;
+#pragma unroll
+        for (int v = 0; v < WNW; ++v)
+          if (v == (j >> 6)) row[w][v] |= l ? 1ull << (j & 63) : 0ull;
       }
-      S.link[i * WNW + w] = bits;
     }
   }
-  __syncthreads();
   uint64_t R[WNW];
+#pragma unroll
   for (int w = 0; w < WNW; ++w) {
     const int k = m - w * 64;
     R[w] = k >= 64 ? ~0ull : (k > 0 ? (1ull << k) - 1ull : 0ull);
   }
   int nout = 0, ncl = 0;
   for (;;) {
-    bool any = false;
-    for (int w = 0; w < WNW; ++w) any |= R[w] != 0;
-    if (!any) break;
+    int seed = -1;
+#pragma unroll
+    for (int w = WNW - 1; w >= 0; --w)
+      if (R[w]) seed = w * 64 + __ffsll((unsigned long long)R[w]) - 1;
+    if (seed < 0) break;
     uint64_t M[WNW];
-    for (int w = 0; w < WNW; ++w) M[w] = 0ull;
-    if (lane == 0) off[ncl] = (int16_t)nout;
-    bool first = true;
-    for (int grown = 1; grown;) {
-      grown = 0;
-      for (int w = 0; w < WNW; ++w) {
-        uint64_t r = R[w];
-        while (r) {
-          const int b = __ffsll((unsigned long long)r) - 1;
-          r &= r - 1;
-          const int pos = w * 64 + b;
-          bool join = first;
-          for (int v = 0; v < WNW && !join; ++v) join = (S.link[pos * WNW + v] & M[v]) != 0;
-          if (join) {
-            first = false;
-            M[w] |= 1ull << b;
-            R[w] &= ~(1ull << b);
-            if (lane == 0) out[nout] = list[pos];
-            ++nout;
-            grown = 1;
+#pragma unroll
+    for (int w = 0; w < WNW; ++w) {
+      M[w] = w == (seed >> 6) ? 1ull << (seed & 63) : 0ull;
+      R[w] &= ~M[w];
+    }
+    if (lane == 0) {
+      off[ncl] = (int16_t)nout;
+      out[nout] = list[seed];
+    }
+    ++nout;
+    for (;;) {  // passes
+      uint64_t J[WNW];
+#pragma unroll
+      for (int w = 0; w < WNW; ++w) J[w] = 0ull;
+      for (;;) {  // fixpoint of one pass
+        uint64_t Jn[WNW];
+        bool same = true;
+#pragma unroll
+        for (int w = 0; w < WNW; ++w) {
+          bool j = false;
+          if ((R[w] >> lane) & 1ull) {
+#pragma unroll
+            for (int v = 0; v < WNW; ++v) {
+              const uint64_t lower = v < w ? J[v] : (v == w ? J[v] & below : 0ull);
+              j |= (row[w][v] & (M[v] | lower)) != 0ull;
+            }
           }
+          Jn[w] = __ballot(j);
+          same &= Jn[w] == J[w];
         }
+#pragma unroll
+        for (int w = 0; w < WNW; ++w) J[w] = Jn[w];
+        if (same) break;
       }
+      bool any = false;
+#pragma unroll
+      for (int w = 0; w < WNW; ++w) {
+        if ((J[w] >> lane) & 1ull) out[nout + __popcll(J[w] & below)] = list[w * 64 + lane];
+        nout += __popcll(J[w]);
+        M[w] |= J[w];
+        R[w] &= ~J[w];
+        any |= J[w] != 0ull;
+      }
+      if (!any) break;
     }
     ++ncl;
   }
@@ -811,30 +847,47 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
       }
       __syncthreads();
       if (nh == 0) continue;
-      // filterHits per side, then clusters of the side and their coincidence check
+      // filterHits (reco/coinc.py:154-182), both sides at once: a hit sums the PE of the hits of its side and layer
+      // within max_dt_adjacent, in hit order, by counter offset; the hits of a (side, layer) are listed together
+      {
+        int pos = 0;
+        for (int grp = 0; grp < 8; ++grp) {
+          if (lane == 0) S.gof[grp] = (int16_t)pos;
+          for (int base = 0; base < nh; base += 64) {
+            const int ii = base + lane;
+            const bool pr = ii < nh && S.side[S.hl[ii]] * 4 + S.layer[S.hl[ii]] == grp;
+            const uint64_t bm = __ballot(pr);
+            if (pr) S.sl[pos + __popcll(bm & lanes_below())] = S.hl[ii];
+            pos += __popcll(bm);
+          }
+        }
+        if (lane == 0) S.gof[8] = (int16_t)pos;
+        __syncthreads();
+        for (int ii = lane; ii < nh; ii += 64) {
+          const int h = S.hl[ii];
+          const Cut ch = cut_of(t, S.sec[h]);
+          const int grp = S.side[h] * 4 + S.layer[h], cnh = S.cit[h];
+          const double th = S.t[h];
+          double same = 0.0, adj1 = 0.0, adj2 = 0.0;
+          for (int jj = S.gof[grp]; jj < S.gof[grp + 1]; ++jj) {
+            const int q = S.sl[jj];
+            if (fabs(S.t[q] - th) > ch.max_dt_adjacent) continue;
+            const int d = S.cit[q] - cnh;
+            same += d == 0 ? S.pe[q] : 0.0;
+            adj1 += d == -1 ? S.pe[q] : 0.0;
+            adj2 += d == 1 ? S.pe[q] : 0.0;
+          }
+          S.fpass[ii] = same + adj1 >= ch.pe_threshold || same + adj2 >= ch.pe_threshold;
+        }
+        __syncthreads();
+      }
+      // per side: the passing hits in hit order, their clusters and the coincidence check of each
       int nco = 0, nfilt = 0;
       for (int side = 0; side < 2; ++side) {
         const int f0 = nfilt;
         for (int base = 0; base < nh; base += 64) {
           const int ii = base + lane;
-          bool keep = false;
-          if (ii < nh) {
-            const int h = S.hl[ii];
-            if (S.side[h] == side) {
-              const Cut ch = cut_of(t, S.sec[h]);
-              const int lh = S.layer[h], cnh = S.cit[h];
-              double same = 0.0, adj1 = 0.0, adj2 = 0.0;
-              for (int jj = 0; jj < nh; ++jj) {
-                const int q = S.hl[jj];
-                if (S.side[q] != side || S.layer[q] != lh || fabs(S.t[q] - S.t[h]) > ch.max_dt_adjacent) continue;
-                const int d = S.cit[q] - cnh;
-                same += d == 0 ? S.pe[q] : 0.0;
-                adj1 += d == -1 ? S.pe[q] : 0.0;
-                adj2 += d == 1 ? S.pe[q] : 0.0;
-              }
-              keep = same + adj1 >= ch.pe_threshold || same + adj2 >= ch.pe_threshold;
-            }
-          }
+          const bool keep = ii < nh && S.side[S.hl[ii]] == side && S.fpass[ii];
           const uint64_t bm = __ballot(keep);
           if (keep) S.filt[nfilt + __popcll(bm & lanes_below())] = S.hl[ii];
           nfilt += __popcll(bm);
