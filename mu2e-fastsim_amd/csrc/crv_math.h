@@ -124,14 +124,6 @@ struct U4 {
   uint32_t x, y, z, w;
 };
 
-CRV_HD uint32_t mulhi32(uint32_t a, uint32_t b) {
-#ifdef __HIP_DEVICE_COMPILE__
-  return __umulhi(a, b);
-#else
-  return (uint32_t)(((uint64_t)a * b) >> 32);
-#endif
-}
-
 CRV_HD U4 philox(U4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
@@ -139,9 +131,9 @@ CRV_HD U4 philox(U4 c, uint32_t k0, uint32_t k1) {
       k0 += 0x9E3779B9u;
       k1 += 0xBB67AE85u;
     }
-    uint32_t hi0 = mulhi32(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-    uint32_t hi1 = mulhi32(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
-    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    // one 32 x 32 -> 64 multiply per word gives both halves (a single v_mad_u64_u32)
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+    c = U4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
   }
   return c;
 }

@@ -328,6 +328,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
   const float pf_full = p1 * (1.f - __expf(-Vov * inv_p2));
   const float qct = (float)dv.q_ct, tauf = (float)p.tau;
   const Draw dr{(uint32_t)p.seed, p.evkey_hi};
+  const float inv_ny = 1.f / (float)t.map_ny;
   const int64_t xb = (int64_t)blockIdx.x * 64 + lane;
   uint16_t *xcur = o.xcur + xb, *xcand = o.xcand + xb;
   uint8_t *xmult = o.xmult + xb;
@@ -355,6 +356,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
       double head1 = 0.0, tb1 = 0.0;
       int r1 = 0, N1 = 0;
       uint32_t j1 = 0;
+      U4 dq1{0u, 0u, 0u, 0u};  // the delay draw holding the stream's next four exponentials
       for (int k = a; k < b; ++k) {
         const int64_t gi = (int64_t)k * 4 + s;
         const int N = nph[gi];
@@ -365,8 +367,10 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
         const double z = st.z[i];
         const double d = (s & 1) == 0 ? z + p.half_length : p.half_length - z;
         const double tb = st.t[i] + d * p.inv_speed;
-        const float S = -__logf(u32f(dr(KIND_DELAY, ev, (uint32_t)ch, j, 0u).x)) / (float)N;
+        const U4 d0 = dr(KIND_DELAY, ev, (uint32_t)ch, j, 0u);
+        const float S = -__logf(u32f(d0.x)) / (float)N;
         if (nstr == 1) {
+          dq1 = d0;
           N1 = N;
           j1 = j;
           tb1 = tb;
@@ -408,8 +412,8 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           r = r1;
           if (r1 + 1 < N1) {
             const int rn = r1 + 1;
-            const U4 uu = dr(KIND_DELAY, ev, (uint32_t)ch, j1, (uint32_t)(rn >> 2));
-            const uint32_t x = (rn & 3) == 0 ? uu.x : (rn & 3) == 1 ? uu.y : (rn & 3) == 2 ? uu.z : uu.w;
+            if ((rn & 3) == 0) dq1 = dr(KIND_DELAY, ev, (uint32_t)ch, j1, (uint32_t)(rn >> 2));
+            const uint32_t x = (rn & 3) == 0 ? dq1.x : (rn & 3) == 1 ? dq1.y : (rn & 3) == 2 ? dq1.z : dq1.w;
             S1 += (-__logf(u32f(x))) / (float)(N1 - rn);
             head1 = tb1 + (double)(tauf * S1);
           }
@@ -445,7 +449,8 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           }
           bin = min(lo, t.ncdf - 1);
         }
-        int px = bin / t.map_ny, py = bin % t.map_ny;
+        int px = (int)(((float)bin + 0.5f) * inv_ny);  // bin < 1600: the float quotient's floor is exact
+        int py = bin - px * t.map_ny;
         if (p.pixel_lrint) {
           px += u32f(u.y) >= 0.5f ? 1 : 0;
           py += u32f(u.z) >= 0.5f ? 1 : 0;
@@ -631,26 +636,29 @@ __device__ __forceinline__ void wave_add(const crv_params &p, const Tables &t, c
     w.overlong = 1;
     return;
   }
-  // here E == j0: j0 never decreases within a cluster and E only advances to it
+  // here E == j0: j0 never decreases within a cluster and E only advances to it. Sample j0 + r sees the template
+  // at t = stt0 + r * 12.5 (stt0 in [0, 12.5)), index rint(t / precision), while that index is inside the template.
   const float q = (float)p.capacitance * vsum / (float)p.single_pe_reference_charge;
   const float inv_prec = 1.f / (float)p.single_pe_precision;
-  const int nt = t.ntmpl;
+  const float b0 = ((float)j0 * PERIODF - dt) * inv_prec, db = PERIODF * inv_prec;
+  const float nt = (float)t.ntmpl;
+  int rr = 0;  // samples reached
 #pragma unroll
   for (int r = 0; r < RINGB; ++r) {
-    if (r < nreach) {
-      const int J = j0 + r;
-      const float stt = (float)J * PERIODF - dt;
-      const float ti = rintf(stt * inv_prec);
-      if (stt >= 0.f && ti < (float)nt) {
-        if (J >= lmax) {
-          w.overlong = 1;
-        } else {
-          w.win[r] += L.tmpl[(int)ti] * q;
-          w.used = max(w.used, J + 1);
-        }
-      }
+    const float ti = rintf(fmaf((float)r, db, b0));
+    if (r < nreach && ti < nt) {
+      w.win[r] += L.tmpl[(int)ti] * q;
+      rr = r + 1;
     }
   }
+  if (j0 + rr > lmax) {  // the part past the window is dropped
+    w.overlong = 1;
+#pragma unroll
+    for (int r = 0; r < RINGB; ++r)
+      if (j0 + r >= lmax) w.win[r] = 0.f;
+    rr = lmax - j0;
+  }
+  w.used = max(w.used, j0 + rr);
 }
 
 __device__ __forceinline__ uint32_t dword_at(const uint32_t m[DWORDS], int q) {
