@@ -25,6 +25,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "CRV events/s (steps->coincidence clusters), FP32 batched vs FP64 plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+PROGRESS = False
 
 # algorithmic bytes per table row (inputs read once, outputs written once)
 STEP_B, RECORD_B, PULSE_B, CLUSTER_B, MEMBER_B = 32, 268, 32, 56, 4
@@ -37,8 +38,8 @@ WAVEFORM_META_B = 40   # start, length, offset, calibration of a reco waveform
 PULSE_IN_B = 40        # event, bar, sipm, time, pe, seq read by the finder
 # FP32 batched chain kernels
 SIPM_STEP_B = 20       # per (step, SiPM) read by kb_sipm: photon count i32, z f64, t f64
-FIRE_B = 14            # fired-pixel log entry written by kb_sipm: time f64, overvoltage f32, pixel i16
-FIRE_IN_B = 12         # log entry read by kb_wave: time, overvoltage
+FIRE_B = 10            # fired-pixel log entry written by kb_sipm: time offset f32, overvoltage f32, pixel i16
+FIRE_IN_B = 8          # log entry read by kb_wave: time offset, overvoltage
 RUN_B = 32             # reco-waveform descriptor: record, first sample, samples, run number i32; SiPM, start TDC i64
 SLOT_B = 48            # pulse slot written by kb_fit: time, location f64; 7 x f32; flags u32
 
@@ -87,10 +88,10 @@ def cpu_baseline(S, n_events, seed, per_worker, workers):
 
 
 class Outputs:
-    """Caller-side pulse and cluster buffers (torch device tensors) sized for n events."""
+    """Caller-side pulse and cluster buffers (torch device tensors) of the given capacities."""
 
-    def __init__(self, L, torch, n_ev):
-        cap_p, cap_c = 90 * n_ev + 1024, 12 * n_ev + 1024
+    def __init__(self, L, torch, cap_p, cap_c):
+        cap_p, cap_c = max(int(cap_p), 1), max(int(cap_c), 1)
         b = {k: torch.empty(cap_p, dtype=torch.int64, device="cuda") for k in ("wf", "seq", "ev")}
         b["chan"] = torch.empty(cap_p, dtype=torch.int32, device="cuda")
         for k in L.PULSE_F64:
@@ -113,32 +114,67 @@ class Outputs:
                             cap_c, 0, L.ptr(c["mem_idx"]), cap_p, 0)
 
 
-def measure(precision, S, n_ev, seed, steps, warmup, batch):
-    """Time `steps` chain passes over the batch; returns (events/s, ms/step, mean timing fields, last counts)."""
+def chunks_of(evkey, n_ev, chunk):
+    """(first step, steps) of each chunk of `chunk` events (the whole batch when chunk is 0); evkey is the steps'
+    event key, a numpy array or a device tensor."""
+    if not chunk or chunk >= n_ev:
+        return [(0, len(evkey))]
+    edges = np.arange(0, n_ev + chunk, chunk).clip(max=n_ev)
+    if isinstance(evkey, np.ndarray):
+        bounds = np.searchsorted(evkey, edges)
+    else:
+        import torch
+        bounds = torch.searchsorted(evkey, torch.as_tensor(edges, dtype=evkey.dtype, device=evkey.device)).cpu().numpy()
+    return [(int(a), int(b - a)) for a, b in zip(bounds[:-1], bounds[1:]) if b > a]
+
+
+def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0):
+    """Time `steps` chain passes over the batch (chunk by chunk, steps resident in HBM); returns (events/s, ms/step,
+    mean timing fields summed over chunks, last counts summed over chunks, extra)."""
     import torch
     from crvhip import _lib as L, config as CF, detector as DT, template as TP
-    from crvhip.chain import DeviceSteps
     cfg = CF.ChainConfig(precision=precision)
     ctx = L.Context(cfg, DT.nominal_conditions(), DT.sector_geom(), TP.load(cfg.waveform), 0)
-    ds = DeviceSteps(S["evkey"], S["counter"], S["edep"], S["z"], S["t"])
     ctx.set_params(cfg, seed, ds.evkey_hi, 32 if precision == "fp32" else 64)
-    out = Outputs(L, torch, n_ev)
-    st = ds.struct()
+    parts = chunks_of(evkey, n_ev, chunk)
+    base = ds.struct()
+    sizes = (4, 4, 8, 8, 8)
+    structs = [L.Steps(*[getattr(base, f) + a * sz for f, sz in zip(("event", "counter", "edep", "z", "t"), sizes)],
+                       n) for a, n in parts]
     tm = L.Timing()
     opts = L.BatchOpts(**batch)
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
-    def step():
+    def call(st, out):
         if precision == "fp32":
-            rc = ctx.lib.crv_chain_run_batched(ctx.h, ctypes.byref(st), ctypes.byref(opts), ctypes.byref(out.P),
-                                               ctypes.byref(out.C), ctypes.byref(tm), stream)
-        else:
-            rc = ctx.lib.crv_chain_run(ctx.h, ctypes.byref(st), ctypes.byref(out.P), ctypes.byref(out.C),
-                                       ctypes.byref(tm), stream)
-        ctx.check(rc)
-        return {f: getattr(tm, f) for f, _ in L.Timing._fields_}
+            return ctx.lib.crv_chain_run_batched(ctx.h, ctypes.byref(st), ctypes.byref(opts), ctypes.byref(out.P),
+                                                 ctypes.byref(out.C), ctypes.byref(tm), stream)
+        return ctx.lib.crv_chain_run(ctx.h, ctypes.byref(st), ctypes.byref(out.P), ctypes.byref(out.C),
+                                     ctypes.byref(tm), stream)
 
-    for _ in range(warmup):
+    # sizing pass (also the first warmup): zero-capacity outputs report the sizes they need
+    need_p = need_c = need_m = 1
+    empty = Outputs(L, torch, 1, 1)
+    empty.P.capacity = empty.C.capacity = empty.C.mem_capacity = 0
+    for st in structs:
+        rc = call(st, empty)
+        if rc not in (0, L.CRV_E_CAPACITY):
+            ctx.check(rc)
+        need_p, need_c, need_m = max(need_p, empty.P.n), max(need_c, empty.C.n), max(need_m, empty.C.n_mem)
+    out = Outputs(L, torch, need_p, max(need_c, need_m))
+
+    def step():
+        acc = {}
+        for i, st in enumerate(structs):
+            ctx.check(call(st, out))
+            if PROGRESS:
+                print(f"[bench] {precision} chunk {i + 1}/{len(structs)}: {tm.total_ms:.1f} ms", file=sys.stderr,
+                      flush=True)
+            for f, _ in L.Timing._fields_:
+                acc[f] = acc.get(f, 0) + getattr(tm, f)
+        return acc
+
+    for _ in range(max(warmup - 1, 0)):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -148,8 +184,9 @@ def measure(precision, S, n_ev, seed, steps, warmup, batch):
     avg = {k: float(np.mean([r[k] for r in runs])) for k in runs[0]}
     extra = {"n_waveforms_batched": opts.n_waveforms, "pulse_slots_used": opts.pulse_slots_used} \
         if precision == "fp32" else {}
+    extra["chunks"] = len(structs)
     ctx.close()
-    del out, ds
+    del out, empty
     torch.cuda.empty_cache()
     return n_ev * steps / dt, dt / steps * 1e3, avg, runs[-1], extra
 
@@ -196,12 +233,19 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--sort", type=int, default=1)
-    ap.add_argument("--cpu-per-worker", type=int, default=150)
+    ap.add_argument("--chunk", type=int, default=0, help="events per chain call (0: the whole batch in one call)")
+    ap.add_argument("--gen", default="auto", choices=("auto", "host", "device"),
+                    help="synthetic input generator: numpy on the host or torch on the device (auto: device for "
+                         "uneven / overlay and for more than 2M events)")
+    ap.add_argument("--cpu-per-worker", type=int, default=0, help="oracle events per CPU worker (0: by workload)")
     ap.add_argument("--cpu-workers", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer three times (the reference kernel for HBM counters)")
+    ap.add_argument("--progress", action="store_true", help="a line on stderr per chain call")
     args = ap.parse_args()
+    global PROGRESS
+    PROGRESS = args.progress
     if args.copy_calibration:
         import torch
         src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
@@ -216,24 +260,43 @@ def main():
               "event ranges, one process per GPU). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    S = generate(args.workload, args.events, args.seed)
+    # inputs: numpy on the host (the oracle's own generator) for the cosmic batch, torch on the device for the large
+    # uneven / overlay batches; the CPU baseline runs first, before anything touches the GPU
+    gen_device = args.gen == "device" or (args.gen == "auto" and (args.workload != "cosmic" or args.events > 2_000_000))
+    per_worker = args.cpu_per_worker or {"cosmic": 150, "uneven": 40, "overlay": 4}[args.workload]
+    workers = args.cpu_workers or min(len(os.sched_getaffinity(0)), 16)
+    S = None if gen_device else generate(args.workload, args.events, args.seed)
     cpu = None
     if not args.no_cpu:
-        workers = args.cpu_workers or min(len(os.sched_getaffinity(0)), 16)
-        cpu = cpu_baseline(S, args.events, args.seed + 1, args.cpu_per_worker, workers)
+        sample = S if S is not None else generate(args.workload, per_worker * workers, args.seed)
+        cpu = cpu_baseline(sample, per_worker * workers, args.seed + 1, per_worker, workers)
+        if S is None:
+            cpu["sample"] += " (host-generated events of the same workload; the GPU batch is generated on the device)"
 
     import torch
     assert torch.cuda.is_available(), "bench needs a GPU"
+    from crvhip.chain import DeviceSteps
+    if gen_device:
+        from crvhip import synth_gpu as SG
+        Sd = SG.events(args.workload, args.events, args.seed)
+        ds = SG.device_steps(Sd)
+        evkey = ds.event
+    else:
+        ds = DeviceSteps(S["evkey"], S["counter"], S["edep"], S["z"], S["t"])
+        evkey = S["evkey"]
     batch = {"grid": args.grid, "tile": args.tile, "sort_tasks": args.sort}
-    v32, ms32, avg32, cnt32, ex32 = measure("fp32", S, args.events, args.seed + 1, args.steps, args.warmup, batch)
+    v32, ms32, avg32, cnt32, ex32 = measure("fp32", ds, evkey, args.events, args.seed + 1, args.steps, args.warmup,
+                                            batch, args.chunk)
     k32 = kernels_fp32(avg32, cnt32, cnt32["n_charges"])
     fp64 = None
-    if args.fp64_events:
+    if args.fp64_events and args.workload == "cosmic":
         n64 = min(args.fp64_events, args.events)
-        ev = S["evkey"]
-        cut = int(np.searchsorted(ev, n64))
-        S64 = {k: v[:cut] for k, v in S.items()}
-        v64, ms64, avg64, cnt64, _ = measure("fp64", S64, n64, args.seed + 1, args.steps, args.warmup, batch)
+        (_, cut), = chunks_of(evkey, args.events, n64)[:1]
+        ev64 = evkey[:cut] if isinstance(evkey, np.ndarray) else evkey[:cut].cpu().numpy().astype(np.int64)
+        ds64 = DeviceSteps(ev64, ds.counter[:cut].cpu().numpy(), ds.edep[:cut].cpu().numpy(),
+                           ds.z[:cut].cpu().numpy(), ds.t[:cut].cpu().numpy())
+        v64, ms64, avg64, cnt64, _ = measure("fp64", ds64, ev64, n64, args.seed + 1, args.steps, args.warmup, batch)
+        del ds64
         k64 = kernels_fp64(avg64, cnt64)
         fp64 = {"value": v64, "unit": "events/s", "events": n64, "ms_per_step": ms64, "dtype": "f64",
                 "speedup_fp32_over_fp64": v32 / v64, "roofline": roofline(k64),
@@ -249,7 +312,9 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.workload} events, FP32 batched chain (3 fused kernels + finder)",
-                   "events": args.events, "seed": args.seed, "precision": "fp32 (fp64 times and 3x3 solve)",
+                   "events": args.events, "chunk": args.chunk or args.events, "seed": args.seed,
+                   "generator": "device (torch)" if gen_device else "host (numpy)",
+                   "precision": "fp32 (fp64 times and 3x3 solve)",
                    "chain": "steps->photons->SiPM->waveforms->pulses->clusters", "grid": args.grid,
                    "tile": args.tile or 64, "sort_tasks": args.sort},
         "roofline": roofline(k32),

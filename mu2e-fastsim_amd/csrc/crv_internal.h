@@ -67,6 +67,8 @@ struct Ctx {
   hipEvent_t kev[8];             // start / end of the main kernel of each stage (sipm, digi, pulse, coinc)
   bool kev_on = false;
   int64_t b_rec_cap = 0, b_run_cap = 0;  // pool sizes of the batched chain (grown on demand)
+  int ws_fail_slot = -1;                  // the workspace request that failed last
+  size_t ws_fail_bytes = 0;
 };
 
 // workspace slot of at least `bytes` bytes (grown, never shrunk)
@@ -77,6 +79,8 @@ enum Flag { F_UNSORTED = 0, F_COUNTER = 1, F_CAPPED = 2, F_OVERFLOW = 3, F_CASCA
             F_NFLAGS = 8 };
 
 int set_error(Ctx *c, int code, const std::string &msg);
+// CRV_E_HIP with the workspace request that could not be allocated
+int ws_fail(Ctx *c);
 #define CRV_HIP(c, call)                                                                  \
   do {                                                                                     \
     hipError_t e_ = (call);                                                                \

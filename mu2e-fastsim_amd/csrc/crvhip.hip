@@ -35,11 +35,18 @@ void *ws(Ctx *c, int slot, size_t bytes) {
     size_t b = bytes + bytes / 8;
     if (hipMalloc(&c->ws[slot], b) != hipSuccess) {
       (void)hipGetLastError();
+      c->ws_fail_slot = slot;
+      c->ws_fail_bytes = b;
       return nullptr;
     }
     c->ws_size[slot] = b;
   }
   return c->ws[slot];
+}
+
+int ws_fail(Ctx *c) {
+  return set_error(c, CRV_E_HIP, "workspace allocation failed (slot " + std::to_string(c->ws_fail_slot) + ", " +
+                                     std::to_string(c->ws_fail_bytes >> 20) + " MiB)");
 }
 
 namespace {
@@ -132,7 +139,7 @@ int waveforms_stage(Ctx *c, const int64_t *label, const int64_t *starttdc, const
   uint8_t *flag = (uint8_t *)ws(c, 130, n + 8);
   int32_t *first = (int32_t *)ws(c, 131, (n + 1) * 4);
   int32_t *nsel = (int32_t *)ws(c, 132, 8);
-  if (!flag || !first || !nsel) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (!flag || !first || !nsel) return ws_fail(c);
   CRV_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int) * F_NFLAGS, s));
   k_wf_flags<<<nblocks(n, 256), 256, 0, s>>>(n, Sz, label, starttdc, flag, c->flags);
   CRV_HIP(c, hipGetLastError());
@@ -482,7 +489,7 @@ int crv_chain_run(crv_ctx *ctx, const crv_steps *steps, crv_pulses *pulses, crv_
   if ((rc = waveforms_stage(c, d.label, d.starttdc, d.adc, d.n, &w, s))) return rc;
   double *wped = (double *)ws(c, 310, w.n * 8 + 8), *warea = (double *)ws(c, 311, w.n * 8 + 8);
   double *wheight = (double *)ws(c, 312, w.n * 8 + 8), *wtoff = (double *)ws(c, 313, w.n * 8 + 8);
-  if (!wped || !warea || !wheight || !wtoff) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (!wped || !warea || !wheight || !wtoff) return ws_fail(c);
   if (w.n) {
     k_gather_cond<<<nblocks(w.n, 256), 256, 0, s>>>(w.n, w.chan, c->t.pedestal, c->t.area, c->t.height, c->t.toff,
                                                     wped, warea, wheight, wtoff);
@@ -495,7 +502,7 @@ int crv_chain_run(crv_ctx *ctx, const crv_steps *steps, crv_pulses *pulses, crv_
     return rc;
   CRV_HIP(c, hipEventRecord(ev[4], s));
   int32_t *pbar = (int32_t *)ws(c, 316, P.n * 4 + 8), *psipm = (int32_t *)ws(c, 317, P.n * 4 + 8);
-  if (!pbar || !psipm) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (!pbar || !psipm) return ws_fail(c);
   if (P.n) {
     k_bar_sipm<<<nblocks(P.n, 256), 256, 0, s>>>(P.n, P.chan, pbar, psipm);
     CRV_HIP(c, hipGetLastError());

@@ -252,7 +252,8 @@ struct Out {
   // fired-pixel log: per group of 64 SiPMs in queue order, entry k of the group's lane l at goff + 64 k + l (the
   // lanes of a wavefront write and read neighbouring words); a group holds gcap entries per lane
   const int64_t *goff, *gcap;
-  double *log_t;
+  float *log_t;     // time - tbase of the SiPM
+  double *tbase;     // per SiPM: the time of its first step
   float *log_v;
   int16_t *log_pix;
   // photon streams of the (step, SiPM) pairs of counters with several steps
@@ -305,12 +306,12 @@ __device__ __forceinline__ bool bget(const uint32_t *bm, int lane, int pix) {
 }
 __device__ __forceinline__ void bset(uint32_t *bm, int lane, int pix) { bm[(pix >> 5) * 64 + lane] |= 1u << (pix & 31); }
 
-// overvoltage of a pixel at time tph: from its last firing in the lane's log (the bitmap says whether there is one)
+// overvoltage of a pixel at time offset toff: from its last firing in the lane's log (the bitmap says whether there is one)
 __device__ __forceinline__ float pixel_v(const uint32_t *bm, int lane, int pix, const Out &o, int64_t lb, int64_t nlog,
-                                         double tph, float Vov, float inv_tau) {
+                                         float toff, float Vov, float inv_tau) {
   if (!bget(bm, lane, pix)) return Vov;
   for (int64_t q = nlog - 1; q >= 0; --q)
-    if (o.log_pix[lb + q * 64] == pix) return Vov * (1.f - __expf(-(float)(tph - o.log_t[lb + q * 64]) * inv_tau));
+    if (o.log_pix[lb + q * 64] == pix) return Vov * (1.f - __expf(-(toff - o.log_t[lb + q * 64]) * inv_tau));
   return Vov;
 }
 // kb_sipm, one lane per SiPM: photons of the counter's steps in time order -> pixel, recovery, firing, cross-talk
@@ -350,6 +351,8 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
       const int nstr = b - a;
       for (int w = 0; w < CRV_DEAD_WORDS; ++w) bm[w * 64 + lane] = 0u;
       const int64_t lb = o.goff[gq >> 6] + (gq & 63), lcap = o.gcap[gq >> 6];
+      const double tbase = st.t[perm[a]];  // log times are FP32 offsets from it (they span microseconds)
+      o.tbase[g] = tbase;
       int64_t nlog = 0;
       // photon streams: in registers for a counter with one step, in global memory otherwise
       float S1 = 0.f;
@@ -404,6 +407,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           }
         }
         if (kb < 0 || tph > p.photon_window_hi) break;  // later photons of every stream are later still
+        const float toff = (float)(tph - tbase);
         // advance the stream
         uint32_t j;
         int r;
@@ -457,14 +461,14 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
         }
         const int pix = px * CRV_NPIX + py;
         if ((t.dead[pix >> 5] >> (pix & 31)) & 1u) continue;
-        const float v = pixel_v(bm, lane, pix, o, lb, nlog, tph, Vov, inv_tau);
+        const float v = pixel_v(bm, lane, pix, o, lb, nlog, toff, Vov, inv_tau);
         const float pf = v == Vov ? pf_full : p1 * (1.f - __expf(-v * inv_p2));
         if (!(u32f(u.w) < pf)) continue;
         if (nlog >= lcap) {
           bad = true;
           break;
         }
-        o.log_t[lb + nlog * 64] = tph;
+        o.log_t[lb + nlog * 64] = toff;
         o.log_v[lb + nlog * 64] = v;
         o.log_pix[lb + nlog * 64] = (int16_t)pix;
         ++nlog;
@@ -516,7 +520,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
             const int c = xcand[ai * xs];
             const int m = xmult[ai * xs];
             if ((t.dead[c >> 5] >> (c & 31)) & 1u) continue;
-            const float vc = pixel_v(bm, lane, c, o, lb, nlog, tph, Vov, inv_tau);
+            const float vc = pixel_v(bm, lane, c, o, lb, nlog, toff, Vov, inv_tau);
             const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
             float qm = q1;
             for (int mm = 2; mm <= m; ++mm) qm *= q1;
@@ -527,7 +531,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
               bad = true;
               break;
             }
-            o.log_t[lb + nlog * 64] = tph;
+            o.log_t[lb + nlog * 64] = toff;
             o.log_v[lb + nlog * 64] = vc;
             o.log_pix[lb + nlog * 64] = (int16_t)c;
             ++nlog;
@@ -747,6 +751,7 @@ __global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv
       L.nrun = 0;
       const int nf = nfired[g];
       const int64_t lb = o.goff[gq >> 6] + (gq & 63);
+      const double tb0 = nf > 0 ? o.tbase[g] : 0.0;
       L.td = 0.0;
       if (nf > 0) {
         const uint32_t fk = p.feb_key_roc ? (uint32_t)(t.roc[L.ch] * 1000 + t.feb[L.ch]) : (uint32_t)t.feb[L.ch];
@@ -759,10 +764,11 @@ __global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv
       int q = 0;
       while (q < nf) {
         // the charges of one time (a fired pixel and its cross-talk)
-        const double tq = o.log_t[lb + (int64_t)q * 64];
+        const float tqf = o.log_t[lb + (int64_t)q * 64];
+        const double tq = tb0 + (double)tqf;
         float vsum = o.log_v[lb + (int64_t)q * 64];
         int n = 1;
-        while (q + n < nf && o.log_t[lb + (int64_t)(q + n) * 64] == tq) {
+        while (q + n < nf && o.log_t[lb + (int64_t)(q + n) * 64] == tqf) {
           vsum += o.log_v[lb + (int64_t)(q + n) * 64];
           ++n;
         }
@@ -781,19 +787,16 @@ __global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv
 
 // kb_fit, one lane per reco waveform of the run pool: pedestal, peaks, Gumbel fits, pulses -> the run's slots; the
 // run's place in the output (SiPM's first run + run number) gets its pulse count and pool slot.
-__global__ __launch_bounds__(64) void kb_fit(crv_params p, Tables t, const uint64_t *__restrict__ task_key,
-                                             const int64_t *__restrict__ roff, Out o, int64_t nruns,
-                                             int *__restrict__ flags) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nruns) return;
+// the pulses of pool run r, with the lane's sample scratch y (stride ys)
+__device__ __forceinline__ void fit_run(const crv_params &p, const Tables &t, const uint64_t *__restrict__ task_key,
+                                        const int64_t *__restrict__ roff, const Out &o, int64_t r, float *y,
+                                        int64_t ys, int *__restrict__ flags) {
   const int64_t g = o.run_g[r];
   const int ch = (int)(task_key[g >> 2] & 8191u) * 4 + (int)(g & 3);
   const int n = o.run_n[r];
   const int64_t st = o.run_tdc[r];
   const int W = p.window_samples;
   const uint16_t *src = o.rec_adc + (int64_t)o.run_rec[r] * W + o.run_i0[r];
-  float *y = o.yscr + r;
-  const int64_t ys = o.run_cap;
   for (int q = 0; q < n; ++q) y[q * ys] = (float)src[q];
   const bool drop = p.ignore_channels && (t.status[ch] & 0b010) != 0;
   const float ped_db = (float)t.pedestal[ch];
@@ -942,8 +945,15 @@ __global__ __launch_bounds__(64) void kb_fit(crv_params p, Tables t, const uint6
   o.rmap[fr] = (int32_t)r;
 }
 
-// fired-pixel log capacity per lane of each group of 64 SiPMs in queue order: the largest of the group's photon
-// counts + a cross-talk margin; the group takes 64 times that
+__global__ __launch_bounds__(64) void kb_fit(crv_params p, Tables t, const uint64_t *__restrict__ task_key,
+                                             const int64_t *__restrict__ roff, Out o, int64_t nruns,
+                                             int *__restrict__ flags) {
+  const int64_t lanes = (int64_t)gridDim.x * blockDim.x, gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t r = gid; r < nruns; r += lanes) fit_run(p, t, task_key, roff, o, r, o.yscr + gid, lanes, flags);
+}
+
+// fired-pixel log capacity per lane of each group of 64 SiPMs in queue order, from the largest photon count of the
+// group (factor / 2 of it + 16; factor doubles on every re-run); the group takes 64 times that
 __global__ void kb_gcaps(const int64_t *__restrict__ nph, const int32_t *__restrict__ order, int64_t L4,
                          int64_t *__restrict__ gcap, int64_t *__restrict__ gsize, int factor) {
   const int64_t grp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -951,7 +961,7 @@ __global__ void kb_gcaps(const int64_t *__restrict__ nph, const int32_t *__restr
   int64_t m = 0;
   for (int64_t gq = grp * 64; gq < L4 && gq < grp * 64 + 64; ++gq) {
     const int64_t v = nph[order ? order[gq] : gq];
-    m = max(m, v + v * factor / 4 + 8);
+    m = max(m, v * factor / 2 + 16);  // fired pixels: about 45 % of the photons
   }
   gcap[grp] = m;
   gsize[grp] = m * 64;
@@ -1059,7 +1069,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   o.nrun = (int32_t *)ws(c, 513, L4 * 4 + 8);
   if (!Tdev || !nph || !sipm_nph || !gcap || !gsize || !goff || !skey || !skey2 || !sidx || !order || !nfired || !counters ||
       !o.xcur || !o.xcand || !o.xmult || !o.gS || !o.gHead || !o.gTb || !o.gR || !o.nrun)
-    return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    return ws_fail(c);
   o.rec_count = counters + 2;
   o.run_count = counters + 3;
   const int32_t T32 = (int32_t)T;
@@ -1075,7 +1085,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     size_t tb = 0;
     CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, skey, skey2, sidx, order, (int)L4, 0, 32, s));
     void *tmp = ws(c, 509, tb);
-    if (!tmp) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    if (!tmp) return ws_fail(c);
     CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, skey, skey2, sidx, order, (int)L4, 0, 32, s));
   }
   const Queue q_sipm{counters, tile, L4, sorted ? order : nullptr}, q_wave{counters + 1, tile, L4, sorted ? order : nullptr};
@@ -1083,16 +1093,17 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   int hf[F_NFLAGS];
   // phase 1: photons -> fired-pixel logs (a log that overflows is grown and the phase run again)
   int64_t nlog = 0;
-  for (int attempt = 0, factor = 1;; ++attempt, factor *= 4) {
+  for (int attempt = 0, factor = 1;; ++attempt, factor *= 2) {
     if (L4) kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, sorted ? order : nullptr, L4, gcap, gsize, factor);
     if ((rc = scan_i64(c, gsize, goff, ngrp, 510, s))) return rc;
     if ((rc = read_i64(c, goff + ngrp, &nlog, s))) return rc;
     o.goff = goff;
     o.gcap = gcap;
-    o.log_t = (double *)ws(c, 538, nlog * 8 + 8);
+    o.log_t = (float *)ws(c, 538, nlog * 4 + 8);
     o.log_v = (float *)ws(c, 546, nlog * 4 + 8);
     o.log_pix = (int16_t *)ws(c, 539, nlog * 2 + 8);
-    if (!o.log_t || !o.log_v || !o.log_pix) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    o.tbase = (double *)ws(c, 519, L4 * 8 + 8);
+    if (!o.log_t || !o.log_v || !o.log_pix || !o.tbase) return ws_fail(c);
     kb_zero<<<1, 64, 0, s>>>(counters, 1);
     kb_zero<<<1, 64, 0, s>>>(c->flags + F_OVERFLOW, 1);
     CRV_HIP(c, hipEventRecord(kev[0], s));
@@ -1108,19 +1119,19 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
                        "batched chain: cross-talk cascade wider than 16 pixels or deeper than 31 generations");
     if (hf[F_COUNTER]) return set_error(c, CRV_E_OVERFLOW, "batched chain: more than 65535 photoelectrons from one step");
     if (!hf[F_OVERFLOW]) break;
-    if (attempt >= 5) return set_error(c, CRV_E_OVERFLOW, "batched chain: fired-pixel log still too short");
+    if (attempt >= 8) return set_error(c, CRV_E_OVERFLOW, "batched chain: fired-pixel log still too short");
   }
   // photons drawn and pixels fired (diagnostics)
   int64_t *sums = (int64_t *)ws(c, 515, 16);
   int64_t *fired64 = (int64_t *)ws(c, 516, L4 * 8 + 8);
-  if (!sums || !fired64) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (!sums || !fired64) return ws_fail(c);
   int64_t hsum[2] = {0, 0};
   if (L4) {
     kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(nfired, fired64, L4);
     size_t tb = 0;
     CRV_HIP(c, hipcub::DeviceReduce::Sum(nullptr, tb, fired64, sums, (int)L4, s));
     void *tmp = ws(c, 517, tb);
-    if (!tmp) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    if (!tmp) return ws_fail(c);
     CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, fired64, sums, (int)L4, s));
     CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, sipm_nph, sums + 1, (int)L4, s));
     CRV_HIP(c, hipMemcpyAsync(hsum, sums, 16, hipMemcpyDeviceToHost, s));
@@ -1141,7 +1152,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     o.run_tdc = (int64_t *)ws(c, 578, o.run_cap * 8);
     if (!o.rec_adc || !o.rec_label || !o.rec_tick || !o.run_rec || !o.run_i0 || !o.run_n || !o.run_k || !o.run_g ||
         !o.run_tdc)
-      return set_error(c, CRV_E_HIP, "workspace allocation failed");
+      return ws_fail(c);
     kb_zero<<<1, 64, 0, s>>>(counters + 1, 3);
     CRV_HIP(c, hipEventRecord(kev[2], s));
     if (L4) kb_wave<<<grid_w, 64, 0, s>>>(p, c->t, c->d, g.task_key, nfired, o, q_wave, c->flags);
@@ -1157,14 +1168,15 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   const int64_t nrec = cnt[0], nruns = cnt[1];
   // the runs' places in the output: each SiPM's runs in run-number order
   int64_t *nrun64 = (int64_t *)ws(c, 579, (L4 + 1) * 8), *roff = (int64_t *)ws(c, 580, (L4 + 2) * 8);
-  if (!nrun64 || !roff) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (!nrun64 || !roff) return ws_fail(c);
   if (L4) kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(o.nrun, nrun64, L4);
   if ((rc = scan_i64(c, nrun64, roff, L4, 514, s))) return rc;
   // phase 3: pulse fits (slots grown and the fits run again when a waveform has more pulses)
-  o.yscr = (float *)ws(c, 581, o.run_cap * W * 4 + 8);
+  const int fit_blocks = (int)std::min<int64_t>(nblocks(nruns, 64), 8192);  // grid-stride; scratch per lane
+  o.yscr = (float *)ws(c, 581, (size_t)fit_blocks * 64 * W * 4 + 8);
   o.npr = (int32_t *)ws(c, 582, nruns * 4 + 8);
   o.rmap = (int32_t *)ws(c, 583, nruns * 4 + 8);
-  if (!o.yscr || !o.npr || !o.rmap) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (!o.yscr || !o.npr || !o.rmap) return ws_fail(c);
   o.pcap = opt->pulse_slots > 0 ? opt->pulse_slots : PSLOTS;
   for (int attempt = 0;; ++attempt) {
     const int64_t ns = nruns * o.pcap + 8;
@@ -1174,10 +1186,10 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     bool okp = o.tp && o.mu;
     for (int i = 0; i < 7; ++i) okp = okp && (*fp[i] = (float *)ws(c, 522 + i, ns * 4));
     o.sf = (uint32_t *)ws(c, 529, ns * 4);
-    if (!okp || !o.sf) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    if (!okp || !o.sf) return ws_fail(c);
     kb_zero<<<1, 64, 0, s>>>(c->flags + F_PSLOT, 1);
     CRV_HIP(c, hipEventRecord(kev[4], s));
-    if (nruns) kb_fit<<<nblocks(nruns, 64), 64, 0, s>>>(p, c->t, g.task_key, roff, o, nruns, c->flags);
+    if (nruns) kb_fit<<<fit_blocks, 64, 0, s>>>(p, c->t, g.task_key, roff, o, nruns, c->flags);
     CRV_HIP(c, hipGetLastError());
     CRV_HIP(c, hipEventRecord(kev[5], s));
     CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
@@ -1189,7 +1201,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   CRV_HIP(c, hipEventRecord(ev[2], s));
   // pulse table
   int64_t *npr64 = (int64_t *)ws(c, 541, (nruns + 1) * 8), *poff = (int64_t *)ws(c, 542, (nruns + 2) * 8);
-  if (!npr64 || !poff) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (!npr64 || !poff) return ws_fail(c);
   if (nruns) kb_widen<<<nblocks(nruns, 256), 256, 0, s>>>(o.npr, npr64, nruns);
   if ((rc = scan_i64(c, npr64, poff, nruns, 511, s))) return rc;
   int64_t np = 0;
@@ -1208,7 +1220,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   bool okp = P.chan && pbar && psipm;
   for (int i = 0; i < 3; ++i) okp = okp && *ip[i] && *bp[i];
   for (int i = 0; i < 11; ++i) okp = okp && *dp[i];
-  if (!okp) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (!okp) return ws_fail(c);
   P.capacity = np;
   P.n = np;
   if (nruns) kb_pulses_out<<<nblocks(nruns, 256), 256, 0, s>>>(p, c->t, nruns, g.task_key, poff, o, P, pbar, psipm);

@@ -1080,7 +1080,7 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     out->n_mem = 0;
     if (out->capacity < 0) {
       out->cl_mem_off = (int64_t *)ws(c, 352, 8);
-      if (!out->cl_mem_off) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+      if (!out->cl_mem_off) return ws_fail(c);
       out->capacity = 0;
       out->mem_capacity = 0;
     }
@@ -1124,7 +1124,7 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
   int64_t *clo = (int64_t *)ws(c, 122, (nev + 1) * 8);
   int64_t *mo = (int64_t *)ws(c, 123, (nev + 1) * 8);
   if (!sc.kept || !ncl || !nmem || !cl_ev || !cl_type || !cl_pc || !cl_sh || !cl_d || !cl_moff || !mem || !clo || !mo)
-    return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    return ws_fail(c);
   Ev e{bar, sipm, seq};
   CRV_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int) * F_NFLAGS, s));
   kmark(c, 3, 0, s);

@@ -411,7 +411,7 @@ int pulses_stage(Ctx *c, const int64_t *ev, const int32_t *chan, const int64_t *
   int64_t *poff = (int64_t *)ws(c, 62, (n_wf + 1) * 8);
   int32_t *npul = (int32_t *)ws(c, 63, (n_wf + 1) * 4);
   int64_t *pout = (int64_t *)ws(c, 64, (n_wf + 1) * 8);
-  if (!yscr || !cap || !poff || !npul || !pout) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (!yscr || !cap || !poff || !npul || !pout) return ws_fail(c);
   k_pulse_caps<<<nblocks(n_wf, 256), 256, 0, s>>>(n_wf, nsamp, cap);
   if ((rc = scan_i64(c, cap, poff, n_wf, 65, s))) return rc;
   int64_t ncap = 0;
@@ -421,12 +421,12 @@ int pulses_stage(Ctx *c, const int64_t *ev, const int32_t *chan, const int64_t *
   double **dd[] = {&sl.pe, &sl.peph, &sl.time, &sl.height, &sl.beta, &sl.chi2, &sl.letime, &sl.ped, &sl.fa, &sl.fmu, &sl.fb};
   for (int i = 0; i < 11; ++i) {
     *dd[i] = (double *)ws(c, 67 + i, (ncap + 1) * 8);
-    if (!*dd[i]) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    if (!*dd[i]) return ws_fail(c);
   }
   sl.failed = (uint8_t *)ws(c, 78, ncap + 8);
   sl.zerondf = (uint8_t *)ws(c, 79, ncap + 8);
   sl.pedfromdb = (uint8_t *)ws(c, 80, ncap + 8);
-  if (!sl.seq || !sl.failed || !sl.zerondf || !sl.pedfromdb) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (!sl.seq || !sl.failed || !sl.zerondf || !sl.pedfromdb) return ws_fail(c);
   kmark(c, 2, 0, s);
   k_pulses<<<nblocks(n_wf, 64), 64, 0, s>>>(c->p, c->t, n_wf, chan, start, nsamp, off, adc, ped, area, height, toff,
                                             yscr, poff, sl, npul);

@@ -343,7 +343,7 @@ int group_steps(Ctx *c, const crv_steps *st, StepGroups &g, hipStream_t s) {
   g.task_off = (int32_t *)ws(c, 7, (n + 1) * 4);
   int32_t *nruns = (int32_t *)ws(c, 8, 8);
   if (!key || !key_s || !idx || !g.perm || !g.evfirst || !g.task_key || !cnt || !g.task_off || !nruns)
-    return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    return ws_fail(c);
   CRV_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int) * F_NFLAGS, s));
   k_step_keys<<<nblocks(n, 256), 256, 0, s>>>(st->event, st->counter, n, key, idx, g.evfirst, c->flags);
   CRV_HIP(c, hipGetLastError());
@@ -392,7 +392,7 @@ int charges_stage(Ctx *c, StepGroups &g, crv_charges *out, hipStream_t s) {
   gen.tb = (double *)ws(c, 18, g.n_steps * 4 * 8);
   gen.r = (int32_t *)ws(c, 19, g.n_steps * 4 * 4);
   if (!nph || !sipm_nph || !cap || !coff || !ccount || !ooff || !gen.S || !gen.head || !gen.tb || !gen.r)
-    return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    return ws_fail(c);
   CRV_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int) * F_NFLAGS, s));
   k_photon_count<<<nblocks(L, 256), 256, 0, s>>>(c->p, c->t, c->d, st, T, g.task_key, g.task_off, g.perm, g.evfirst,
                                                  nph, sipm_nph, c->flags);

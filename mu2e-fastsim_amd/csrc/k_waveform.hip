@@ -251,7 +251,7 @@ int digis_stage(Ctx *c, const int64_t *label, const double *time, const double *
   int32_t *seg_cnt = (int32_t *)ws(c, 31, (n + 1) * 4);
   int32_t *nruns = (int32_t *)ws(c, 32, 8);
   int64_t *soff = (int64_t *)ws(c, 33, (n + 1) * 8);
-  if (!seg_label || !seg_cnt || !nruns || !soff) return set_error(c, CRV_E_HIP, "workspace allocation failed");
+  if (!seg_label || !seg_cnt || !nruns || !soff) return ws_fail(c);
   size_t tb = 0;
   CRV_HIP(c, hipcub::DeviceRunLengthEncode::Encode(nullptr, tb, label, seg_label, seg_cnt, nruns, (int)n, s));
   void *tmp = ws(c, 34, tb);
@@ -289,7 +289,7 @@ int digis_stage(Ctx *c, const int64_t *label, const double *time, const double *
   int64_t *rout = (int64_t *)ws(c, 53, (nrec + 1) * 8);
   if (!rec_adc || !rec_has || !rec_tick || !rec_tdc0 || !ds.rec || !ds.sample || !ds.start || !ndig || !dout ||
       !rflag || !rout)
-    return set_error(c, CRV_E_HIP, "workspace allocation failed");
+    return ws_fail(c);
   CRV_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int) * F_NFLAGS, s));
   kmark(c, 1, 0, s);
   k_wf_synth<<<nblocks(nseg, 64), 64, 0, s>>>(p, c->t, c->d.nreach, nseg, seg_label, soff, time, charge, roff, doff,
