@@ -38,7 +38,7 @@ WAVEFORM_META_B = 40   # start, length, offset, calibration of a reco waveform
 PULSE_IN_B = 40        # event, bar, sipm, time, pe, seq read by the finder
 # FP32 batched chain kernels
 SIPM_STEP_B = 20       # per (step, SiPM) read by kb_sipm: photon count i32, z f64, t f64
-FIRE_B = 10            # fired-pixel log entry written by kb_sipm: time offset f32, overvoltage f32, pixel i16
+FIRE_B = 8             # fired-pixel log entry written by kb_sipm: time offset f32, overvoltage f32
 FIRE_IN_B = 8          # log entry read by kb_wave: time offset, overvoltage
 RUN_B = 32             # reco-waveform descriptor: record, first sample, samples, run number i32; SiPM, start TDC i64
 SLOT_B = 48            # pulse slot written by kb_fit: time, location f64; 7 x f32; flags u32

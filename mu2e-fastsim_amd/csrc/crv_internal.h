@@ -30,6 +30,7 @@ struct Tables {
   const uint32_t *dead = nullptr;  // CRV_DEAD_WORDS words, bit pix = x * 41 + y
   // finder geometry
   int32_t nbars = 0, nsectors = 0, ntypes = 0;
+  int32_t ncit = 0;  // 1 + the largest counter_in_type
   const int32_t *sector = nullptr, *layer = nullptr, *cit = nullptr;
   const double *bx = nullptr, *by = nullptr, *bpos = nullptr;
   const int32_t *stype = nullptr, *ldir = nullptr;

@@ -344,9 +344,12 @@ int crv_upload_geometry(crv_ctx *ctx, int32_t nbars, const int32_t *sector, cons
   if (int rc = guard(ctx)) return rc;
   Ctx *c = &ctx->c;
   if (nbars <= 0 || nsectors <= 0) return set_error(c, CRV_E_ARG, "geometry: bad sizes");
+  int32_t ncit = 0;
   for (int i = 0; i < nbars; ++i) {
     if (sector[i] < 0 || sector[i] >= nsectors) return set_error(c, CRV_E_ARG, "geometry: bad sector index");
     if (layer[i] < 0 || layer[i] >= 4) return set_error(c, CRV_E_ARG, "geometry: layer must be 0..3");
+    if (counter_in_type[i] < 0) return set_error(c, CRV_E_ARG, "geometry: negative counter_in_type");
+    ncit = std::max(ncit, counter_in_type[i] + 1);
   }
   // sector-type limits (reco/coinc.py:112-122): over every sector of the type; types without cuts are dropped
   std::map<int, std::vector<int>> bytype;
@@ -397,6 +400,7 @@ int crv_upload_geometry(crv_ctx *ctx, int32_t nbars, const int32_t *sector, cons
   c->t.nbars = nbars;
   c->t.nsectors = nsectors;
   c->t.ntypes = (int32_t)tid.size();
+  c->t.ncit = ncit;
   return CRV_OK;
 }
 

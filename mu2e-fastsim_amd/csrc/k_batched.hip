@@ -255,7 +255,7 @@ struct Out {
   float *log_t;     // time - tbase of the SiPM
   double *tbase;     // per SiPM: the time of its first step
   float *log_v;
-  int16_t *log_pix;
+  float *lastt;      // per persistent lane: time offset of each pixel's last firing, [pixel][grid lane]
   // photon streams of the (step, SiPM) pairs of counters with several steps
   float *gS;
   double *gHead, *gTb;
@@ -306,13 +306,11 @@ __device__ __forceinline__ bool bget(const uint32_t *bm, int lane, int pix) {
 }
 __device__ __forceinline__ void bset(uint32_t *bm, int lane, int pix) { bm[(pix >> 5) * 64 + lane] |= 1u << (pix & 31); }
 
-// overvoltage of a pixel at time offset toff: from its last firing in the lane's log (the bitmap says whether there is one)
-__device__ __forceinline__ float pixel_v(const uint32_t *bm, int lane, int pix, const Out &o, int64_t lb, int64_t nlog,
+// overvoltage of a pixel at time offset toff: from its last firing (the bitmap says whether it fired in this SiPM)
+__device__ __forceinline__ float pixel_v(const uint32_t *bm, int lane, int pix, const float *lastt, int64_t xs,
                                          float toff, float Vov, float inv_tau) {
   if (!bget(bm, lane, pix)) return Vov;
-  for (int64_t q = nlog - 1; q >= 0; --q)
-    if (o.log_pix[lb + q * 64] == pix) return Vov * (1.f - __expf(-(toff - o.log_t[lb + q * 64]) * inv_tau));
-  return Vov;
+  return Vov * (1.f - __expf(-(toff - lastt[pix * xs]) * inv_tau));
 }
 // kb_sipm, one lane per SiPM: photons of the counter's steps in time order -> pixel, recovery, firing, cross-talk
 // cascade -> the SiPM's fired-pixel log (time, overvoltage, pixel), in firing order.
@@ -334,6 +332,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
   uint16_t *xcur = o.xcur + xb, *xcand = o.xcand + xb;
   uint8_t *xmult = o.xmult + xb;
   const int64_t xs = o.xstride;
+  float *lastt = o.lastt + xb;  // read only where the pixel's bitmap bit is set: no clearing between SiPMs
   int64_t base = 0;
   while (grab(qu, &tile_base, base)) {
     for (int64_t gq = base + lane; gq < base + qu.tile && gq < qu.n; gq += 64) {
@@ -461,7 +460,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
         }
         const int pix = px * CRV_NPIX + py;
         if ((t.dead[pix >> 5] >> (pix & 31)) & 1u) continue;
-        const float v = pixel_v(bm, lane, pix, o, lb, nlog, toff, Vov, inv_tau);
+        const float v = pixel_v(bm, lane, pix, lastt, xs, toff, Vov, inv_tau);
         const float pf = v == Vov ? pf_full : p1 * (1.f - __expf(-v * inv_p2));
         if (!(u32f(u.w) < pf)) continue;
         if (nlog >= lcap) {
@@ -470,7 +469,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
         }
         o.log_t[lb + nlog * 64] = toff;
         o.log_v[lb + nlog * 64] = v;
-        o.log_pix[lb + nlog * 64] = (int16_t)pix;
+        lastt[pix * xs] = toff;
         ++nlog;
         bset(bm, lane, pix);
         // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order
@@ -520,7 +519,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
             const int c = xcand[ai * xs];
             const int m = xmult[ai * xs];
             if ((t.dead[c >> 5] >> (c & 31)) & 1u) continue;
-            const float vc = pixel_v(bm, lane, c, o, lb, nlog, toff, Vov, inv_tau);
+            const float vc = pixel_v(bm, lane, c, lastt, xs, toff, Vov, inv_tau);
             const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
             float qm = q1;
             for (int mm = 2; mm <= m; ++mm) qm *= q1;
@@ -533,7 +532,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
             }
             o.log_t[lb + nlog * 64] = toff;
             o.log_v[lb + nlog * 64] = vc;
-            o.log_pix[lb + nlog * 64] = (int16_t)c;
+            lastt[c * xs] = toff;
             ++nlog;
             bset(bm, lane, c);
             xcur[ncur * xs] = (uint16_t)c;
@@ -1062,13 +1061,14 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   o.xcur = (uint16_t *)ws(c, 543, (size_t)o.xstride * CMAXB * 2);
   o.xcand = (uint16_t *)ws(c, 544, (size_t)o.xstride * CMAXB * 2);
   o.xmult = (uint8_t *)ws(c, 545, (size_t)o.xstride * CMAXB);
+  o.lastt = (float *)ws(c, 539, (size_t)o.xstride * CRV_NPIX * CRV_NPIX * 4);
   o.gS = (float *)ws(c, 534, n * 16 + 16);
   o.gHead = (double *)ws(c, 535, n * 32 + 16);
   o.gTb = (double *)ws(c, 536, n * 32 + 16);
   o.gR = (int32_t *)ws(c, 537, n * 16 + 16);
   o.nrun = (int32_t *)ws(c, 513, L4 * 4 + 8);
   if (!Tdev || !nph || !sipm_nph || !gcap || !gsize || !goff || !skey || !skey2 || !sidx || !order || !nfired || !counters ||
-      !o.xcur || !o.xcand || !o.xmult || !o.gS || !o.gHead || !o.gTb || !o.gR || !o.nrun)
+      !o.xcur || !o.xcand || !o.xmult || !o.lastt || !o.gS || !o.gHead || !o.gTb || !o.gR || !o.nrun)
     return ws_fail(c);
   o.rec_count = counters + 2;
   o.run_count = counters + 3;
@@ -1101,9 +1101,8 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     o.gcap = gcap;
     o.log_t = (float *)ws(c, 538, nlog * 4 + 8);
     o.log_v = (float *)ws(c, 546, nlog * 4 + 8);
-    o.log_pix = (int16_t *)ws(c, 539, nlog * 2 + 8);
     o.tbase = (double *)ws(c, 519, L4 * 8 + 8);
-    if (!o.log_t || !o.log_v || !o.log_pix || !o.tbase) return ws_fail(c);
+    if (!o.log_t || !o.log_v || !o.tbase) return ws_fail(c);
     kb_zero<<<1, 64, 0, s>>>(counters, 1);
     kb_zero<<<1, 64, 0, s>>>(c->flags + F_OVERFLOW, 1);
     CRV_HIP(c, hipEventRecord(kev[0], s));

@@ -1,5 +1,5 @@
 // Coincidence finder: pulses -> coincidence clusters (FP64): one wavefront per event of up to 256 pulses (the
-// event's hits in LDS), one lane per larger event.
+// event's hits in LDS), one 256-thread block per larger event (scratch in HBM, bucketed and binned searches).
 //
 // Follows the reference's reco/coinc.py:112-392 and reco/coinc_cpu.py:11-51 step by step: hits from first pulses
 // with the fibre-length time correction and partner-status PE compensation; filterHits per readout side (same and
@@ -8,7 +8,7 @@
 // slopes, slope differences, maxDistance widened to the x gaps of passing combinations, kept hits in pulse index
 // order); findClusters again across sides; opposite-side attachment (nearest cluster per candidate, one pulse per
 // SiPM in (cluster, bar, sipm, dt, candidate) order); cluster moments as sequential sums in member order.
-// Lists live in per-event scratch ranges of the pulse arrays.
+// Outputs go to event-local slots (the event's first pulse onwards), compacted by k_coinc_out.
 #include <hipcub/hipcub.hpp>
 
 #include "crv_internal.h"
@@ -27,186 +27,10 @@ __device__ __forceinline__ Cut cut_of(const Tables &t, int sector) {
   return Cut{c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], c[9]};
 }
 
-struct Scr {  // per-pulse scratch (indexed from the event's first pulse)
-  double *t, *pe, *maxd, *wmaxd, *dt;
-  int32_t *hl, *filt, *rem, *items, *cl_of, *coinc, *bylayer, *chosen_c, *chosen_k, *cl2_off, *cl2_items;
-  uint8_t *kept;
-};
-
 struct Ev {  // what the finder needs per pulse: bar, sipm, seq, idx and the derived hit fields
   const int32_t *bar, *sipm;
   const int64_t *seq;
 };
-
-// link test of reco/coinc.py:142-146
-__device__ __forceinline__ bool linked(const Tables &t, const Ev &e, const Scr &s, int h, int c, const double *maxd,
-                                       double max_dt, bool opposite, double dt_opp) {
-  double md = fmax(maxd[h], maxd[c]);
-  int sh = e.sipm[h] % 2, sc = e.sipm[c] % 2;
-  double mt = max_dt + ((opposite && sh != sc) ? dt_opp : 0.0);
-  return fabs(t.bx[e.bar[h]] - t.bx[e.bar[c]]) <= md && fabs(s.t[h] - s.t[c]) < mt;
-}
-
-// greedy findClusters of list[0..n): writes members in join order to out[0..n) and cluster starts to off;
-// returns the number of clusters. rem is scratch of size n.
-__device__ int find_clusters(const Tables &t, const Ev &e, const Scr &s, const int32_t *list, int n,
-                             const double *maxd, double max_dt, bool opposite, double dt_opp, int32_t *rem,
-                             int32_t *out, int32_t *off) {
-  for (int i = 0; i < n; ++i) rem[i] = list[i];
-  int nrem = n, nout = 0, ncl = 0;
-  while (nrem > 0) {
-    int cs = nout;
-    off[ncl] = cs;
-    int last = -1;
-    while (last != nout - cs) {
-      last = nout - cs;
-      int w = 0;
-      for (int i = 0; i < nrem; ++i) {
-        int h = rem[i];
-        bool join = nout == cs;
-        for (int m = cs; m < nout && !join; ++m) join = linked(t, e, s, h, out[m], maxd, max_dt, opposite, dt_opp);
-        if (join) out[nout++] = h; else rem[w++] = h;
-      }
-      nrem = w;
-    }
-    ++ncl;
-  }
-  off[ncl] = nout;
-  return ncl;
-}
-
-// checkCoincidence (reco/coinc.py:184-207) of cluster members m[0..n): appends kept hits (pulse index order) to
-// coinc with their widened max distance; returns the new coinc length.
-__device__ int check_coincidence(const crv_params &p, const Tables &t, const Ev &e, const Scr &s, const int32_t *m,
-                                 int n, int32_t *coinc, int nco, double *cmaxd, int32_t *byl, double *wmaxd,
-                                 uint8_t *kept) {
-  if (n == 0) return nco;
-  int lay_off[5] = {0, 0, 0, 0, 0};
-  int nmin = 1 << 30, nmax = -1;
-  for (int i = 0; i < n; ++i) {
-    Cut c = cut_of(t, t.sector[e.bar[m[i]]]);
-    int cl = (int)c.coincidence_layers;
-    nmin = cl < nmin ? cl : nmin;
-    nmax = cl > nmax ? cl : nmax;
-  }
-  // by_layer lists in cluster order; wmaxd / kept indexed by position in byl
-  int pos = 0;
-  for (int l = 0; l < 4; ++l) {
-    lay_off[l] = pos;
-    for (int i = 0; i < n; ++i)
-      if (t.layer[e.bar[m[i]]] == l) {
-        byl[pos] = m[i];
-        wmaxd[pos] = cut_of(t, t.sector[e.bar[m[i]]]).final_max_distance;
-        kept[pos] = 0;
-        ++pos;
-      }
-  }
-  lay_off[4] = pos;
-  int nonempty = 0;
-  for (int l = 0; l < 4; ++l) nonempty += lay_off[l + 1] > lay_off[l];
-  if (n > p.big_cluster_threshold && nonempty >= nmin) {
-    for (int i = 0; i < n; ++i) {
-      coinc[nco] = m[i];
-      cmaxd[nco] = s.maxd[m[i]];
-      ++nco;
-    }
-    return nco;
-  }
-  for (int nl = 2; nl <= 4; ++nl) {
-    bool need = nl == 2 ? nmin == 2 : nl == 3 ? (nmin <= 3 && 3 <= nmax) : nmax == 4;
-    if (!need) continue;
-    // layer combinations in lexicographic order
-    int L[4];
-    for (int i = 0; i < nl; ++i) L[i] = i;
-    for (;;) {
-      bool empty = false;
-      for (int i = 0; i < nl; ++i) empty |= lay_off[L[i] + 1] == lay_off[L[i]];
-      if (!empty) {
-        int idx[4];
-        for (int i = 0; i < nl; ++i) idx[i] = lay_off[L[i]];
-        for (;;) {
-          // combo = byl[idx[0..nl)]
-          bool skip = true;
-          double mdt = -INFINITY, lo = INFINITY, hi = -INFINITY, msd = -INFINITY;
-          double tmin = INFINITY, tmax = -INFINITY;
-          for (int i = 0; i < nl; ++i) {
-            int h = byl[idx[i]];
-            Cut c = cut_of(t, t.sector[e.bar[h]]);
-            if (!((int)c.coincidence_layers > nl)) skip = false;
-            mdt = fmax(mdt, c.max_dt);
-            lo = fmin(lo, c.min_slope);
-            hi = fmax(hi, c.max_slope);
-            msd = fmax(msd, c.max_slope_diff);
-            tmin = fmin(tmin, s.t[h]);
-            tmax = fmax(tmax, s.t[h]);
-          }
-          bool pass = !skip && !(tmax - tmin > mdt);
-          double sl[3];
-          for (int i = 0; pass && i + 1 < nl; ++i) {
-            int a = byl[idx[i]], b = byl[idx[i + 1]];
-            double v = (t.bx[e.bar[b]] - t.bx[e.bar[a]]) / (t.by[e.bar[b]] - t.by[e.bar[a]]);
-            if (v < lo || v > hi) pass = false;
-            sl[i] = v;
-          }
-          if (pass && nl > 2 && fabs(sl[0] - sl[1]) > msd) pass = false;
-          if (pass && nl > 3 && (fabs(sl[0] - sl[2]) > msd || fabs(sl[1] - sl[2]) > msd)) pass = false;
-          if (pass) {
-            // stable sort of the combo by x, then widen to the neighbour gaps
-            int ord[4];
-            for (int i = 0; i < nl; ++i) ord[i] = i;
-            for (int i = 1; i < nl; ++i) {
-              int v = ord[i], j = i - 1;
-              double xv = t.bx[e.bar[byl[idx[v]]]];
-              while (j >= 0 && t.bx[e.bar[byl[idx[ord[j]]]]] > xv) {
-                ord[j + 1] = ord[j];
-                --j;
-              }
-              ord[j + 1] = v;
-            }
-            for (int i = 0; i < nl; ++i) {
-              int q = idx[ord[i]];
-              double x = t.bx[e.bar[byl[q]]];
-              if (i > 0) wmaxd[q] = fmax(wmaxd[q], x - t.bx[e.bar[byl[idx[ord[i - 1]]]]]);
-              if (i + 1 < nl) wmaxd[q] = fmax(wmaxd[q], t.bx[e.bar[byl[idx[ord[i + 1]]]]] - x);
-              kept[q] = 1;
-            }
-          }
-          // next product element (last list fastest)
-          int k = nl - 1;
-          while (k >= 0) {
-            if (++idx[k] < lay_off[L[k] + 1]) break;
-            idx[k] = lay_off[L[k]];
-            --k;
-          }
-          if (k < 0) break;
-        }
-      }
-      // next layer combination
-      int k = nl - 1;
-      while (k >= 0 && L[k] == 4 - nl + k) --k;
-      if (k < 0) break;
-      ++L[k];
-      for (int i = k + 1; i < nl; ++i) L[i] = L[i - 1] + 1;
-    }
-  }
-  // kept hits in pulse index order (the event's pulse rows are in index order)
-  int first = nco;
-  for (int q = 0; q < pos; ++q) {
-    if (!kept[q]) continue;
-    int h = byl[q];
-    int at = nco;
-    while (at > first && coinc[at - 1] > h) {
-      coinc[at] = coinc[at - 1];
-      cmaxd[at] = cmaxd[at - 1];
-      --at;
-    }
-    coinc[at] = h;
-    cmaxd[at] = wmaxd[q];
-    ++nco;
-  }
-  return nco;
-}
-
 
 struct ClSlots {  // per-cluster outputs of the finder kernels (event-local slots)
   int64_t *cl_ev;
@@ -308,174 +132,6 @@ __device__ bool cluster_emit(const crv_params &p, const Tables &t, const Acc &A,
   d[10] = st[0];
   d[11] = st[1];
   return true;
-}
-
-struct SerialAcc {
-  const Ev &E;
-  const Scr &s;
-  __device__ double pe(int h) const { return s.pe[h]; }
-  __device__ double tm(int h) const { return s.t[h]; }
-  __device__ int bar(int h) const { return E.bar[h]; }
-  __device__ int sipm(int h) const { return E.sipm[h]; }
-};
-
-__global__ void k_coinc(crv_params p, Tables t, int64_t nev, const int64_t *__restrict__ eoff,
-                        const int64_t *__restrict__ evk, const double *__restrict__ ptime,
-                        const double *__restrict__ ppe, Ev e, Scr s0, int32_t *__restrict__ ncl,
-                        int32_t *__restrict__ nmem, int64_t *__restrict__ cl_ev, int32_t *__restrict__ cl_type,
-                        int32_t *__restrict__ cl_poscalc, int32_t *__restrict__ cl_sidehits,
-                        double *__restrict__ cl_d /* [slot * 12] */, int32_t *__restrict__ cl_moff,
-                        int32_t *__restrict__ mem, int64_t skip_n, int *__restrict__ flags) {
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nev) return;
-  const int64_t a = eoff[g], n = eoff[g + 1] - a;
-  if (n <= skip_n) return;  // taken by the wavefront kernel
-  const int64_t evkey = evk[g];
-  // event-local views: scratch ranges start at a + g, so every event owns n + 1 entries
-  const int64_t sb = a + g;
-  Ev E{e.bar + a, e.sipm + a, e.seq + a};
-  Scr s = s0;
-  s.t += sb; s.pe += sb; s.maxd += sb; s.wmaxd += sb; s.dt += sb;
-  s.hl += sb; s.filt += sb; s.rem += sb; s.items += sb; s.cl_of += sb; s.coinc += sb; s.bylayer += sb;
-  s.chosen_c += sb; s.chosen_k += sb; s.cl2_off += sb; s.cl2_items += sb; s.kept += sb;
-  int32_t *cl_moff_e = cl_moff + sb;  // cluster member offsets
-  const ClSlots slots{cl_ev, cl_type, cl_poscalc, cl_sidehits, cl_d};
-  int32_t *mem_e = mem + a;
-  int nclus = 0, nm = 0;
-  // hits: per sector type, in pulse order (types ascending)
-  for (int ti = 0; ti < t.ntypes; ++ti) {
-    const int type = t.type_id[ti];
-    const double P0 = t.tp0[ti], P1 = t.tp1[ti], half = t.thalf[ti], tmaxdt = t.tmaxdt[ti];
-    const bool both = t.tside0[ti] && t.tside1[ti];
-    int nh = 0;
-    for (int k = 0; k < n; ++k) {
-      if (p.ignore_secondary_pulses && E.seq[k] > 0) continue;
-      int bar = E.bar[k];
-      if (bar < 0 || bar >= t.nbars || E.sipm[k] < 0 || E.sipm[k] > 3 || bar * 4 + 3 >= t.n_channels) {
-        atomicAdd(&flags[F_BADCHAN], 1);
-        continue;
-      }
-      int sec = t.sector[bar];
-      if (!t.hascuts[sec] || t.stype[sec] != type) continue;
-      int sipm = E.sipm[k];
-      double tt = ptime[a + k];
-      if (sipm % 2 == 0) tt += (t.ps0[sec] - P0) / p.fiber_signal_speed;
-      else tt += (P1 - t.ps1[sec]) / p.fiber_signal_speed;
-      double pe = ppe[a + k];
-      int other = bar * 4 + sipm + (sipm < 2 ? 2 : -2);
-      if (t.status[other] & p.compensate_mask) pe *= 2.0;
-      s.t[k] = tt;
-      s.pe[k] = pe;
-      s.maxd[k] = cut_of(t, sec).initial_max_distance;
-      s.hl[nh++] = k;
-    }
-    if (nh == 0) continue;
-    int nco = 0, nfilt = 0;
-    for (int side = 0; side < 2; ++side) {
-      int f0 = nfilt;
-      for (int ii = 0; ii < nh; ++ii) {
-        int h = s.hl[ii];
-        if (E.sipm[h] % 2 != side) continue;
-        Cut ch = cut_of(t, t.sector[E.bar[h]]);
-        int lh = t.layer[E.bar[h]], cnh = t.cit[E.bar[h]];
-        double same = 0.0, adj1 = 0.0, adj2 = 0.0;
-        for (int jj = 0; jj < nh; ++jj) {
-          int q = s.hl[jj];
-          if (E.sipm[q] % 2 != side || t.layer[E.bar[q]] != lh || fabs(s.t[q] - s.t[h]) > ch.max_dt_adjacent) continue;
-          int d = t.cit[E.bar[q]] - cnh;
-          same += d == 0 ? s.pe[q] : 0.0;
-          adj1 += d == -1 ? s.pe[q] : 0.0;
-          adj2 += d == 1 ? s.pe[q] : 0.0;
-        }
-        if (same + adj1 >= ch.pe_threshold || same + adj2 >= ch.pe_threshold) s.filt[nfilt++] = h;
-      }
-      // clusters of this side's filtered hits, then the coincidence check per cluster
-      int ncs = find_clusters(t, E, s, s.filt + f0, nfilt - f0, s.maxd, tmaxdt, false, 0.0, s.rem, s.items,
-                              s.cl_of);
-      for (int c = 0; c < ncs; ++c) {
-        int c0 = s.cl_of[c], c1 = s.cl_of[c + 1];
-        nco = check_coincidence(p, t, E, s, s.items + c0, c1 - c0, s.coinc, nco, s.dt, s.bylayer, s.wmaxd, s.kept);
-      }
-    }
-    const double dt_opp = 2.0 * half / p.fiber_signal_speed;
-    // coinc hits carry their widened max distance in s.dt; cluster across sides
-    for (int i = 0; i < nco; ++i) s.wmaxd[s.coinc[i]] = s.dt[i];
-    int ncl2 = find_clusters(t, E, s, s.coinc, nco, s.wmaxd, tmaxdt, both, dt_opp, s.rem, s.cl2_items, s.cl2_off);
-    int nitems = s.cl2_off[ncl2];
-    // attachment (reco/coinc_cpu.py:11-51): appended per cluster after its members
-    int nchosen = 0;
-    if (both && p.add_opposite_side_pulses && nitems > 0 && nfilt > 0) {
-      const double mdt = tmaxdt + dt_opp;
-      for (int ci = 0; ci < nfilt; ++ci) {
-        int h = s.filt[ci];
-        if (E.seq[h] != 0) continue;
-        bool member = false;
-        for (int m = 0; m < nitems && !member; ++m) member = s.cl2_items[m] == h;
-        if (member) continue;
-        int hb = E.bar[h], hs = E.sipm[h] % 2;
-        int best_c = -1;
-        double best_dt = 0.0;
-        for (int c = 0; c < ncl2; ++c)
-          for (int m = s.cl2_off[c]; m < s.cl2_off[c + 1]; ++m) {
-            int an = s.cl2_items[m];
-            if (E.seq[an] != 0 || E.bar[an] != hb || hs != 1 - E.sipm[an] % 2) continue;
-            double dt = fabs(s.t[h] - s.t[an]);
-            if (!(dt < mdt)) continue;
-            bool occ = false;
-            for (int q = s.cl2_off[c]; q < s.cl2_off[c + 1] && !occ; ++q)
-              occ = E.bar[s.cl2_items[q]] == hb && E.sipm[s.cl2_items[q]] % 2 == hs;
-            if (occ) continue;
-            if (best_c < 0 || dt < best_dt || (dt == best_dt && c < best_c)) {
-              best_c = c;
-              best_dt = dt;
-            }
-          }
-        if (best_c < 0) continue;
-        // insert in (cluster, bar, sipm, dt, candidate) order
-        int at = nchosen;
-        while (at > 0) {
-          int pc = s.chosen_k[at - 1], ph = s.filt[s.chosen_c[at - 1]];
-          double pdt = s.dt[at - 1];
-          bool greater = pc > best_c ||
-                         (pc == best_c && (E.bar[ph] > hb || (E.bar[ph] == hb && (E.sipm[ph] > E.sipm[h] ||
-                         (E.sipm[ph] == E.sipm[h] && (pdt > best_dt || (pdt == best_dt && s.chosen_c[at - 1] > ci)))))));
-          if (!greater) break;
-          s.chosen_k[at] = s.chosen_k[at - 1];
-          s.chosen_c[at] = s.chosen_c[at - 1];
-          s.dt[at] = s.dt[at - 1];
-          --at;
-        }
-        s.chosen_k[at] = best_c;
-        s.chosen_c[at] = ci;
-        s.dt[at] = best_dt;
-        ++nchosen;
-      }
-    }
-    // clusters with their attachments, moments
-    int q = 0;
-    for (int c = 0; c < ncl2; ++c) {
-      int base = nm;
-      for (int m = s.cl2_off[c]; m < s.cl2_off[c + 1]; ++m) mem_e[nm++] = s.cl2_items[m];
-      int lastb = -1, lasts = -1;
-      for (; q < nchosen && s.chosen_k[q] == c; ++q) {
-        int h = s.filt[s.chosen_c[q]];
-        if (E.bar[h] == lastb && E.sipm[h] == lasts) continue;  // one pulse per (cluster, bar, sipm)
-        lastb = E.bar[h];
-        lasts = E.sipm[h];
-        mem_e[nm++] = h;
-      }
-      const int mn = nm - base;
-      if (!cluster_emit(p, t, SerialAcc{E, s}, mem_e + base, mn, type, P0, P1, evkey, a + nclus, slots)) {
-        nm = base;
-        continue;
-      }
-      cl_moff_e[nclus] = base;
-      ++nclus;
-    }
-  }
-  cl_moff_e[nclus] = nm;
-  ncl[g] = nclus;
-  nmem[g] = nm;
 }
 
 // ---- wavefront finder: one 64-lane wavefront per event of at most WMAX pulses, the event's hits in LDS ----
@@ -1035,6 +691,749 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
   }
 }
 
+// ---- block finder: one 256-thread block per event of more than 256 pulses (showers, overlays) ----
+// The steps of k_coinc with algorithms that scale with the event: filterHits over (side, layer, counter column)
+// buckets, each summed in hit order (the FP64 sums of the serial scan); greedy clusters as the connected components
+// of the link graph in the order of their lowest list position, with the join order given by the pass number
+//   pass(seed) = 1,  pass(p) = min over linked q of pass(q) + [q > p]
+// (a hit joins in the pass after its earliest linked member unless that member comes before it in the list), ties
+// by list position; link candidates from time bins at least one link window wide; layer combinations by one thread
+// per cluster or by the whole block for large products; attachments looked up by bar.
+constexpr int BT = 256;      // threads of a block-finder block
+constexpr int BH = 12288;    // LDS histogram bins: (side, layer, counter column) buckets, time bins, bars
+constexpr int BIG_COMBOS = 4096;
+constexpr int INF_PASS = 1 << 29;
+
+struct BS {  // event-local scratch of the block finder (index: pulse in the event, or position in a list)
+  double *t, *pe, *x, *y, *maxd, *wm, *cmaxd, *wq, *cdt;
+  int32_t *bar, *sec, *cit, *hl, *filt, *items, *off, *coinc, *byl, *cl2, *off2, *bsort, *tsort, *label, *pass, *chk,
+      *sk, *sc, *clmc, *tmp, *clid, *tbin;
+  uint8_t *layer, *side, *sipm, *seq0, *kept, *fpass, *memf;
+};
+
+struct BlockLds {
+  union {
+    hipcub::BlockScan<int, BT>::TempStorage scan;
+    hipcub::BlockReduce<double, BT>::TempStorage red;
+  } cub;
+  int hist[BH];
+  double dshare[2];
+  int flag;
+};
+
+__device__ __forceinline__ int bscan(BlockLds &L, int v, int &total) {
+  int r;
+  hipcub::BlockScan<int, BT>(L.cub.scan).ExclusiveSum(v, r, total);
+  __syncthreads();
+  return r;
+}
+
+// exclusive scan of hist[0..nb) in place (nb <= BH); returns the total
+__device__ int bscan_hist(BlockLds &L, int nb) {
+  const int per = (nb + BT - 1) / BT, a = threadIdx.x * per;
+  int s = 0;
+  for (int i = a; i < a + per && i < nb; ++i) s += L.hist[i];
+  int tot;
+  int run = bscan(L, s, tot);
+  for (int i = a; i < a + per && i < nb; ++i) {
+    const int v = L.hist[i];
+    L.hist[i] = run;
+    run += v;
+  }
+  __syncthreads();
+  return tot;
+}
+
+// in-place heap sort of v[0..n) by key(v) = (a[v] * big + v) ascending (a: pass number, ties by the value itself)
+__device__ void sort_by_pass(int32_t *v, int n, const int32_t *pass) {
+  auto key = [&](int32_t x) -> int64_t { return (int64_t)pass[x] * (1ll << 32) + x; };
+  auto sift = [&](int root, int end) {
+    while (2 * root + 1 < end) {
+      int ch = 2 * root + 1;
+      if (ch + 1 < end && key(v[ch + 1]) > key(v[ch])) ++ch;
+      if (key(v[root]) >= key(v[ch])) return;
+      const int32_t t = v[root];
+      v[root] = v[ch];
+      v[ch] = t;
+      root = ch;
+    }
+  };
+  for (int s = n / 2 - 1; s >= 0; --s) sift(s, n);
+  for (int e = n - 1; e > 0; --e) {
+    const int32_t t = v[0];
+    v[0] = v[e];
+    v[e] = t;
+    sift(0, e);
+  }
+}
+
+__device__ __forceinline__ bool blink(const BS &S, int h, int c, const double *md, double max_dt, bool opposite,
+                                      double dt_opp) {
+  const double mdd = fmax(md[h], md[c]);
+  const double mt = max_dt + ((opposite && S.side[h] != S.side[c]) ? dt_opp : 0.0);
+  return fabs(S.x[h] - S.x[c]) <= mdd && fabs(S.t[h] - S.t[c]) < mt;
+}
+
+// greedy findClusters of the pulses list[0..m): members (pulses) to out[] in join order, cluster starts to off[];
+// returns the number of clusters
+__device__ int block_clusters(BlockLds &L, const BS &S, const int32_t *list, int m, const double *md, double max_dt,
+                              bool opposite, double dt_opp, int32_t *out, int32_t *off) {
+  const int tid = threadIdx.x;
+  if (m == 0) {
+    if (tid == 0) off[0] = 0;
+    __syncthreads();
+    return 0;
+  }
+  // time bins at least one link window wide
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = tid; i < m; i += BT) {
+    lo = fmin(lo, S.t[list[i]]);
+    hi = fmax(hi, S.t[list[i]]);
+  }
+  lo = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(lo, hipcub::Min());
+  __syncthreads();
+  hi = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(hi, hipcub::Max());
+  __syncthreads();
+  if (tid == 0) {
+    L.flag = 0;
+    L.dshare[0] = lo;
+    L.dshare[1] = hi;
+  }
+  __syncthreads();
+  lo = L.dshare[0];
+  hi = L.dshare[1];
+  __syncthreads();
+  const double mt = max_dt + (opposite ? dt_opp : 0.0);
+  const double width = fmax(mt, (hi - lo) / (double)(BH - 2)) * (1.0 + 1e-12) + 1e-9;
+  const int nb = (int)((hi - lo) / width) + 1;
+  for (int b = tid; b < nb + 1; b += BT) L.hist[b] = 0;
+  __syncthreads();
+  for (int i = tid; i < m; i += BT) {
+    const double fb = (S.t[list[i]] - lo) / width;
+    const int b = fb >= 0.0 ? min((int)fmin(fb, (double)(nb - 1)), nb - 1) : 0;  // NaN -> bin 0
+    S.tbin[i] = b;
+    atomicAdd(&L.hist[b], 1);
+  }
+  __syncthreads();
+  bscan_hist(L, nb + 1);
+  for (int i = tid; i < m; i += BT) S.tsort[atomicAdd(&L.hist[S.tbin[i]], 1)] = i;  // hist[b] ends as end of bin b
+  __syncthreads();
+  // components: min-label propagation with pointer jumping
+  for (int i = tid; i < m; i += BT) {
+    S.label[i] = i;
+    S.pass[i] = INF_PASS;
+  }
+  __syncthreads();
+  for (;;) {
+    if (tid == 0) L.flag = 0;
+    __syncthreads();
+    for (int i = tid; i < m; i += BT) {
+      const int h = list[i], b = S.tbin[i];
+      int best = S.label[i];
+      for (int bb = max(b - 1, 0); bb <= min(b + 1, nb - 1); ++bb)
+        for (int k = (bb ? L.hist[bb - 1] : 0); k < L.hist[bb]; ++k) {
+          const int j = S.tsort[k];
+          if (j == i) continue;
+          const int lj = S.label[j];
+          if (lj < best && blink(S, h, list[j], md, max_dt, opposite, dt_opp)) best = lj;
+        }
+      if (best < S.label[i]) {
+        atomicMin(&S.label[i], best);
+        L.flag = 1;
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < m; i += BT) {
+      int l = S.label[i];
+      while (S.label[l] < l) l = S.label[l];
+      S.label[i] = l;
+    }
+    __syncthreads();
+    if (!L.flag) break;
+  }
+  // pass numbers: 0-1 relaxation from the seeds (a component's lowest position)
+  for (int i = tid; i < m; i += BT)
+    if (S.label[i] == i) S.pass[i] = 1;
+  __syncthreads();
+  for (;;) {
+    if (tid == 0) L.flag = 0;
+    __syncthreads();
+    for (int i = tid; i < m; i += BT) {
+      if (S.label[i] == i) continue;
+      const int h = list[i], b = S.tbin[i];
+      int best = S.pass[i];
+      for (int bb = max(b - 1, 0); bb <= min(b + 1, nb - 1); ++bb)
+        for (int k = (bb ? L.hist[bb - 1] : 0); k < L.hist[bb]; ++k) {
+          const int j = S.tsort[k];
+          if (j == i) continue;
+          const int cand = S.pass[j] + (j > i ? 1 : 0);
+          if (cand < best && blink(S, h, list[j], md, max_dt, opposite, dt_opp)) best = cand;
+        }
+      if (best < S.pass[i]) {
+        S.pass[i] = best;
+        L.flag = 1;
+      }
+    }
+    __syncthreads();
+    if (!L.flag) break;
+  }
+  // clusters in seed order; members by (pass, position)
+  for (int i = tid; i < m; i += BT) S.tmp[i] = 0;
+  __syncthreads();
+  for (int i = tid; i < m; i += BT) atomicAdd(&S.tmp[S.label[i]], 1);
+  __syncthreads();
+  int base = 0, cbase = 0;
+  for (int c = 0; c < m; c += BT) {  // member offsets and cluster numbers over seeds in position order
+    const int i = c + tid;
+    const int cnt = i < m ? S.tmp[i] : 0;
+    int tot, ctot;
+    const int r = bscan(L, cnt, tot);
+    const int cr = bscan(L, cnt > 0, ctot);
+    if (cnt > 0) {
+      off[cbase + cr] = base + r;
+      S.clid[i] = cbase + cr;
+      S.tmp[i] = base + r;  // cursor
+    }
+    base += tot;
+    cbase += ctot;
+  }
+  __syncthreads();
+  if (tid == 0) off[cbase] = base;
+  for (int i = tid; i < m; i += BT) S.items[atomicAdd(&S.tmp[S.label[i]], 1)] = i;  // positions, unordered per cluster
+  __syncthreads();
+  for (int c = tid; c < cbase; c += BT) sort_by_pass(S.items + off[c], off[c + 1] - off[c], S.pass);
+  __syncthreads();
+  for (int k = tid; k < base; k += BT) out[k] = list[S.items[k]];
+  __syncthreads();
+  return cbase;
+}
+
+// checkCoincidence of a cluster's members mem[0..n) for one thread; by-layer lists in byl[0..n) with their widened
+// distance in wq / kept flags (same offsets); returns the number of combinations when they are too many for one
+// thread (the block then enumerates them), else 0
+__device__ int64_t thread_check(const crv_params &p, const Tables &t, const BS &S, const int32_t *mem, int n,
+                                int32_t *byl, double *wq, uint8_t *kept, int lay_off[5], int &nmin, int &nmax,
+                                bool &all_kept) {
+  nmin = 1 << 30;
+  nmax = -1;
+  for (int i = 0; i < n; ++i) {
+    const int cl = (int)cut_of(t, S.sec[mem[i]]).coincidence_layers;
+    nmin = min(nmin, cl);
+    nmax = max(nmax, cl);
+  }
+  int pos = 0;
+  for (int l = 0; l < 4; ++l) {
+    lay_off[l] = pos;
+    for (int i = 0; i < n; ++i)
+      if (S.layer[mem[i]] == l) {
+        byl[pos] = mem[i];
+        wq[pos] = cut_of(t, S.sec[mem[i]]).final_max_distance;
+        kept[pos] = 0;
+        ++pos;
+      }
+  }
+  lay_off[4] = pos;
+  int nonempty = 0;
+  for (int l = 0; l < 4; ++l) nonempty += lay_off[l + 1] > lay_off[l];
+  all_kept = n > p.big_cluster_threshold && nonempty >= nmin;
+  if (all_kept) return 0;
+  int64_t total = 0;
+  for (int nl = 2; nl <= 4; ++nl) {
+    const bool need = nl == 2 ? nmin == 2 : nl == 3 ? (nmin <= 3 && 3 <= nmax) : nmax == 4;
+    if (!need) continue;
+    int Lc[4];
+    for (int i = 0; i < nl; ++i) Lc[i] = i;
+    for (;;) {
+      int64_t P = 1;
+      for (int i = 0; i < nl; ++i) P *= lay_off[Lc[i] + 1] - lay_off[Lc[i]];
+      total += P;
+      int k = nl - 1;
+      while (k >= 0 && Lc[k] == 4 - nl + k) --k;
+      if (k < 0) break;
+      ++Lc[k];
+      for (int i = k + 1; i < nl; ++i) Lc[i] = Lc[i - 1] + 1;
+    }
+  }
+  return total;
+}
+
+// layer offsets of a cluster's by-layer lists and the range of its coincidence_layers (byl already built)
+__device__ void cluster_layout(const Tables &t, const BS &S, const int32_t *mem, int n, int lay_off[5], int &nmin,
+                               int &nmax) {
+  nmin = 1 << 30;
+  nmax = -1;
+  int pos = 0;
+  for (int i = 0; i < n; ++i) {
+    const int cl = (int)cut_of(t, S.sec[mem[i]]).coincidence_layers;
+    nmin = min(nmin, cl);
+    nmax = max(nmax, cl);
+  }
+  for (int l = 0; l < 4; ++l) {
+    lay_off[l] = pos;
+    for (int i = 0; i < n; ++i) pos += S.layer[mem[i]] == l;
+  }
+  lay_off[4] = pos;
+}
+
+// one layer combination idx[0..nl) of byl: the pass test, then widening and kept flags (atomic when shared)
+__device__ __forceinline__ void combo_eval(const Tables &t, const BS &S, const int32_t *byl, double *wq, uint8_t *kept,
+                                           const int idx[4], int nl, bool atomic_) {
+  bool skip = true;
+  double mdt = -INFINITY, lo = INFINITY, hi = -INFINITY, msd = -INFINITY, tmin = INFINITY, tmax = -INFINITY;
+  for (int i = 0; i < nl; ++i) {
+    const int h = byl[idx[i]];
+    const Cut c = cut_of(t, S.sec[h]);
+    if (!((int)c.coincidence_layers > nl)) skip = false;
+    mdt = fmax(mdt, c.max_dt);
+    lo = fmin(lo, c.min_slope);
+    hi = fmax(hi, c.max_slope);
+    msd = fmax(msd, c.max_slope_diff);
+    tmin = fmin(tmin, S.t[h]);
+    tmax = fmax(tmax, S.t[h]);
+  }
+  bool pass = !skip && !(tmax - tmin > mdt);
+  double sl[3];
+  for (int i = 0; pass && i + 1 < nl; ++i) {
+    const int a = byl[idx[i]], b = byl[idx[i + 1]];
+    const double v = (S.x[b] - S.x[a]) / (S.y[b] - S.y[a]);
+    if (v < lo || v > hi) pass = false;
+    sl[i] = v;
+  }
+  if (pass && nl > 2 && fabs(sl[0] - sl[1]) > msd) pass = false;
+  if (pass && nl > 3 && (fabs(sl[0] - sl[2]) > msd || fabs(sl[1] - sl[2]) > msd)) pass = false;
+  if (!pass) return;
+  int ord[4];
+  for (int i = 0; i < nl; ++i) ord[i] = i;
+  for (int i = 1; i < nl; ++i) {
+    const int v = ord[i];
+    const double xv = S.x[byl[idx[v]]];
+    int j = i - 1;
+    while (j >= 0 && S.x[byl[idx[ord[j]]]] > xv) {
+      ord[j + 1] = ord[j];
+      --j;
+    }
+    ord[j + 1] = v;
+  }
+  for (int i = 0; i < nl; ++i) {
+    const int q = idx[ord[i]];
+    const double x = S.x[byl[q]];
+    double w = -INFINITY;
+    if (i > 0) w = fmax(w, x - S.x[byl[idx[ord[i - 1]]]]);
+    if (i + 1 < nl) w = fmax(w, S.x[byl[idx[ord[i + 1]]]] - x);
+    if (atomic_) {
+      unsigned long long *a = (unsigned long long *)&wq[q];
+      unsigned long long old = *a, assumed;
+      do {
+        assumed = old;
+        if (!(w > __longlong_as_double((long long)assumed))) break;
+        old = atomicCAS(a, assumed, (unsigned long long)__double_as_longlong(w));
+      } while (old != assumed);
+    } else {
+      wq[q] = fmax(wq[q], w);
+    }
+    kept[q] = 1;
+  }
+}
+
+// every combination of the cluster's by-layer lists from combination number c0 with stride cs
+__device__ void combos(const Tables &t, const BS &S, const int32_t *byl, double *wq, uint8_t *kept, const int lay_off[5],
+                       int nmin, int nmax, int64_t c0, int64_t cs, bool atomic_) {
+  for (int nl = 2; nl <= 4; ++nl) {
+    const bool need = nl == 2 ? nmin == 2 : nl == 3 ? (nmin <= 3 && 3 <= nmax) : nmax == 4;
+    if (!need) continue;
+    int Lc[4];
+    for (int i = 0; i < nl; ++i) Lc[i] = i;
+    for (;;) {
+      int64_t P = 1;
+      for (int i = 0; i < nl; ++i) P *= lay_off[Lc[i] + 1] - lay_off[Lc[i]];
+      for (int64_t ci = c0; ci < P; ci += cs) {
+        int idx[4];
+        int64_t rest = ci;
+        for (int i = nl - 1; i >= 0; --i) {
+          const int sz = lay_off[Lc[i] + 1] - lay_off[Lc[i]];
+          idx[i] = lay_off[Lc[i]] + (int)(rest % sz);
+          rest /= sz;
+        }
+        combo_eval(t, S, byl, wq, kept, idx, nl, atomic_);
+      }
+      int k = nl - 1;
+      while (k >= 0 && Lc[k] == 4 - nl + k) --k;
+      if (k < 0) break;
+      ++Lc[k];
+      for (int i = k + 1; i < nl; ++i) Lc[i] = Lc[i - 1] + 1;
+    }
+  }
+}
+
+struct BlockAcc {
+  const BS &S;
+  __device__ double pe(int h) const { return S.pe[h]; }
+  __device__ double tm(int h) const { return S.t[h]; }
+  __device__ int bar(int h) const { return S.bar[h]; }
+  __device__ int sipm(int h) const { return S.sipm[h]; }
+};
+
+__global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int64_t nev, const int64_t *__restrict__ eoff,
+                                                    const int64_t *__restrict__ evk, const double *__restrict__ ptime,
+                                                    const double *__restrict__ ppe, Ev e, BS s0,
+                                                    int32_t *__restrict__ ncl, int32_t *__restrict__ nmem,
+                                                    ClSlots slots, int32_t *__restrict__ cl_moff,
+                                                    int32_t *__restrict__ mem, int64_t min_n,
+                                                    int *__restrict__ flags) {
+  __shared__ BlockLds L;
+  __shared__ int big_list[BT];
+  const int tid = threadIdx.x;
+  for (int64_t g = blockIdx.x; g < nev; g += gridDim.x) {
+    const int64_t a = eoff[g], n64 = eoff[g + 1] - a;
+    if (n64 < min_n) continue;
+    const int n = (int)n64;
+    const int64_t evkey = evk[g];
+    const int64_t sb = a + g;
+    BS S = s0;
+    {
+      double **dp[] = {&S.t, &S.pe, &S.x, &S.y, &S.maxd, &S.wm, &S.cmaxd, &S.wq, &S.cdt};
+      for (auto q : dp) *q += sb;
+      int32_t **ip[] = {&S.bar, &S.sec, &S.cit, &S.hl, &S.filt, &S.items, &S.off, &S.coinc, &S.byl, &S.cl2, &S.off2,
+                        &S.bsort, &S.tsort, &S.label, &S.pass, &S.chk, &S.sk, &S.sc, &S.clmc, &S.tmp, &S.clid,
+                        &S.tbin};
+      for (auto q : ip) *q += sb;
+      uint8_t **up[] = {&S.layer, &S.side, &S.sipm, &S.seq0, &S.kept, &S.fpass, &S.memf};
+      for (auto q : up) *q += sb;
+    }
+    int32_t *cl_moff_e = cl_moff + sb;
+    int32_t *mem_e = mem + a;
+    int nclus = 0, nm = 0;
+    for (int k = tid; k < n; k += BT) {
+      S.bar[k] = e.bar[a + k];
+      S.sipm[k] = (uint8_t)(e.sipm[a + k] & 0xFF);
+      S.seq0[k] = e.seq[a + k] == 0;
+    }
+    __syncthreads();
+    for (int ti = 0; ti < t.ntypes; ++ti) {
+      const int type = t.type_id[ti];
+      const double P0 = t.tp0[ti], P1 = t.tp1[ti], half = t.thalf[ti], tmaxdt = t.tmaxdt[ti];
+      const bool both = t.tside0[ti] && t.tside1[ti];
+      // hits of this sector type, in pulse order
+      int nh = 0;
+      for (int c = 0; c < n; c += BT) {
+        const int k = c + tid;
+        bool hit = false;
+        if (k < n && !(p.ignore_secondary_pulses && !S.seq0[k])) {
+          const int bar = e.bar[a + k], sp = e.sipm[a + k];
+          if (bar < 0 || bar >= t.nbars || sp < 0 || sp > 3 || bar * 4 + 3 >= t.n_channels) {
+            atomicAdd(&flags[F_BADCHAN], 1);
+          } else {
+            const int sec = t.sector[bar];
+            if (t.hascuts[sec] && t.stype[sec] == type) {
+              double tt = ptime[a + k];
+              if (sp % 2 == 0) tt += (t.ps0[sec] - P0) / p.fiber_signal_speed;
+              else tt += (P1 - t.ps1[sec]) / p.fiber_signal_speed;
+              double pe = ppe[a + k];
+              const int other = bar * 4 + sp + (sp < 2 ? 2 : -2);
+              if (t.status[other] & p.compensate_mask) pe *= 2.0;
+              S.t[k] = tt;
+              S.pe[k] = pe;
+              S.maxd[k] = cut_of(t, sec).initial_max_distance;
+              S.x[k] = t.bx[bar];
+              S.y[k] = t.by[bar];
+              S.sec[k] = sec;
+              S.layer[k] = (uint8_t)t.layer[bar];
+              S.cit[k] = t.cit[bar];
+              S.side[k] = (uint8_t)(sp % 2);
+              hit = true;
+            }
+          }
+        }
+        int tot;
+        const int r = bscan(L, hit, tot);
+        if (hit) S.hl[nh + r] = k;
+        nh += tot;
+      }
+      __syncthreads();
+      if (nh == 0) continue;
+      // filterHits (reco/coinc.py:154-182): buckets of (side, layer, counter column) in hit order
+      const int nbk = 8 * t.ncit;
+      for (int b = tid; b < nbk + 1; b += BT) L.hist[b] = 0;
+      __syncthreads();
+      for (int ii = tid; ii < nh; ii += BT) {
+        const int h = S.hl[ii];
+        atomicAdd(&L.hist[(S.side[h] * 4 + S.layer[h]) * t.ncit + S.cit[h]], 1);
+      }
+      __syncthreads();
+      bscan_hist(L, nbk + 1);
+      for (int ii = tid; ii < nh; ii += BT) {
+        const int h = S.hl[ii];
+        S.bsort[atomicAdd(&L.hist[(S.side[h] * 4 + S.layer[h]) * t.ncit + S.cit[h]], 1)] = h;
+      }
+      __syncthreads();  // hist[b] is now the end of bucket b; order each bucket by pulse (= hit order)
+      for (int b = tid; b < nbk; b += BT) {
+        const int b0 = b ? L.hist[b - 1] : 0, b1 = L.hist[b];
+        for (int i = b0 + 1; i < b1; ++i) {
+          const int v = S.bsort[i];
+          int j = i - 1;
+          while (j >= b0 && S.bsort[j] > v) {
+            S.bsort[j + 1] = S.bsort[j];
+            --j;
+          }
+          S.bsort[j + 1] = v;
+        }
+      }
+      __syncthreads();
+      for (int ii = tid; ii < nh; ii += BT) {
+        const int h = S.hl[ii];
+        const Cut ch = cut_of(t, S.sec[h]);
+        const int row = (S.side[h] * 4 + S.layer[h]) * t.ncit, cnh = S.cit[h];
+        double sums[3] = {0.0, 0.0, 0.0};  // offset -1, 0, +1
+        for (int d = -1; d <= 1; ++d) {
+          if (cnh + d < 0 || cnh + d >= t.ncit) continue;
+          const int b = row + cnh + d;
+          for (int k = b ? L.hist[b - 1] : 0; k < L.hist[b]; ++k) {
+            const int q = S.bsort[k];
+            if (fabs(S.t[q] - S.t[h]) > ch.max_dt_adjacent) continue;
+            sums[d + 1] += S.pe[q];
+          }
+        }
+        S.fpass[ii] = sums[1] + sums[0] >= ch.pe_threshold || sums[1] + sums[2] >= ch.pe_threshold;
+      }
+      __syncthreads();
+      // per side: passing hits in hit order, their clusters and the coincidence check of each
+      int nco = 0, nfilt = 0;
+      for (int side = 0; side < 2; ++side) {
+        const int f0 = nfilt;
+        for (int c = 0; c < nh; c += BT) {
+          const int ii = c + tid;
+          const bool keep = ii < nh && S.side[S.hl[ii]] == side && S.fpass[ii];
+          int tot;
+          const int r = bscan(L, keep, tot);
+          if (keep) S.filt[nfilt + r] = S.hl[ii];
+          nfilt += tot;
+        }
+        __syncthreads();
+        // the clusters go to cl2 / off2 as scratch here (the second clustering reuses them later)
+        const int ncs = block_clusters(L, S, S.filt + f0, nfilt - f0, S.maxd, tmaxdt, false, 0.0, S.cl2, S.off2);
+        // coincidence check: one thread per cluster, the block for clusters with many combinations
+        int nbig = 0;
+        for (int c0 = 0; c0 < ncs; c0 += BT) {
+          const int c = c0 + tid;
+          int64_t P = 0;
+          if (c < ncs) {
+            const int o0 = S.off2[c], mn = S.off2[c + 1] - o0;
+            int lay[5], nmin, nmax;
+            bool allk;
+            P = thread_check(p, t, S, S.cl2 + o0, mn, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, allk);
+            S.clmc[c] = allk ? -1 : 0;  // -1: every member kept with its initial distance
+            S.tmp[c] = (int32_t)min<int64_t>(P, INT32_MAX);
+            if (!allk && P <= BIG_COMBOS) combos(t, S, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, 0, 1, false);
+          }
+          int tot;
+          const bool big = P > BIG_COMBOS;
+          const int r = bscan(L, big, tot);
+          if (big && nbig + r < BT) big_list[nbig + r] = c;
+          nbig += tot;
+        }
+        __syncthreads();
+        for (int bi = 0; bi < min(nbig, BT); ++bi) {  // the block on each cluster with many combinations
+          const int c = big_list[bi], o0 = S.off2[c];
+          int lay[5], nmin, nmax;
+          cluster_layout(t, S, S.cl2 + o0, S.off2[c + 1] - o0, lay, nmin, nmax);
+          combos(t, S, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, tid, BT, true);
+          __syncthreads();
+        }
+        if (nbig > BT) {  // beyond the list: their own thread
+          for (int c = tid; c < ncs; c += BT) {
+            if (S.tmp[c] <= BIG_COMBOS || S.clmc[c] < 0) continue;
+            bool listed = false;
+            for (int q = 0; q < BT && !listed; ++q) listed = big_list[q] == c;
+            if (listed) continue;
+            const int o0 = S.off2[c];
+            int lay[5], nmin, nmax;
+            cluster_layout(t, S, S.cl2 + o0, S.off2[c + 1] - o0, lay, nmin, nmax);
+            combos(t, S, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, 0, 1, false);
+          }
+          __syncthreads();
+        }
+        // kept hits per cluster in pulse order, appended in cluster order with their widened distance
+        for (int c0 = 0; c0 < ncs; c0 += BT) {
+          const int c = c0 + tid;
+          int cnt = 0;
+          if (c < ncs) {
+            const int o0 = S.off2[c], mn = S.off2[c + 1] - o0;
+            if (S.clmc[c] < 0) cnt = mn;
+            else
+              for (int q = 0; q < mn; ++q) cnt += S.kept[o0 + q];
+          }
+          int tot;
+          const int r = bscan(L, cnt, tot);
+          if (c < ncs && cnt) {
+            const int o0 = S.off2[c], mn = S.off2[c + 1] - o0, w0 = nco + r;
+            if (S.clmc[c] < 0) {
+              for (int i = 0; i < mn; ++i) {
+                S.coinc[w0 + i] = S.cl2[o0 + i];
+                S.cmaxd[w0 + i] = S.maxd[S.cl2[o0 + i]];
+              }
+            } else {
+              int w = w0;
+              for (int q = 0; q < mn; ++q)
+                if (S.kept[o0 + q]) {
+                  const int h = S.byl[o0 + q];
+                  int at = w;  // insertion by pulse
+                  while (at > w0 && S.coinc[at - 1] > h) {
+                    S.coinc[at] = S.coinc[at - 1];
+                    S.cmaxd[at] = S.cmaxd[at - 1];
+                    --at;
+                  }
+                  S.coinc[at] = h;
+                  S.cmaxd[at] = S.wq[o0 + q];
+                  ++w;
+                }
+            }
+          }
+          nco += tot;
+        }
+        __syncthreads();
+      }
+      const double dt_opp = 2.0 * half / p.fiber_signal_speed;
+      for (int i = tid; i < nco; i += BT) S.wm[S.coinc[i]] = S.cmaxd[i];
+      __syncthreads();
+      const int ncl2 = block_clusters(L, S, S.coinc, nco, S.wm, tmaxdt, both, dt_opp, S.cl2, S.off2);
+      const int nitems = ncl2 ? S.off2[ncl2] : 0;
+      // attachment of opposite-side pulses (reco/coinc_cpu.py:11-51), candidates looked up by bar
+      int nchosen = 0;
+      if (both && p.add_opposite_side_pulses && nitems > 0 && nfilt > 0) {
+        for (int b = tid; b < t.nbars + 1; b += BT) L.hist[b] = 0;
+        for (int k = tid; k < n; k += BT) S.memf[k] = 0;
+        __syncthreads();
+        for (int c = tid; c < ncl2; c += BT)
+          for (int m = S.off2[c]; m < S.off2[c + 1]; ++m) {
+            S.clid[m] = c;
+            S.memf[S.cl2[m]] = 1;
+            atomicAdd(&L.hist[S.bar[S.cl2[m]]], 1);
+          }
+        __syncthreads();
+        bscan_hist(L, t.nbars + 1);
+        for (int m = tid; m < nitems; m += BT) S.bsort[atomicAdd(&L.hist[S.bar[S.cl2[m]]], 1)] = m;
+        __syncthreads();  // hist[bar] = end of the bar's items
+        const double mdt = tmaxdt + dt_opp;
+        for (int ci = tid; ci < nfilt; ci += BT) {
+          const int h = S.filt[ci];
+          int best_c = -1;
+          double best_dt = 0.0;
+          if (S.seq0[h] && !S.memf[h]) {
+            const int hb = S.bar[h], hs = S.side[h];
+            const int k0 = hb ? L.hist[hb - 1] : 0, k1 = L.hist[hb];
+            for (int k = k0; k < k1; ++k) {
+              const int m = S.bsort[k], an = S.cl2[m], c = S.clid[m];
+              if (!S.seq0[an] || hs != 1 - S.side[an]) continue;
+              const double dt = fabs(S.t[h] - S.t[an]);
+              if (!(dt < mdt)) continue;
+              bool occ = false;
+              for (int kk = k0; kk < k1 && !occ; ++kk)
+                occ = S.clid[S.bsort[kk]] == c && S.side[S.cl2[S.bsort[kk]]] == hs;
+              if (occ) continue;
+              if (best_c < 0 || dt < best_dt || (dt == best_dt && c < best_c)) {
+                best_c = c;
+                best_dt = dt;
+              }
+            }
+          }
+          S.chk[ci] = best_c;
+          S.cdt[ci] = best_dt;
+        }
+        __syncthreads();
+        for (int c0 = 0; c0 < nfilt; c0 += BT) {  // rank in (cluster, bar, sipm, dt, candidate) order
+          const int ci = c0 + tid;
+          const bool chosen = ci < nfilt && S.chk[ci] >= 0;
+          if (chosen) {
+            const int k = S.chk[ci], h = S.filt[ci], b = S.bar[h], sp = S.sipm[h];
+            const double dt = S.cdt[ci];
+            int rank = 0;
+            for (int cj = 0; cj < nfilt; ++cj) {
+              const int kj = S.chk[cj];
+              if (kj < 0 || cj == ci) continue;
+              const int hj = S.filt[cj], bj = S.bar[hj], sj = S.sipm[hj];
+              const double dj = S.cdt[cj];
+              rank += kj < k || (kj == k && (bj < b || (bj == b && (sj < sp || (sj == sp && (dj < dt ||
+                      (dj == dt && cj < ci)))))));
+            }
+            S.sk[rank] = k;
+            S.sc[rank] = ci;
+          }
+          int tot;
+          bscan(L, chosen, tot);
+          nchosen += tot;
+        }
+        __syncthreads();
+      }
+      // members (items, then attachments one per (bar, sipm)) and the min-PE test per cluster; offsets; moments
+      for (int c = tid; c < ncl2; c += BT) {
+        int mc = S.off2[c + 1] - S.off2[c];
+        double tot = 0.0, mincp = INFINITY;
+        for (int m = S.off2[c]; m < S.off2[c + 1]; ++m) {
+          const int h = S.cl2[m];
+          tot = tot + S.pe[h];
+          mincp = fmin(mincp, cut_of(t, S.sec[h]).min_cluster_pes);
+        }
+        int q0 = 0, q1 = nchosen;  // attachments of cluster c: a range of the sorted list
+        while (q0 < q1) {
+          const int mid = (q0 + q1) >> 1;
+          if (S.sk[mid] < c) q0 = mid + 1;
+          else q1 = mid;
+        }
+        int lastb = -1, lasts = -1;
+        for (int q = q0; q < nchosen && S.sk[q] == c; ++q) {
+          const int h = S.filt[S.sc[q]];
+          if (S.bar[h] == lastb && S.sipm[h] == lasts) continue;
+          lastb = S.bar[h];
+          lasts = S.sipm[h];
+          ++mc;
+          tot = tot + S.pe[h];
+          mincp = fmin(mincp, cut_of(t, S.sec[h]).min_cluster_pes);
+        }
+        S.clmc[c] = tot < mincp ? 0 : mc;
+      }
+      __syncthreads();
+      for (int c0 = 0; c0 < ncl2; c0 += BT) {
+        const int c = c0 + tid;
+        const int mc = c < ncl2 ? S.clmc[c] : 0;
+        int mtot, ktot;
+        const int mr = bscan(L, mc, mtot);
+        const int kr = bscan(L, mc > 0, ktot);
+        if (mc > 0) {
+          const int mbase = nm + mr, cslot = nclus + kr;
+          int32_t *mm = mem_e + mbase;
+          int w = 0;
+          for (int m = S.off2[c]; m < S.off2[c + 1]; ++m) mm[w++] = S.cl2[m];
+          int q0 = 0, q1 = nchosen;
+          while (q0 < q1) {
+            const int mid = (q0 + q1) >> 1;
+            if (S.sk[mid] < c) q0 = mid + 1;
+            else q1 = mid;
+          }
+          int lastb = -1, lasts = -1;
+          for (int q = q0; q < nchosen && S.sk[q] == c; ++q) {
+            const int h = S.filt[S.sc[q]];
+            if (S.bar[h] == lastb && S.sipm[h] == lasts) continue;
+            lastb = S.bar[h];
+            lasts = S.sipm[h];
+            mm[w++] = h;
+          }
+          cluster_emit(p, t, BlockAcc{S}, mm, mc, type, P0, P1, evkey, a + cslot, slots);
+          cl_moff_e[cslot] = mbase;
+        }
+        nm += mtot;
+        nclus += ktot;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      cl_moff_e[nclus] = nm;
+      ncl[g] = nclus;
+      nmem[g] = nm;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_coinc_out(int64_t nev, const int64_t *__restrict__ eoff, const int64_t *__restrict__ clo,
                             const int64_t *__restrict__ mo, const int32_t *__restrict__ ncl,
                             const int64_t *__restrict__ cl_ev, const int32_t *__restrict__ cl_type,
@@ -1104,14 +1503,7 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
   const int64_t nev = hn;
   int rc = scan_i32_to_i64(c, cnt, eoff, nev, 95, s);
   if (rc) return rc;
-  Scr sc;
-  double **dp[] = {&sc.t, &sc.pe, &sc.maxd, &sc.wmaxd, &sc.dt};
-  const int64_t ns = n + nev + 2;
-  for (int i = 0; i < 5; ++i) *dp[i] = (double *)ws(c, 96 + i, ns * 8);
-  int32_t **ip[] = {&sc.hl, &sc.filt, &sc.rem, &sc.items, &sc.cl_of, &sc.coinc, &sc.bylayer, &sc.chosen_c,
-                    &sc.chosen_k, &sc.cl2_off, &sc.cl2_items};
-  for (int i = 0; i < 11; ++i) *ip[i] = (int32_t *)ws(c, 101 + i, ns * 4);
-  sc.kept = (uint8_t *)ws(c, 112, ns + 8);
+  const int64_t ns = n + nev + 2;  // event-local scratch: every event owns n + 1 entries
   int32_t *ncl = (int32_t *)ws(c, 113, (nev + 1) * 4);
   int32_t *nmem = (int32_t *)ws(c, 114, (nev + 1) * 4);
   int64_t *cl_ev = (int64_t *)ws(c, 115, (n + 1) * 8);
@@ -1123,7 +1515,7 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
   int32_t *mem = (int32_t *)ws(c, 121, (n + 1) * 4);
   int64_t *clo = (int64_t *)ws(c, 122, (nev + 1) * 8);
   int64_t *mo = (int64_t *)ws(c, 123, (nev + 1) * 8);
-  if (!sc.kept || !ncl || !nmem || !cl_ev || !cl_type || !cl_pc || !cl_sh || !cl_d || !cl_moff || !mem || !clo || !mo)
+  if (!ncl || !nmem || !cl_ev || !cl_type || !cl_pc || !cl_sh || !cl_d || !cl_moff || !mem || !clo || !mo)
     return ws_fail(c);
   Ev e{bar, sipm, seq};
   CRV_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int) * F_NFLAGS, s));
@@ -1137,10 +1529,23 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     k_coinc_wave<256><<<std::min(grid, 4096), 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots,
                                                           cl_moff, mem, (int64_t)129, c->flags);
     CRV_HIP(c, hipGetLastError());
+    if (c->t.nbars + 1 > BH || 8 * c->t.ncit + 1 > BH)
+      return set_error(c, CRV_E_ARG, "clusters: geometry too large for the block finder's bar / column buckets");
+    BS b{};
+    double **bd[] = {&b.t, &b.pe, &b.x, &b.y, &b.maxd, &b.wm, &b.cmaxd, &b.wq, &b.cdt};
+    for (int i = 0; i < 9; ++i) *bd[i] = (double *)ws(c, 130 + i, ns * 8);
+    int32_t **bi[] = {&b.bar, &b.sec, &b.cit, &b.hl, &b.filt, &b.items, &b.off, &b.coinc, &b.byl, &b.cl2, &b.off2,
+                      &b.bsort, &b.tsort, &b.label, &b.pass, &b.chk, &b.sk, &b.sc, &b.clmc, &b.tmp, &b.clid, &b.tbin};
+    for (int i = 0; i < 22; ++i) *bi[i] = (int32_t *)ws(c, 140 + i, ns * 4);
+    uint8_t **bu[] = {&b.layer, &b.side, &b.sipm, &b.seq0, &b.kept, &b.fpass, &b.memf};
+    for (int i = 0; i < 7; ++i) *bu[i] = (uint8_t *)ws(c, 170 + i, ns + 8);
+    for (auto q : bd) if (!*q) return ws_fail(c);
+    for (auto q : bi) if (!*q) return ws_fail(c);
+    for (auto q : bu) if (!*q) return ws_fail(c);
+    k_coinc_block<<<(int)std::min<int64_t>(nev, 2048), BT, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, b, ncl,
+                                                                  nmem, slots, cl_moff, mem, (int64_t)257, c->flags);
+    CRV_HIP(c, hipGetLastError());
   }
-  k_coinc<<<nblocks(nev, 64), 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, sc, ncl, nmem, cl_ev, cl_type,
-                                          cl_pc, cl_sh, cl_d, cl_moff, mem, (int64_t)256, c->flags);
-  CRV_HIP(c, hipGetLastError());
   kmark(c, 3, 1, s);
   if ((rc = scan_i32_to_i64(c, ncl, clo, nev, 124, s))) return rc;
   if ((rc = scan_i32_to_i64(c, nmem, mo, nev, 125, s))) return rc;
