@@ -701,7 +701,7 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
 // per cluster or by the whole block for large products; attachments looked up by bar.
 constexpr int BT = 256;      // threads of a block-finder block
 constexpr int BH = 12288;    // LDS histogram bins: (side, layer, counter column) buckets, time bins, bars
-constexpr int BIG_COMBOS = 4096;
+constexpr int BIG_COMBOS = 512;
 constexpr int INF_PASS = 1 << 29;
 
 struct BS {  // event-local scratch of the block finder (index: pulse in the event, or position in a list)
@@ -717,7 +717,8 @@ struct BlockLds {
     hipcub::BlockReduce<double, BT>::TempStorage red;
   } cub;
   int hist[BH];
-  double dshare[2];
+  double dshare[4];
+  int ishare[8];
   int flag;
 };
 
@@ -1065,6 +1066,273 @@ __device__ void combos(const Tables &t, const BS &S, const int32_t *byl, double 
   }
 }
 
+// atomic max of a double in global memory (no-op unless w is larger; NaN never stored)
+__device__ __forceinline__ void gmax_double(double *a, double w) {
+  unsigned long long *q = (unsigned long long *)a;
+  unsigned long long old = *q, assumed;
+  do {
+    assumed = old;
+    if (!(w > __longlong_as_double((long long)assumed))) break;
+    old = atomicCAS(q, assumed, (unsigned long long)__double_as_longlong(w));
+  } while (old != assumed);
+}
+
+// A cluster's hits grouped by bar (layer, then bar, then time): every hit of a group has the bar's x, y and sector
+// cuts, so the slope tests, the skip test and the widened distances of a layer combination depend only on the groups
+// it takes; only its time window depends on the hits.
+struct Groups {
+  const int32_t *byl;  // hits by (layer, bar, time, pulse)
+  const int32_t *gs;   // group g holds byl[gs[g] .. gs[g+1])
+  double *wq;          // per hit: widened distance
+  uint8_t *kept;       // per hit: in some passing combination
+  double *gw;          // per group: widening from combinations that pass for every hit of the group
+  uint8_t *gk;         // per group: some combination passes for every hit of the group
+};
+
+// the first time >= m in group g is within mdt of m (times ascend within a group)
+__device__ __forceinline__ bool group_covers(const BS &S, const Groups &G, int g, double m, double mdt) {
+  for (int k = G.gs[g]; k < G.gs[g + 1]; ++k) {
+    const double u = S.t[G.byl[k]];
+    if (u >= m) return !(u - m > mdt);
+  }
+  return false;
+}
+
+// whether hit time th of tuple member i takes part in a combination of one hit per group within the time window:
+// a combination passes iff every hit is within mdt of its earliest hit, which is th itself or a hit u < th of
+// another group (then every other group needs a hit in [u, u + mdt], the same rounding as max - min)
+__device__ bool tuple_time_ok(const BS &S, const Groups &G, const int *g, int nl, int i, double th, double mdt) {
+  bool ok = true;
+  for (int j = 0; j < nl && ok; ++j)
+    if (j != i) ok = group_covers(S, G, g[j], th, mdt);
+  if (ok) return true;
+  for (int j = 0; j < nl; ++j) {
+    if (j == i) continue;
+    for (int k = G.gs[g[j]]; k < G.gs[g[j] + 1]; ++k) {
+      const double u = S.t[G.byl[k]];
+      if (!(u < th)) break;
+      if (th - u > mdt) continue;
+      bool all = true;
+      for (int j2 = 0; j2 < nl && all; ++j2)
+        if (j2 != i && j2 != j) all = group_covers(S, G, g[j2], u, mdt);
+      if (all) return true;
+    }
+  }
+  return false;
+}
+
+// one tuple of groups g[0..nl) in layer order with its consecutive slopes sl[0..nl-1): the combination tests of
+// reference reco/coinc.py:156-181 with the tuple's cuts, then the widening of every hit that takes part
+__device__ __noinline__ void tuple_leaf(const Tables &t, const BS &S, const Groups &G, const int *g, int nl, const double *sl) {
+  bool skip = true;
+  double mdt = -INFINITY, lo = INFINITY, hi = -INFINITY, msd = -INFINITY, m_all = INFINITY, M_all = -INFINITY;
+  double xs[4];
+  for (int i = 0; i < nl; ++i) {
+    const int h = G.byl[G.gs[g[i]]];
+    const Cut c = cut_of(t, S.sec[h]);
+    if (!((int)c.coincidence_layers > nl)) skip = false;
+    mdt = fmax(mdt, c.max_dt);
+    lo = fmin(lo, c.min_slope);
+    hi = fmax(hi, c.max_slope);
+    msd = fmax(msd, c.max_slope_diff);
+    xs[i] = S.x[h];
+    m_all = fmin(m_all, S.t[h]);
+    M_all = fmax(M_all, S.t[G.byl[G.gs[g[i] + 1] - 1]]);
+  }
+  if (skip) return;
+  for (int i = 0; i + 1 < nl; ++i)
+    if (sl[i] < lo || sl[i] > hi) return;
+  if (nl > 2 && fabs(sl[0] - sl[1]) > msd) return;
+  if (nl > 3 && (fabs(sl[0] - sl[2]) > msd || fabs(sl[1] - sl[2]) > msd)) return;
+  int ord[4];
+  for (int i = 0; i < nl; ++i) ord[i] = i;
+  for (int i = 1; i < nl; ++i) {  // stable by x: ties keep layer order
+    const int v = ord[i];
+    int j = i - 1;
+    while (j >= 0 && xs[ord[j]] > xs[v]) {
+      ord[j + 1] = ord[j];
+      --j;
+    }
+    ord[j + 1] = v;
+  }
+  double w[4];
+  for (int i = 0; i < nl; ++i) {
+    double wi = -INFINITY;
+    if (i > 0) wi = fmax(wi, xs[ord[i]] - xs[ord[i - 1]]);
+    if (i + 1 < nl) wi = fmax(wi, xs[ord[i + 1]] - xs[ord[i]]);
+    w[ord[i]] = wi;
+  }
+  if (!(M_all - m_all > mdt)) {  // every choice of hits is within the window
+    for (int i = 0; i < nl; ++i) {
+      gmax_double(&G.gw[g[i]], w[i]);
+      G.gk[g[i]] = 1;
+    }
+    return;
+  }
+  for (int i = 0; i < nl; ++i)
+    for (int k = G.gs[g[i]]; k < G.gs[g[i] + 1]; ++k)
+      if (tuple_time_ok(S, G, g, nl, i, S.t[G.byl[k]], mdt)) {
+        gmax_double(&G.wq[k], w[i]);
+        G.kept[k] = 1;
+      }
+}
+
+__device__ __forceinline__ double group_slope(const BS &S, const Groups &G, int ga, int gb) {
+  const int a = G.byl[G.gs[ga]], b = G.byl[G.gs[gb]];
+  return (S.x[b] - S.x[a]) / (S.y[b] - S.y[a]);
+}
+
+// checkCoincidence (reference reco/coinc.py:184-207) of the cluster at o0 (n hits in byl, layer order) by the whole
+// block over tuples of bars, pruned by the widest slope and slope-difference cuts in the cluster. Leaves byl sorted by
+// (layer, bar, time, pulse) with wq / kept per position. Returns false, with byl untouched, when a time is not finite
+// (the caller then enumerates hit combinations).
+__device__ __noinline__ bool block_tuple_check(BlockLds &L, const Tables &t, const BS &S, int o0, int n) {
+  const int tid = threadIdx.x;
+  int32_t *byl = S.byl + o0, *srt = S.items + o0, *gs = S.label + o0, *gid = S.tbin + o0;
+  double *wq = S.wq + o0, *gw = S.cdt + o0;
+  uint8_t *kept = S.kept + o0, *gk = S.memf + o0;
+  if (tid == 0) {
+    L.flag = 0;
+    for (int l = 0; l < 8; ++l) L.ishare[l] = 0;
+    L.ishare[5] = 1 << 30;
+    L.ishare[6] = -1;
+  }
+  __syncthreads();
+  double lo = INFINITY, hi = -INFINITY, msd = -INFINITY;
+  int nmin = 1 << 30, nmax = -1;
+  for (int i = tid; i < n; i += BT) {
+    const int h = byl[i];
+    if (!isfinite(S.t[h])) L.flag = 1;
+    const Cut c = cut_of(t, S.sec[h]);
+    lo = fmin(lo, c.min_slope);
+    hi = fmax(hi, c.max_slope);
+    msd = fmax(msd, c.max_slope_diff);
+    nmin = min(nmin, (int)c.coincidence_layers);
+    nmax = max(nmax, (int)c.coincidence_layers);
+  }
+  atomicMin(&L.ishare[5], nmin);
+  atomicMax(&L.ishare[6], nmax);
+  lo = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(lo, hipcub::Min());
+  __syncthreads();
+  if (tid == 0) L.dshare[0] = lo;
+  hi = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(hi, hipcub::Max());
+  __syncthreads();
+  if (tid == 0) L.dshare[1] = hi;
+  msd = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(msd, hipcub::Max());
+  __syncthreads();
+  if (tid == 0) L.dshare[2] = msd;
+  __syncthreads();
+  if (L.flag) {
+    __syncthreads();
+    return false;
+  }
+  lo = L.dshare[0];
+  hi = L.dshare[1];
+  msd = L.dshare[2];
+  nmin = L.ishare[5];
+  nmax = L.ishare[6];
+  // rank sort by (layer, bar, time, pulse): a total order, so ranks are a permutation
+  for (int i = tid; i < n; i += BT) {
+    const int h = byl[i], ka = S.layer[h] * t.nbars + S.bar[h];
+    const double ta = S.t[h];
+    int r = 0;
+    for (int j = 0; j < n; ++j) {
+      const int hj = byl[j], kb = S.layer[hj] * t.nbars + S.bar[hj];
+      const double tb = S.t[hj];
+      r += kb < ka || (kb == ka && (tb < ta || (tb == ta && hj < h)));
+    }
+    srt[r] = h;
+  }
+  __syncthreads();
+  for (int q = tid; q < n; q += BT) {
+    const int h = srt[q];
+    byl[q] = h;
+    wq[q] = cut_of(t, S.sec[h]).final_max_distance;
+    kept[q] = 0;
+  }
+  __syncthreads();
+  // groups: runs of one bar
+  int ng = 0;
+  for (int c = 0; c < n; c += BT) {
+    const int q = c + tid;
+    const bool f = q < n && (q == 0 || S.bar[byl[q]] != S.bar[byl[q - 1]]);
+    int tot;
+    const int r = bscan(L, f, tot);
+    if (q < n) {
+      const int g = ng + r + (f ? 0 : -1);
+      gid[q] = g;
+      if (f) {
+        gs[g] = q;
+        atomicAdd(&L.ishare[S.layer[byl[q]]], 1);
+      }
+    }
+    ng += tot;
+  }
+  __syncthreads();
+  if (tid == 0) gs[ng] = n;
+  int lgo[5];
+  lgo[0] = 0;
+  for (int l = 0; l < 4; ++l) lgo[l + 1] = lgo[l] + L.ishare[l];
+  for (int g = tid; g < ng; g += BT) {
+    gw[g] = -INFINITY;
+    gk[g] = 0;
+  }
+  __syncthreads();
+  const Groups G{byl, gs, wq, kept, gw, gk};
+  for (int nl = 2; nl <= 4; ++nl) {
+    const bool need = nl == 2 ? nmin == 2 : nl == 3 ? (nmin <= 3 && 3 <= nmax) : nmax == 4;
+    if (!need) continue;
+    int Lc[4];
+    for (int i = 0; i < nl; ++i) Lc[i] = i;
+    for (;;) {
+      const int n0 = lgo[Lc[0] + 1] - lgo[Lc[0]], n1 = lgo[Lc[1] + 1] - lgo[Lc[1]];
+      const int P01 = n0 * n1;
+      bool empty = false;
+      for (int i = 2; i < nl; ++i) empty |= lgo[Lc[i] + 1] == lgo[Lc[i]];
+      for (int ci = empty ? P01 : tid; ci < P01; ci += BT) {
+        int g[4];
+        double sl[3];
+        g[0] = lgo[Lc[0]] + ci / n1;
+        g[1] = lgo[Lc[1]] + ci % n1;
+        sl[0] = group_slope(S, G, g[0], g[1]);
+        if (sl[0] < lo || sl[0] > hi) continue;
+        if (nl == 2) {
+          tuple_leaf(t, S, G, g, 2, sl);
+          continue;
+        }
+        for (g[2] = lgo[Lc[2]]; g[2] < lgo[Lc[2] + 1]; ++g[2]) {
+          sl[1] = group_slope(S, G, g[1], g[2]);
+          if (sl[1] < lo || sl[1] > hi || fabs(sl[0] - sl[1]) > msd) continue;
+          if (nl == 3) {
+            tuple_leaf(t, S, G, g, 3, sl);
+            continue;
+          }
+          for (g[3] = lgo[Lc[3]]; g[3] < lgo[Lc[3] + 1]; ++g[3]) {
+            sl[2] = group_slope(S, G, g[2], g[3]);
+            if (sl[2] < lo || sl[2] > hi || fabs(sl[0] - sl[2]) > msd || fabs(sl[1] - sl[2]) > msd) continue;
+            tuple_leaf(t, S, G, g, 4, sl);
+          }
+        }
+      }
+      int k = nl - 1;
+      while (k >= 0 && Lc[k] == 4 - nl + k) --k;
+      if (k < 0) break;
+      ++Lc[k];
+      for (int i = k + 1; i < nl; ++i) Lc[i] = Lc[i - 1] + 1;
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < n; q += BT) {
+    const int g = gid[q];
+    if (gk[g]) {
+      kept[q] = 1;
+      wq[q] = fmax(wq[q], gw[g]);
+    }
+  }
+  __syncthreads();
+  return true;
+}
+
 struct BlockAcc {
   const BS &S;
   __device__ double pe(int h) const { return S.pe[h]; }
@@ -1081,7 +1349,6 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
                                                     int32_t *__restrict__ mem, int64_t min_n,
                                                     int *__restrict__ flags) {
   __shared__ BlockLds L;
-  __shared__ int big_list[BT];
   const int tid = threadIdx.x;
   for (int64_t g = blockIdx.x; g < nev; g += gridDim.x) {
     const int64_t a = eoff[g], n64 = eoff[g + 1] - a;
@@ -1228,27 +1495,16 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
           int tot;
           const bool big = P > BIG_COMBOS;
           const int r = bscan(L, big, tot);
-          if (big && nbig + r < BT) big_list[nbig + r] = c;
+          if (big) S.sk[nbig + r] = c;
           nbig += tot;
         }
         __syncthreads();
-        for (int bi = 0; bi < min(nbig, BT); ++bi) {  // the block on each cluster with many combinations
-          const int c = big_list[bi], o0 = S.off2[c];
-          int lay[5], nmin, nmax;
-          cluster_layout(t, S, S.cl2 + o0, S.off2[c + 1] - o0, lay, nmin, nmax);
-          combos(t, S, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, tid, BT, true);
-          __syncthreads();
-        }
-        if (nbig > BT) {  // beyond the list: their own thread
-          for (int c = tid; c < ncs; c += BT) {
-            if (S.tmp[c] <= BIG_COMBOS || S.clmc[c] < 0) continue;
-            bool listed = false;
-            for (int q = 0; q < BT && !listed; ++q) listed = big_list[q] == c;
-            if (listed) continue;
-            const int o0 = S.off2[c];
+        for (int bi = 0; bi < nbig; ++bi) {  // the block on each cluster with many combinations
+          const int c = S.sk[bi], o0 = S.off2[c], mn = S.off2[c + 1] - o0;
+          if (!block_tuple_check(L, t, S, o0, mn)) {
             int lay[5], nmin, nmax;
-            cluster_layout(t, S, S.cl2 + o0, S.off2[c + 1] - o0, lay, nmin, nmax);
-            combos(t, S, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, 0, 1, false);
+            cluster_layout(t, S, S.cl2 + o0, mn, lay, nmin, nmax);
+            combos(t, S, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, tid, BT, true);
           }
           __syncthreads();
         }

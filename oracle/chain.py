@@ -48,20 +48,29 @@ def clusters(W, cond, G, cfg, P=None):
     """Reference chain.py:296-362 numpy branch, detail=True. Returns (out dict, P)."""
     if P is None:
         P = pulses(W, cond, cfg)
-    fcfg = cfg.finder
     wf = P["wf"]
     ev = W["ev"][wf]
     o = np.lexsort((P["seq"], wf, ev))
     ev, wf = ev[o], wf[o]
     idx = np.arange(len(ev)) - np.searchsorted(ev, ev)
     bar, sipm, time, pe, seq = W["bar"][wf], W["sipm"][wf], P["time"][o], P["pe"][o], P["seq"][o]
+    out = find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G)
+    out["pulses"] = dict(ev=ev, idx=idx, bar=bar, sipm=sipm, seq=seq, pe=pe, time=time)
+    return out, P
+
+
+def find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G):
+    """Pulses sorted by event (collection order within an event) -> cluster table: coinc.find_event per event
+    (reference chain.py:337-362)."""
+    ev = np.asarray(ev)
+    idx = np.arange(len(ev)) - np.searchsorted(ev, ev)
     cl = {k: [] for k in ("cl_event", "cl_type", "cl_pe", "cl_start", "cl_end", "cl_slope", "cl_avgtime", "cl_poscalc")}
     mem = {"cl_pos": [], "mem_cl": [], "mem_idx": [], "mem_event": []}
     ue, lo = np.unique(ev, return_index=True)
     hi = np.r_[lo[1:], len(ev)]
     for e, a, b in zip(ue, lo, hi):
         p = dict(idx=idx[a:b], bar=bar[a:b], sipm=sipm[a:b], time=time[a:b], pe=pe[a:b], seq=seq[a:b])
-        for r in C.find_event(p, G, cond.status, fcfg):
+        for r in C.find_event(p, G, cond.status, cfg.finder):
             cl["cl_event"].append(e)
             for k, src in (("cl_type", "sectortype"), ("cl_pe", "pe"), ("cl_start", "start"), ("cl_end", "end"),
                            ("cl_slope", "slope"), ("cl_avgtime", "avgtime"), ("cl_poscalc", "poscalc")):
@@ -74,8 +83,7 @@ def clusters(W, cond, G, cfg, P=None):
     out = {k: np.asarray(v) for k, v in cl.items()}
     out.update({k: np.asarray(v, np.int64) for k, v in mem.items() if k != "cl_pos"})
     out["cl_pos"] = np.asarray(mem["cl_pos"], float).reshape(-1, 3)
-    out["pulses"] = dict(ev=ev, idx=idx, bar=bar, sipm=sipm, seq=seq, pe=pe, time=time)
-    return out, P
+    return out
 
 
 def run(steps, cond, G, cfg, seed, tmpl):

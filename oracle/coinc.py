@@ -130,6 +130,12 @@ def check_coincidence(cluster, cfg):
     nmax = max(h["cuts"].coincidence_layers for h in cluster)
     if len(cluster) > cfg.big_cluster_threshold and sum(1 for l in by_layer if l) >= nmin:
         return list(cluster)
+    sizes = [len(l) for l in by_layer]
+    total = sum(int(np.prod([sizes[l] for l in layers]))
+                for n, need in ((2, nmin == 2), (3, nmin <= 3 <= nmax), (4, nmax == 4)) if need
+                for layers in combinations(range(N_LAYERS), n))
+    if total > NUMPY_COMBOS:
+        return _check_coincidence_np(by_layer, nmin, nmax)
     keep = {}
     for n, need in ((2, nmin == 2), (3, nmin <= 3 <= nmax), (4, nmax == 4)):
         if not need:
@@ -142,6 +148,67 @@ def check_coincidence(cluster, cfg):
                     for h in combo:
                         keep[h["idx"]] = h
     return [keep[k] for k in sorted(keep)]
+
+
+NUMPY_COMBOS = 2000  # clusters with more layer combinations than this take the numpy path
+
+
+def _check_coincidence_np(by_layer, nmin, nmax, chunk=1 << 20):
+    """check_coincidence over numpy arrays of the combinations, chunked, with every hit's own cuts: the same tests
+    as check_combination (time window with the combination's widest max_dt, slopes within its widest slope range,
+    slope differences, the skip of combinations whose hits all need more layers) and the same widening (x gaps to the
+    neighbours of a stable sort by x of the layer-ordered combination), as the reference's numpy variant
+    reco/coinc.py:209-250 does for uniform cuts. Times are finite."""
+    H = [h for l in by_layer for h in l]
+    off = np.cumsum([0] + [len(l) for l in by_layer])
+    T = np.array([h["t"] for h in H], float)
+    X = np.array([h["x"] for h in H], float)
+    Y = np.array([h["y"] for h in H], float)
+    c = [h["cuts"] for h in H]
+    CL = np.array([q.coincidence_layers for q in c])
+    MDT = np.array([q.max_dt for q in c], float)
+    LO = np.array([q.min_slope for q in c], float)
+    HI = np.array([q.max_slope for q in c], float)
+    MSD = np.array([q.max_slope_diff for q in c], float)
+    wid = np.full(len(H), -np.inf)
+    kept = np.zeros(len(H), bool)
+    for n, need in ((2, nmin == 2), (3, nmin <= 3 <= nmax), (4, nmax == 4)):
+        if not need:
+            continue
+        for layers in combinations(range(N_LAYERS), n):
+            shape = tuple(int(off[l + 1] - off[l]) for l in layers)
+            P = int(np.prod(shape))
+            for a in range(0, P, chunk):
+                idx = np.unravel_index(np.arange(a, min(a + chunk, P)), shape)
+                cb = np.stack([off[l] + i for l, i in zip(layers, idx)], 1)
+                t, x, y = T[cb], X[cb], Y[cb]
+                ok = ~(CL[cb] > n).all(1)
+                ok &= ~(t.max(1) - t.min(1) > MDT[cb].max(1))
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    sl = (x[:, 1:] - x[:, :-1]) / (y[:, 1:] - y[:, :-1])
+                lo, hi, msd = LO[cb].min(1), HI[cb].max(1), MSD[cb].max(1)
+                ok &= ~((sl < lo[:, None]) | (sl > hi[:, None])).any(1)
+                if n > 2:
+                    ok &= ~(np.abs(sl[:, 0] - sl[:, 1]) > msd)
+                if n > 3:
+                    ok &= ~((np.abs(sl[:, 0] - sl[:, 2]) > msd) | (np.abs(sl[:, 1] - sl[:, 2]) > msd))
+                cb, x = cb[ok], x[ok]
+                if not len(cb):
+                    continue
+                o = np.argsort(x, axis=1, kind="stable")
+                xs, hs = np.take_along_axis(x, o, 1), np.take_along_axis(cb, o, 1)
+                gap = np.diff(xs, axis=1)
+                w = np.full(xs.shape, -np.inf)
+                w[:, 1:] = gap
+                w[:, :-1] = np.maximum(w[:, :-1], gap)
+                np.maximum.at(wid, hs.ravel(), w.ravel())
+                kept[hs.ravel()] = True
+    out = []
+    for i in np.flatnonzero(kept):
+        h = H[i]
+        h["maxd"] = max(h["maxd"], float(wid[i]))
+        out.append(h)
+    return sorted(out, key=lambda h: h["idx"])
 
 
 def attach_opposite_side_pulses(clusters, hits, max_dt):

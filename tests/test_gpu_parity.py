@@ -120,6 +120,53 @@ def test_clusters_match_in_every_event_size_class(setup):
     _cluster_compare(out[0], ref)
 
 
+def _dense_pulses(rng, n_events):
+    """Pulse tables of shower-like events: bars of 3 to 24 adjacent columns over two modules fire on every layer
+    within a few ns, four SiPMs each, with secondary and late pulses (late ones test the time window hit by hit).
+    Rows sorted by (event, bar, SiPM, time) with seq the pulse's rank on its SiPM."""
+    rows = []
+    for e in range(n_events):
+        ncol = int(rng.integers(3, 25))
+        c0 = int(rng.integers(0, 2 * DT.N_COLS - ncol + 1))
+        m0 = int(rng.integers(0, DT.N_MODULES - 1))
+        occ = rng.uniform(0.4, 1.0)
+        t0 = rng.uniform(100.0, 1500.0)
+        for layer in range(DT.N_LAYERS):
+            for gc in range(c0, c0 + ncol):
+                if rng.random() > occ:
+                    continue
+                bar = int(DT.counter_id(m0 + gc // DT.N_COLS, layer, gc % DT.N_COLS))
+                for sp in range(4):
+                    if rng.random() < 0.1:
+                        continue
+                    ts = [t0 + rng.normal(0.0, 2.0)]
+                    if rng.random() < 0.15:
+                        ts.append(ts[0] + rng.uniform(8.0, 20.0))
+                    if rng.random() < 0.1:
+                        ts[0] += rng.uniform(6.0, 14.0)
+                    for k, tt in enumerate(sorted(ts)):
+                        rows.append((e, bar, sp, tt, rng.uniform(3.0, 40.0), k))
+    a = np.array(rows)
+    o = np.lexsort((a[:, 3], a[:, 2], a[:, 1], a[:, 0]))
+    a = a[o]
+    return (a[:, 0].astype(np.int64), a[:, 1].astype(np.int64), a[:, 2].astype(np.int64), a[:, 3].copy(),
+            a[:, 4].copy(), a[:, 5].astype(np.int64))
+
+
+def test_clusters_match_on_dense_events():
+    """Large clusters (up to about 10^5 layer combinations on a side) in events of every finder size class."""
+    cfg = CF.ChainConfig()
+    cond = DT.nominal_conditions()
+    G = DT.sector_geom()
+    ev, bar, sipm, time, pe, seq = _dense_pulses(np.random.default_rng(21), 24)
+    npl = np.bincount(ev)
+    assert (npl > 256).any() and ((npl > 128) & (npl <= 256)).any() and (npl <= 128).any()
+    out = CH.find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G)
+    ref = OC.find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G)
+    assert len(ref["cl_event"]) >= 20
+    _cluster_compare(out, ref)
+
+
 def test_clusters_match_reference_golden(golden):
     cfg = CF.ChainConfig()
     cond = DT.nominal_conditions()

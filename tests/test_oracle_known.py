@@ -445,3 +445,40 @@ def test_coinc_attachment_rescues_min_cluster_pes():
     p = _seeds() + [_pulse(13, 0, 1, 105.)]
     assert _members(_run(p, cfg)) == [[10, 11, 12, 13]]
     assert _run(p, replace(cfg, add_opposite_side_pulses=False)) == []
+
+
+# --- checkCoincidence: the numpy path equals the combination loop ----------------------------------------------------
+
+def _random_cluster(rng, n_per_layer, mixed):
+    cuts = [OCo.SectorCuts(pe_threshold=8.0, max_dt_adjacent=10.0, max_dt=10.0, min_slope=-6.0, max_slope=6.0,
+                           max_slope_diff=2.0, coincidence_layers=3, min_cluster_pes=10.0, initial_max_distance=100.0,
+                           final_max_distance=60.0),
+            OCo.SectorCuts(pe_threshold=8.0, max_dt_adjacent=10.0, max_dt=6.0, min_slope=-2.0, max_slope=3.0,
+                           max_slope_diff=1.0, coincidence_layers=2, min_cluster_pes=10.0, initial_max_distance=100.0,
+                           final_max_distance=40.0)]
+    hits, idx = [], 0
+    for layer, k in enumerate(n_per_layer):
+        for _ in range(k):
+            col = int(rng.integers(0, 6))
+            c = cuts[int(rng.integers(0, 2))] if mixed else cuts[0]
+            t = float(rng.choice([0.0, 4.0, 9.0, 12.5]) + rng.integers(0, 3) * 0.5)   # repeated times, window edges
+            hits.append(dict(idx=idx, layer=layer, x=50.0 * col, y=20.0 * layer, t=t, cuts=c, maxd=100.0))
+            idx += 1
+    rng.shuffle(hits)
+    return hits
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_check_coincidence_numpy_path_equals_combination_loop(mixed, monkeypatch):
+    """Hits on few columns (x ties, so the stable sort by x matters), times on a coarse grid (window edges hit
+    exactly), and per-hit cuts from two sectors (skip rule, widest window and slope range of a combination)."""
+    rng = np.random.default_rng(11 + mixed)
+    cfg = OCo.FinderConfig()
+    for trial in range(40):
+        cl = _random_cluster(rng, [int(v) for v in rng.integers(0, 5, 4)], mixed)
+        monkeypatch.setattr(OCo, "NUMPY_COMBOS", 1 << 40)
+        loop = OCo.check_coincidence([dict(h) for h in cl], cfg)
+        monkeypatch.setattr(OCo, "NUMPY_COMBOS", -1)
+        vec = OCo.check_coincidence([dict(h) for h in cl], cfg)
+        assert [h["idx"] for h in vec] == [h["idx"] for h in loop], trial
+        assert [h["maxd"] for h in vec] == [h["maxd"] for h in loop], trial
