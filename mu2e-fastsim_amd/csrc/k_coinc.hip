@@ -702,13 +702,30 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
 constexpr int BT = 256;      // threads of a block-finder block
 constexpr int BH = 12288;    // LDS histogram bins: (side, layer, counter column) buckets, time bins, bars
 constexpr int BIG_COMBOS = 512;
+constexpr int SMALL_CLUSTER = 24;  // larger clusters are checked by the whole block
 constexpr int INF_PASS = 1 << 29;
+constexpr int CM = 1024, CB = 1024;  // clustering staged in LDS: positions, time bins
+constexpr int TN = 1536, TG = 384;   // coincidence check staged in LDS: hits, bar groups
 
 struct BS {  // event-local scratch of the block finder (index: pulse in the event, or position in a list)
-  double *t, *pe, *x, *y, *maxd, *wm, *cmaxd, *wq, *cdt;
+  double *t, *pe, *x, *y, *maxd, *wm, *cmaxd, *wq, *cdt, *px, *pt, *pmd;
   int32_t *bar, *sec, *cit, *hl, *filt, *items, *off, *coinc, *byl, *cl2, *off2, *bsort, *tsort, *label, *pass, *chk,
       *sk, *sc, *clmc, *tmp, *clid, *tbin;
-  uint8_t *layer, *side, *sipm, *seq0, *kept, *fpass, *memf;
+  uint8_t *layer, *side, *sipm, *seq0, *kept, *fpass, *memf, *pside;
+};
+
+struct ClLds {  // findClusters of up to CM positions
+  double x[CM], t[CM], md[CM];
+  int32_t label[CM], pass[CM], tsort[CM], tbin[CM], bins[CB];
+  uint8_t side[CM];
+};
+
+struct TpLds {  // checkCoincidence of up to TN hits in up to TG bar groups
+  double ts[TN];
+  double gx[TG], gy[TG], gtmin[TG], gtmax[TG], gmdt[TG], glo[TG], ghi[TG], gmsd[TG], gw[TG];
+  int32_t key[TN], gs[TG + 1];
+  int8_t gcl[TG];
+  uint8_t gk[TG];
 };
 
 struct BlockLds {
@@ -716,7 +733,11 @@ struct BlockLds {
     hipcub::BlockScan<int, BT>::TempStorage scan;
     hipcub::BlockReduce<double, BT>::TempStorage red;
   } cub;
-  int hist[BH];
+  union {
+    int hist[BH];
+    ClLds cl;
+    TpLds tp;
+  } u;
   double dshare[4];
   int ishare[8];
   int flag;
@@ -729,159 +750,161 @@ __device__ __forceinline__ int bscan(BlockLds &L, int v, int &total) {
   return r;
 }
 
-// exclusive scan of hist[0..nb) in place (nb <= BH); returns the total
-__device__ int bscan_hist(BlockLds &L, int nb) {
+// exclusive scan of h[0..nb) in place (an LDS array); returns the total
+__device__ int bscan_arr(BlockLds &L, int *h, int nb) {
   const int per = (nb + BT - 1) / BT, a = threadIdx.x * per;
   int s = 0;
-  for (int i = a; i < a + per && i < nb; ++i) s += L.hist[i];
+  for (int i = a; i < a + per && i < nb; ++i) s += h[i];
   int tot;
   int run = bscan(L, s, tot);
   for (int i = a; i < a + per && i < nb; ++i) {
-    const int v = L.hist[i];
-    L.hist[i] = run;
+    const int v = h[i];
+    h[i] = run;
     run += v;
   }
   __syncthreads();
   return tot;
 }
 
-// in-place heap sort of v[0..n) by key(v) = (a[v] * big + v) ascending (a: pass number, ties by the value itself)
-__device__ void sort_by_pass(int32_t *v, int n, const int32_t *pass) {
-  auto key = [&](int32_t x) -> int64_t { return (int64_t)pass[x] * (1ll << 32) + x; };
-  auto sift = [&](int root, int end) {
-    while (2 * root + 1 < end) {
-      int ch = 2 * root + 1;
-      if (ch + 1 < end && key(v[ch + 1]) > key(v[ch])) ++ch;
-      if (key(v[root]) >= key(v[ch])) return;
-      const int32_t t = v[root];
-      v[root] = v[ch];
-      v[ch] = t;
-      root = ch;
-    }
-  };
-  for (int s = n / 2 - 1; s >= 0; --s) sift(s, n);
-  for (int e = n - 1; e > 0; --e) {
-    const int32_t t = v[0];
-    v[0] = v[e];
-    v[e] = t;
-    sift(0, e);
-  }
+__device__ __forceinline__ int bscan_hist(BlockLds &L, int nb) { return bscan_arr(L, L.u.hist, nb); }
+
+__device__ __forceinline__ double block_min(BlockLds &L, double v) {
+  v = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(v, hipcub::Min());
+  __syncthreads();
+  if (threadIdx.x == 0) L.dshare[3] = v;
+  __syncthreads();
+  v = L.dshare[3];
+  __syncthreads();
+  return v;
 }
 
-__device__ __forceinline__ bool blink(const BS &S, int h, int c, const double *md, double max_dt, bool opposite,
-                                      double dt_opp) {
-  const double mdd = fmax(md[h], md[c]);
-  const double mt = max_dt + ((opposite && S.side[h] != S.side[c]) ? dt_opp : 0.0);
-  return fabs(S.x[h] - S.x[c]) <= mdd && fabs(S.t[h] - S.t[c]) < mt;
+__device__ __forceinline__ double block_max(BlockLds &L, double v) {
+  v = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(v, hipcub::Max());
+  __syncthreads();
+  if (threadIdx.x == 0) L.dshare[3] = v;
+  __syncthreads();
+  v = L.dshare[3];
+  __syncthreads();
+  return v;
 }
 
-// greedy findClusters of the pulses list[0..m): members (pulses) to out[] in join order, cluster starts to off[];
-// returns the number of clusters
-__device__ int block_clusters(BlockLds &L, const BS &S, const int32_t *list, int m, const double *md, double max_dt,
-                              bool opposite, double dt_opp, int32_t *out, int32_t *off) {
+// greedy findClusters of the pulses list[0..m) (reco/coinc.py:125-152): members (pulses) to out[] in join order,
+// cluster starts to off[]; returns the number of clusters. Clusters are the link graph's connected components in
+// the order of their first position; a member's join order is (pass, position) with pass(seed) = 1 and
+// pass(p) = min over linked q of pass(q) + [q > p]. The positions' times, x, link distances and sides are staged in
+// LDS (LM) or in the scratch arrays px / pt / pmd / pside; link candidates come from time bins at least one link
+// window wide.
+template <bool LM>
+__device__ int block_clusters_t(BlockLds &L, const BS &S, const int32_t *list, int m, const double *md,
+                                double max_dt, bool opposite, double dt_opp, int32_t *out, int32_t *off) {
   const int tid = threadIdx.x;
-  if (m == 0) {
-    if (tid == 0) off[0] = 0;
-    __syncthreads();
-    return 0;
+  double *X, *T, *MD;
+  uint8_t *SD;
+  int32_t *LAB, *PAS, *TSO, *TBI, *BINS;
+  int maxb;
+  if constexpr (LM) {
+    X = L.u.cl.x, T = L.u.cl.t, MD = L.u.cl.md, SD = L.u.cl.side;
+    LAB = L.u.cl.label, PAS = L.u.cl.pass, TSO = L.u.cl.tsort, TBI = L.u.cl.tbin, BINS = L.u.cl.bins;
+    maxb = CB;
+  } else {
+    X = S.px, T = S.pt, MD = S.pmd, SD = S.pside;
+    LAB = S.label, PAS = S.pass, TSO = S.tsort, TBI = S.tbin, BINS = L.u.hist;
+    maxb = BH;
   }
-  // time bins at least one link window wide
   double lo = INFINITY, hi = -INFINITY;
   for (int i = tid; i < m; i += BT) {
-    lo = fmin(lo, S.t[list[i]]);
-    hi = fmax(hi, S.t[list[i]]);
+    const int h = list[i];
+    const double tt = S.t[h];
+    X[i] = S.x[h];
+    T[i] = tt;
+    MD[i] = md[h];
+    SD[i] = S.side[h];
+    lo = fmin(lo, tt);
+    hi = fmax(hi, tt);
   }
-  lo = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(lo, hipcub::Min());
-  __syncthreads();
-  hi = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(hi, hipcub::Max());
-  __syncthreads();
-  if (tid == 0) {
-    L.flag = 0;
-    L.dshare[0] = lo;
-    L.dshare[1] = hi;
-  }
-  __syncthreads();
-  lo = L.dshare[0];
-  hi = L.dshare[1];
-  __syncthreads();
+  lo = block_min(L, lo);
+  hi = block_max(L, hi);
   const double mt = max_dt + (opposite ? dt_opp : 0.0);
-  const double width = fmax(mt, (hi - lo) / (double)(BH - 2)) * (1.0 + 1e-12) + 1e-9;
-  const int nb = (int)((hi - lo) / width) + 1;
-  for (int b = tid; b < nb + 1; b += BT) L.hist[b] = 0;
+  const double span = hi - lo;
+  const bool fin = span >= 0.0 && span < 1e300;  // else one bin: every position is a candidate of every other
+  const double width = fin ? fmax(mt, span / (double)(maxb - 2)) * (1.0 + 1e-12) + 1e-9 : 1.0;
+  const int nb = fin ? min((int)(span / width) + 1, maxb - 1) : 1;
+  for (int b = tid; b < nb + 1; b += BT) BINS[b] = 0;
   __syncthreads();
   for (int i = tid; i < m; i += BT) {
-    const double fb = (S.t[list[i]] - lo) / width;
+    const double fb = fin ? (T[i] - lo) / width : 0.0;
     const int b = fb >= 0.0 ? min((int)fmin(fb, (double)(nb - 1)), nb - 1) : 0;  // NaN -> bin 0
-    S.tbin[i] = b;
-    atomicAdd(&L.hist[b], 1);
+    TBI[i] = b;
+    atomicAdd(&BINS[b], 1);
   }
   __syncthreads();
-  bscan_hist(L, nb + 1);
-  for (int i = tid; i < m; i += BT) S.tsort[atomicAdd(&L.hist[S.tbin[i]], 1)] = i;  // hist[b] ends as end of bin b
+  bscan_arr(L, BINS, nb + 1);
+  for (int i = tid; i < m; i += BT) TSO[atomicAdd(&BINS[TBI[i]], 1)] = i;  // BINS[b] ends as the end of bin b
   __syncthreads();
+  auto link = [&](int i, int j) {
+    const double mdd = fmax(MD[i], MD[j]);
+    const double w = max_dt + ((opposite && SD[i] != SD[j]) ? dt_opp : 0.0);
+    return fabs(X[i] - X[j]) <= mdd && fabs(T[i] - T[j]) < w;
+  };
   // components: min-label propagation with pointer jumping
   for (int i = tid; i < m; i += BT) {
-    S.label[i] = i;
-    S.pass[i] = INF_PASS;
+    LAB[i] = i;
+    PAS[i] = INF_PASS;
   }
   __syncthreads();
-  for (;;) {
-    if (tid == 0) L.flag = 0;
-    __syncthreads();
+  for (;;) {  // every thread leaves together: the vote is a barrier with a block-wide OR
+    int changed = 0;
     for (int i = tid; i < m; i += BT) {
-      const int h = list[i], b = S.tbin[i];
-      int best = S.label[i];
-      for (int bb = max(b - 1, 0); bb <= min(b + 1, nb - 1); ++bb)
-        for (int k = (bb ? L.hist[bb - 1] : 0); k < L.hist[bb]; ++k) {
-          const int j = S.tsort[k];
-          if (j == i) continue;
-          const int lj = S.label[j];
-          if (lj < best && blink(S, h, list[j], md, max_dt, opposite, dt_opp)) best = lj;
-        }
-      if (best < S.label[i]) {
-        atomicMin(&S.label[i], best);
-        L.flag = 1;
+      const int b = TBI[i];
+      int best = LAB[i];
+      const int k0 = b > 1 ? BINS[b - 2] : 0, k1 = BINS[min(b + 1, nb - 1)];
+      for (int k = k0; k < k1; ++k) {
+        const int j = TSO[k];
+        const int lj = LAB[j];
+        if (lj < best && j != i && link(i, j)) best = lj;
+      }
+      if (best < LAB[i]) {
+        atomicMin(&LAB[i], best);
+        changed = 1;
       }
     }
     __syncthreads();
     for (int i = tid; i < m; i += BT) {
-      int l = S.label[i];
-      while (S.label[l] < l) l = S.label[l];
-      S.label[i] = l;
+      int l = LAB[i];
+      while (LAB[l] < l) l = LAB[l];
+      LAB[i] = l;
     }
     __syncthreads();
-    if (!L.flag) break;
+    if (!__syncthreads_or(changed)) break;
   }
   // pass numbers: 0-1 relaxation from the seeds (a component's lowest position)
   for (int i = tid; i < m; i += BT)
-    if (S.label[i] == i) S.pass[i] = 1;
+    if (LAB[i] == i) PAS[i] = 1;
   __syncthreads();
   for (;;) {
-    if (tid == 0) L.flag = 0;
-    __syncthreads();
+    int changed = 0;
     for (int i = tid; i < m; i += BT) {
-      if (S.label[i] == i) continue;
-      const int h = list[i], b = S.tbin[i];
-      int best = S.pass[i];
-      for (int bb = max(b - 1, 0); bb <= min(b + 1, nb - 1); ++bb)
-        for (int k = (bb ? L.hist[bb - 1] : 0); k < L.hist[bb]; ++k) {
-          const int j = S.tsort[k];
-          if (j == i) continue;
-          const int cand = S.pass[j] + (j > i ? 1 : 0);
-          if (cand < best && blink(S, h, list[j], md, max_dt, opposite, dt_opp)) best = cand;
-        }
-      if (best < S.pass[i]) {
-        S.pass[i] = best;
-        L.flag = 1;
+      if (LAB[i] == i) continue;
+      const int b = TBI[i];
+      int best = PAS[i];
+      const int k0 = b > 1 ? BINS[b - 2] : 0, k1 = BINS[min(b + 1, nb - 1)];
+      for (int k = k0; k < k1; ++k) {
+        const int j = TSO[k];
+        const int cand = PAS[j] + (j > i ? 1 : 0);
+        if (cand < best && j != i && link(i, j)) best = cand;
+      }
+      if (best < PAS[i]) {
+        PAS[i] = best;
+        changed = 1;
       }
     }
     __syncthreads();
-    if (!L.flag) break;
+    if (!__syncthreads_or(changed)) break;
   }
   // clusters in seed order; members by (pass, position)
   for (int i = tid; i < m; i += BT) S.tmp[i] = 0;
   __syncthreads();
-  for (int i = tid; i < m; i += BT) atomicAdd(&S.tmp[S.label[i]], 1);
+  for (int i = tid; i < m; i += BT) atomicAdd(&S.tmp[LAB[i]], 1);
   __syncthreads();
   int base = 0, cbase = 0;
   for (int c = 0; c < m; c += BT) {  // member offsets and cluster numbers over seeds in position order
@@ -900,13 +923,32 @@ __device__ int block_clusters(BlockLds &L, const BS &S, const int32_t *list, int
   }
   __syncthreads();
   if (tid == 0) off[cbase] = base;
-  for (int i = tid; i < m; i += BT) S.items[atomicAdd(&S.tmp[S.label[i]], 1)] = i;  // positions, unordered per cluster
+  int32_t *ITM = LM ? TBI : S.items;  // the bins are no longer needed
+  for (int i = tid; i < m; i += BT) ITM[atomicAdd(&S.tmp[LAB[i]], 1)] = i;  // positions, unordered per cluster
   __syncthreads();
-  for (int c = tid; c < cbase; c += BT) sort_by_pass(S.items + off[c], off[c + 1] - off[c], S.pass);
-  __syncthreads();
-  for (int k = tid; k < base; k += BT) out[k] = list[S.items[k]];
+  for (int k = tid; k < base; k += BT) {  // rank by (pass, position) within the cluster
+    const int i = ITM[k], c = S.clid[LAB[i]], pi = PAS[i];
+    const int o = off[c], e = off[c + 1];
+    int r = 0;
+    for (int q = o; q < e; ++q) {
+      const int j = ITM[q], pj = PAS[j];
+      r += pj < pi || (pj == pi && j < i);
+    }
+    out[o + r] = list[i];
+  }
   __syncthreads();
   return cbase;
+}
+
+__device__ int block_clusters(BlockLds &L, const BS &S, const int32_t *list, int m, const double *md, double max_dt,
+                              bool opposite, double dt_opp, int32_t *out, int32_t *off) {
+  if (m == 0) {
+    if (threadIdx.x == 0) off[0] = 0;
+    __syncthreads();
+    return 0;
+  }
+  return m <= CM ? block_clusters_t<true>(L, S, list, m, md, max_dt, opposite, dt_opp, out, off)
+                 : block_clusters_t<false>(L, S, list, m, md, max_dt, opposite, dt_opp, out, off);
 }
 
 // checkCoincidence of a cluster's members mem[0..n) for one thread; by-layer lists in byl[0..n) with their widened
@@ -1079,20 +1121,55 @@ __device__ __forceinline__ void gmax_double(double *a, double w) {
 
 // A cluster's hits grouped by bar (layer, then bar, then time): every hit of a group has the bar's x, y and sector
 // cuts, so the slope tests, the skip test and the widened distances of a layer combination depend only on the groups
-// it takes; only its time window depends on the hits.
-struct Groups {
-  const int32_t *byl;  // hits by (layer, bar, time, pulse)
-  const int32_t *gs;   // group g holds byl[gs[g] .. gs[g+1])
-  double *wq;          // per hit: widened distance
-  uint8_t *kept;       // per hit: in some passing combination
-  double *gw;          // per group: widening from combinations that pass for every hit of the group
-  uint8_t *gk;         // per group: some combination passes for every hit of the group
+// it takes; only its time window depends on the hits. TupL reads groups staged in LDS, TupG reads them through the
+// hit arrays in global memory.
+struct TupL {
+  TpLds &T;
+  __device__ double x(int g) const { return T.gx[g]; }
+  __device__ double y(int g) const { return T.gy[g]; }
+  __device__ double tmin(int g) const { return T.gtmin[g]; }
+  __device__ double tmax(int g) const { return T.gtmax[g]; }
+  __device__ void cut(int g, int &cl, double &mdt, double &lo, double &hi, double &msd) const {
+    cl = T.gcl[g], mdt = T.gmdt[g], lo = T.glo[g], hi = T.ghi[g], msd = T.gmsd[g];
+  }
+  __device__ int h0(int g) const { return T.gs[g]; }
+  __device__ int h1(int g) const { return T.gs[g + 1]; }
+  __device__ double ht(int k) const { return T.ts[k]; }
+  __device__ void mark(int g, double w) const {
+    gmax_double(&T.gw[g], w);
+    T.gk[g] = 1;
+  }
+};
+
+struct TupG {
+  const BS &S;
+  const Tables &t;
+  const int32_t *byl, *gs;
+  double *gw;
+  uint8_t *gk;
+  __device__ int rep(int g) const { return byl[gs[g]]; }
+  __device__ double x(int g) const { return S.x[rep(g)]; }
+  __device__ double y(int g) const { return S.y[rep(g)]; }
+  __device__ double tmin(int g) const { return S.t[rep(g)]; }
+  __device__ double tmax(int g) const { return S.t[byl[gs[g + 1] - 1]]; }
+  __device__ void cut(int g, int &cl, double &mdt, double &lo, double &hi, double &msd) const {
+    const Cut c = cut_of(t, S.sec[rep(g)]);
+    cl = (int)c.coincidence_layers, mdt = c.max_dt, lo = c.min_slope, hi = c.max_slope, msd = c.max_slope_diff;
+  }
+  __device__ int h0(int g) const { return gs[g]; }
+  __device__ int h1(int g) const { return gs[g + 1]; }
+  __device__ double ht(int k) const { return S.t[byl[k]]; }
+  __device__ void mark(int g, double w) const {
+    gmax_double(&gw[g], w);
+    gk[g] = 1;
+  }
 };
 
 // the first time >= m in group g is within mdt of m (times ascend within a group)
-__device__ __forceinline__ bool group_covers(const BS &S, const Groups &G, int g, double m, double mdt) {
-  for (int k = G.gs[g]; k < G.gs[g + 1]; ++k) {
-    const double u = S.t[G.byl[k]];
+template <class A>
+__device__ __forceinline__ bool tup_covers(const A &G, int g, double m, double mdt) {
+  for (int k = G.h0(g); k < G.h1(g); ++k) {
+    const double u = G.ht(k);
     if (u >= m) return !(u - m > mdt);
   }
   return false;
@@ -1100,21 +1177,22 @@ __device__ __forceinline__ bool group_covers(const BS &S, const Groups &G, int g
 
 // whether hit time th of tuple member i takes part in a combination of one hit per group within the time window:
 // a combination passes iff every hit is within mdt of its earliest hit, which is th itself or a hit u < th of
-// another group (then every other group needs a hit in [u, u + mdt], the same rounding as max - min)
-__device__ bool tuple_time_ok(const BS &S, const Groups &G, const int *g, int nl, int i, double th, double mdt) {
+// another group (then every other group needs a hit in [u, u + mdt]; the same rounding as max - min)
+template <class A>
+__device__ bool tup_time_ok(const A &G, const int *g, int nl, int i, double th, double mdt) {
   bool ok = true;
   for (int j = 0; j < nl && ok; ++j)
-    if (j != i) ok = group_covers(S, G, g[j], th, mdt);
+    if (j != i) ok = tup_covers(G, g[j], th, mdt);
   if (ok) return true;
   for (int j = 0; j < nl; ++j) {
     if (j == i) continue;
-    for (int k = G.gs[g[j]]; k < G.gs[g[j] + 1]; ++k) {
-      const double u = S.t[G.byl[k]];
+    for (int k = G.h0(g[j]); k < G.h1(g[j]); ++k) {
+      const double u = G.ht(k);
       if (!(u < th)) break;
       if (th - u > mdt) continue;
       bool all = true;
       for (int j2 = 0; j2 < nl && all; ++j2)
-        if (j2 != i && j2 != j) all = group_covers(S, G, g[j2], u, mdt);
+        if (j2 != i && j2 != j) all = tup_covers(G, g[j2], u, mdt);
       if (all) return true;
     }
   }
@@ -1123,21 +1201,23 @@ __device__ bool tuple_time_ok(const BS &S, const Groups &G, const int *g, int nl
 
 // one tuple of groups g[0..nl) in layer order with its consecutive slopes sl[0..nl-1): the combination tests of
 // reference reco/coinc.py:156-181 with the tuple's cuts, then the widening of every hit that takes part
-__device__ __noinline__ void tuple_leaf(const Tables &t, const BS &S, const Groups &G, const int *g, int nl, const double *sl) {
+template <class A>
+__device__ __noinline__ void tup_leaf(const A &G, const int *g, int nl, const double *sl, double *wq, uint8_t *kept) {
   bool skip = true;
   double mdt = -INFINITY, lo = INFINITY, hi = -INFINITY, msd = -INFINITY, m_all = INFINITY, M_all = -INFINITY;
   double xs[4];
   for (int i = 0; i < nl; ++i) {
-    const int h = G.byl[G.gs[g[i]]];
-    const Cut c = cut_of(t, S.sec[h]);
-    if (!((int)c.coincidence_layers > nl)) skip = false;
-    mdt = fmax(mdt, c.max_dt);
-    lo = fmin(lo, c.min_slope);
-    hi = fmax(hi, c.max_slope);
-    msd = fmax(msd, c.max_slope_diff);
-    xs[i] = S.x[h];
-    m_all = fmin(m_all, S.t[h]);
-    M_all = fmax(M_all, S.t[G.byl[G.gs[g[i] + 1] - 1]]);
+    int cl;
+    double cmdt, clo, chi, cmsd;
+    G.cut(g[i], cl, cmdt, clo, chi, cmsd);
+    if (!(cl > nl)) skip = false;
+    mdt = fmax(mdt, cmdt);
+    lo = fmin(lo, clo);
+    hi = fmax(hi, chi);
+    msd = fmax(msd, cmsd);
+    xs[i] = G.x(g[i]);
+    m_all = fmin(m_all, G.tmin(g[i]));
+    M_all = fmax(M_all, G.tmax(g[i]));
   }
   if (skip) return;
   for (int i = 0; i + 1 < nl; ++i)
@@ -1163,34 +1243,79 @@ __device__ __noinline__ void tuple_leaf(const Tables &t, const BS &S, const Grou
     w[ord[i]] = wi;
   }
   if (!(M_all - m_all > mdt)) {  // every choice of hits is within the window
-    for (int i = 0; i < nl; ++i) {
-      gmax_double(&G.gw[g[i]], w[i]);
-      G.gk[g[i]] = 1;
-    }
+    for (int i = 0; i < nl; ++i) G.mark(g[i], w[i]);
     return;
   }
   for (int i = 0; i < nl; ++i)
-    for (int k = G.gs[g[i]]; k < G.gs[g[i] + 1]; ++k)
-      if (tuple_time_ok(S, G, g, nl, i, S.t[G.byl[k]], mdt)) {
-        gmax_double(&G.wq[k], w[i]);
-        G.kept[k] = 1;
+    for (int k = G.h0(g[i]); k < G.h1(g[i]); ++k)
+      if (tup_time_ok(G, g, nl, i, G.ht(k), mdt)) {
+        gmax_double(&wq[k], w[i]);
+        kept[k] = 1;
       }
 }
 
-__device__ __forceinline__ double group_slope(const BS &S, const Groups &G, int ga, int gb) {
-  const int a = G.byl[G.gs[ga]], b = G.byl[G.gs[gb]];
-  return (S.x[b] - S.x[a]) / (S.y[b] - S.y[a]);
+template <class A>
+__device__ __forceinline__ double tup_slope(const A &G, int ga, int gb) {
+  return (G.x(gb) - G.x(ga)) / (G.y(gb) - G.y(ga));
 }
 
-// checkCoincidence (reference reco/coinc.py:184-207) of the cluster at o0 (n hits in byl, layer order) by the whole
-// block over tuples of bars, pruned by the widest slope and slope-difference cuts in the cluster. Leaves byl sorted by
-// (layer, bar, time, pulse) with wq / kept per position. Returns false, with byl untouched, when a time is not finite
-// (the caller then enumerates hit combinations).
-__device__ __noinline__ bool block_tuple_check(BlockLds &L, const Tables &t, const BS &S, int o0, int n) {
+// every tuple of the needed layer sets, split over the block by its first two groups and pruned by the widest slope
+// range and slope difference of the cluster (a pruned tuple fails its own, narrower cuts too)
+template <class A>
+__device__ void tup_search(const A &G, const int lgo[5], int nmin, int nmax, double lo, double hi, double msd,
+                           double *wq, uint8_t *kept) {
+  for (int nl = 2; nl <= 4; ++nl) {
+    const bool need = nl == 2 ? nmin == 2 : nl == 3 ? (nmin <= 3 && 3 <= nmax) : nmax == 4;
+    if (!need) continue;
+    int Lc[4];
+    for (int i = 0; i < nl; ++i) Lc[i] = i;
+    for (;;) {
+      const int n1 = lgo[Lc[1] + 1] - lgo[Lc[1]];
+      const int P01 = (lgo[Lc[0] + 1] - lgo[Lc[0]]) * n1;
+      bool empty = false;
+      for (int i = 2; i < nl; ++i) empty |= lgo[Lc[i] + 1] == lgo[Lc[i]];
+      for (int ci = empty ? P01 : (int)threadIdx.x; ci < P01; ci += BT) {
+        int g[4];
+        double sl[3];
+        g[0] = lgo[Lc[0]] + ci / n1;
+        g[1] = lgo[Lc[1]] + ci % n1;
+        sl[0] = tup_slope(G, g[0], g[1]);
+        if (sl[0] < lo || sl[0] > hi) continue;
+        if (nl == 2) {
+          tup_leaf(G, g, 2, sl, wq, kept);
+          continue;
+        }
+        for (g[2] = lgo[Lc[2]]; g[2] < lgo[Lc[2] + 1]; ++g[2]) {
+          sl[1] = tup_slope(G, g[1], g[2]);
+          if (sl[1] < lo || sl[1] > hi || fabs(sl[0] - sl[1]) > msd) continue;
+          if (nl == 3) {
+            tup_leaf(G, g, 3, sl, wq, kept);
+            continue;
+          }
+          for (g[3] = lgo[Lc[3]]; g[3] < lgo[Lc[3] + 1]; ++g[3]) {
+            sl[2] = tup_slope(G, g[2], g[3]);
+            if (sl[2] < lo || sl[2] > hi || fabs(sl[0] - sl[2]) > msd || fabs(sl[1] - sl[2]) > msd) continue;
+            tup_leaf(G, g, 4, sl, wq, kept);
+          }
+        }
+      }
+      int k = nl - 1;
+      while (k >= 0 && Lc[k] == 4 - nl + k) --k;
+      if (k < 0) break;
+      ++Lc[k];
+      for (int i = k + 1; i < nl; ++i) Lc[i] = Lc[i - 1] + 1;
+    }
+  }
+}
+
+// checkCoincidence (reference reco/coinc.py:184-207) of a cluster mem[0..n) by the whole block. Writes its hits to
+// byl[0..n) sorted by (layer, bar, time, pulse) with wq / kept per position. Returns 2 when the big-cluster rule keeps
+// every member with its initial distance, else 1. Finite times: a search over tuples of bar groups (staged in LDS
+// when the cluster fits); otherwise every hit combination, as the reference enumerates them.
+__device__ __noinline__ int block_check(BlockLds &L, const crv_params &p, const Tables &t, const BS &S,
+                                        const int32_t *mem, int n, int32_t *byl, double *wq, uint8_t *kept,
+                                        int32_t *gs, int32_t *gid, double *gw, uint8_t *gk) {
   const int tid = threadIdx.x;
-  int32_t *byl = S.byl + o0, *srt = S.items + o0, *gs = S.label + o0, *gid = S.tbin + o0;
-  double *wq = S.wq + o0, *gw = S.cdt + o0;
-  uint8_t *kept = S.kept + o0, *gk = S.memf + o0;
   if (tid == 0) {
     L.flag = 0;
     for (int l = 0; l < 8; ++l) L.ishare[l] = 0;
@@ -1199,9 +1324,9 @@ __device__ __noinline__ bool block_tuple_check(BlockLds &L, const Tables &t, con
   }
   __syncthreads();
   double lo = INFINITY, hi = -INFINITY, msd = -INFINITY;
-  int nmin = 1 << 30, nmax = -1;
+  int nmin = 1 << 30, nmax = -1, lmask = 0;
   for (int i = tid; i < n; i += BT) {
-    const int h = byl[i];
+    const int h = mem[i];
     if (!isfinite(S.t[h])) L.flag = 1;
     const Cut c = cut_of(t, S.sec[h]);
     lo = fmin(lo, c.min_slope);
@@ -1209,48 +1334,58 @@ __device__ __noinline__ bool block_tuple_check(BlockLds &L, const Tables &t, con
     msd = fmax(msd, c.max_slope_diff);
     nmin = min(nmin, (int)c.coincidence_layers);
     nmax = max(nmax, (int)c.coincidence_layers);
+    lmask |= 1 << S.layer[h];
   }
   atomicMin(&L.ishare[5], nmin);
   atomicMax(&L.ishare[6], nmax);
-  lo = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(lo, hipcub::Min());
-  __syncthreads();
-  if (tid == 0) L.dshare[0] = lo;
-  hi = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(hi, hipcub::Max());
-  __syncthreads();
-  if (tid == 0) L.dshare[1] = hi;
-  msd = hipcub::BlockReduce<double, BT>(L.cub.red).Reduce(msd, hipcub::Max());
-  __syncthreads();
-  if (tid == 0) L.dshare[2] = msd;
-  __syncthreads();
-  if (L.flag) {
-    __syncthreads();
-    return false;
-  }
-  lo = L.dshare[0];
-  hi = L.dshare[1];
-  msd = L.dshare[2];
+  atomicOr(&L.ishare[7], lmask);
+  lo = block_min(L, lo);
+  hi = block_max(L, hi);
+  msd = block_max(L, msd);
+  const bool fin = !L.flag;
   nmin = L.ishare[5];
   nmax = L.ishare[6];
-  // rank sort by (layer, bar, time, pulse): a total order, so ranks are a permutation
+  lmask = L.ishare[7];
+  __syncthreads();
+  if (n > p.big_cluster_threshold && __popc(lmask) >= nmin) return 2;
+  // rank sort by (layer, bar, time, pulse) (time left out when a time is not finite): a total order
+  const bool lm = n <= TN;
+  TpLds &T = L.u.tp;
+  if (lm) {
+    for (int i = tid; i < n; i += BT) {
+      const int h = mem[i];
+      T.key[i] = S.layer[h] * t.nbars + S.bar[h];
+      T.ts[i] = fin ? S.t[h] : 0.0;
+    }
+    __syncthreads();
+  }
   for (int i = tid; i < n; i += BT) {
-    const int h = byl[i], ka = S.layer[h] * t.nbars + S.bar[h];
-    const double ta = S.t[h];
+    const int h = mem[i];
+    const int ka = lm ? T.key[i] : S.layer[h] * t.nbars + S.bar[h];
+    const double ta = lm ? T.ts[i] : fin ? S.t[h] : 0.0;
     int r = 0;
     for (int j = 0; j < n; ++j) {
-      const int hj = byl[j], kb = S.layer[hj] * t.nbars + S.bar[hj];
-      const double tb = S.t[hj];
+      const int hj = mem[j];
+      const int kb = lm ? T.key[j] : S.layer[hj] * t.nbars + S.bar[hj];
+      const double tb = lm ? T.ts[j] : fin ? S.t[hj] : 0.0;
       r += kb < ka || (kb == ka && (tb < ta || (tb == ta && hj < h)));
     }
-    srt[r] = h;
+    byl[r] = h;
   }
   __syncthreads();
   for (int q = tid; q < n; q += BT) {
-    const int h = srt[q];
-    byl[q] = h;
+    const int h = byl[q];
     wq[q] = cut_of(t, S.sec[h]).final_max_distance;
     kept[q] = 0;
   }
   __syncthreads();
+  if (!fin) {  // every hit combination, layer by layer
+    int lay[5], cmin, cmax;
+    cluster_layout(t, S, byl, n, lay, cmin, cmax);
+    combos(t, S, byl, wq, kept, lay, cmin, cmax, tid, BT, true);
+    __syncthreads();
+    return 1;
+  }
   // groups: runs of one bar
   int ng = 0;
   for (int c = 0; c < n; c += BT) {
@@ -1273,64 +1408,47 @@ __device__ __noinline__ bool block_tuple_check(BlockLds &L, const Tables &t, con
   int lgo[5];
   lgo[0] = 0;
   for (int l = 0; l < 4; ++l) lgo[l + 1] = lgo[l] + L.ishare[l];
-  for (int g = tid; g < ng; g += BT) {
-    gw[g] = -INFINITY;
-    gk[g] = 0;
-  }
   __syncthreads();
-  const Groups G{byl, gs, wq, kept, gw, gk};
-  for (int nl = 2; nl <= 4; ++nl) {
-    const bool need = nl == 2 ? nmin == 2 : nl == 3 ? (nmin <= 3 && 3 <= nmax) : nmax == 4;
-    if (!need) continue;
-    int Lc[4];
-    for (int i = 0; i < nl; ++i) Lc[i] = i;
-    for (;;) {
-      const int n0 = lgo[Lc[0] + 1] - lgo[Lc[0]], n1 = lgo[Lc[1] + 1] - lgo[Lc[1]];
-      const int P01 = n0 * n1;
-      bool empty = false;
-      for (int i = 2; i < nl; ++i) empty |= lgo[Lc[i] + 1] == lgo[Lc[i]];
-      for (int ci = empty ? P01 : tid; ci < P01; ci += BT) {
-        int g[4];
-        double sl[3];
-        g[0] = lgo[Lc[0]] + ci / n1;
-        g[1] = lgo[Lc[1]] + ci % n1;
-        sl[0] = group_slope(S, G, g[0], g[1]);
-        if (sl[0] < lo || sl[0] > hi) continue;
-        if (nl == 2) {
-          tuple_leaf(t, S, G, g, 2, sl);
-          continue;
-        }
-        for (g[2] = lgo[Lc[2]]; g[2] < lgo[Lc[2] + 1]; ++g[2]) {
-          sl[1] = group_slope(S, G, g[1], g[2]);
-          if (sl[1] < lo || sl[1] > hi || fabs(sl[0] - sl[1]) > msd) continue;
-          if (nl == 3) {
-            tuple_leaf(t, S, G, g, 3, sl);
-            continue;
-          }
-          for (g[3] = lgo[Lc[3]]; g[3] < lgo[Lc[3] + 1]; ++g[3]) {
-            sl[2] = group_slope(S, G, g[2], g[3]);
-            if (sl[2] < lo || sl[2] > hi || fabs(sl[0] - sl[2]) > msd || fabs(sl[1] - sl[2]) > msd) continue;
-            tuple_leaf(t, S, G, g, 4, sl);
-          }
-        }
-      }
-      int k = nl - 1;
-      while (k >= 0 && Lc[k] == 4 - nl + k) --k;
-      if (k < 0) break;
-      ++Lc[k];
-      for (int i = k + 1; i < nl; ++i) Lc[i] = Lc[i - 1] + 1;
+  const bool lg = lm && ng <= TG;
+  if (lg) {
+    for (int q = tid; q < n; q += BT) T.ts[q] = S.t[byl[q]];
+    for (int g = tid; g <= ng; g += BT) T.gs[g] = g < ng ? gs[g] : n;
+    __syncthreads();
+    for (int g = tid; g < ng; g += BT) {
+      const int h = byl[T.gs[g]];
+      const Cut c = cut_of(t, S.sec[h]);
+      T.gx[g] = S.x[h];
+      T.gy[g] = S.y[h];
+      T.gtmin[g] = T.ts[T.gs[g]];
+      T.gtmax[g] = T.ts[T.gs[g + 1] - 1];
+      T.gcl[g] = (int8_t)max(-128, min(127, (int)c.coincidence_layers));
+      T.gmdt[g] = c.max_dt;
+      T.glo[g] = c.min_slope;
+      T.ghi[g] = c.max_slope;
+      T.gmsd[g] = c.max_slope_diff;
+      T.gw[g] = -INFINITY;
+      T.gk[g] = 0;
     }
+    __syncthreads();
+    tup_search(TupL{T}, lgo, nmin, nmax, lo, hi, msd, wq, kept);
+  } else {
+    for (int g = tid; g < ng; g += BT) {
+      gw[g] = -INFINITY;
+      gk[g] = 0;
+    }
+    __syncthreads();
+    tup_search(TupG{S, t, byl, gs, gw, gk}, lgo, nmin, nmax, lo, hi, msd, wq, kept);
   }
   __syncthreads();
   for (int q = tid; q < n; q += BT) {
     const int g = gid[q];
-    if (gk[g]) {
+    if (lg ? T.gk[g] : gk[g]) {
       kept[q] = 1;
-      wq[q] = fmax(wq[q], gw[g]);
+      wq[q] = fmax(wq[q], lg ? T.gw[g] : gw[g]);
     }
   }
   __syncthreads();
-  return true;
+  return 1;
 }
 
 struct BlockAcc {
@@ -1358,13 +1476,13 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
     const int64_t sb = a + g;
     BS S = s0;
     {
-      double **dp[] = {&S.t, &S.pe, &S.x, &S.y, &S.maxd, &S.wm, &S.cmaxd, &S.wq, &S.cdt};
+      double **dp[] = {&S.t, &S.pe, &S.x, &S.y, &S.maxd, &S.wm, &S.cmaxd, &S.wq, &S.cdt, &S.px, &S.pt, &S.pmd};
       for (auto q : dp) *q += sb;
       int32_t **ip[] = {&S.bar, &S.sec, &S.cit, &S.hl, &S.filt, &S.items, &S.off, &S.coinc, &S.byl, &S.cl2, &S.off2,
                         &S.bsort, &S.tsort, &S.label, &S.pass, &S.chk, &S.sk, &S.sc, &S.clmc, &S.tmp, &S.clid,
                         &S.tbin};
       for (auto q : ip) *q += sb;
-      uint8_t **up[] = {&S.layer, &S.side, &S.sipm, &S.seq0, &S.kept, &S.fpass, &S.memf};
+      uint8_t **up[] = {&S.layer, &S.side, &S.sipm, &S.seq0, &S.kept, &S.fpass, &S.memf, &S.pside};
       for (auto q : up) *q += sb;
     }
     int32_t *cl_moff_e = cl_moff + sb;
@@ -1420,21 +1538,21 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
       if (nh == 0) continue;
       // filterHits (reco/coinc.py:154-182): buckets of (side, layer, counter column) in hit order
       const int nbk = 8 * t.ncit;
-      for (int b = tid; b < nbk + 1; b += BT) L.hist[b] = 0;
+      for (int b = tid; b < nbk + 1; b += BT) L.u.hist[b] = 0;
       __syncthreads();
       for (int ii = tid; ii < nh; ii += BT) {
         const int h = S.hl[ii];
-        atomicAdd(&L.hist[(S.side[h] * 4 + S.layer[h]) * t.ncit + S.cit[h]], 1);
+        atomicAdd(&L.u.hist[(S.side[h] * 4 + S.layer[h]) * t.ncit + S.cit[h]], 1);
       }
       __syncthreads();
       bscan_hist(L, nbk + 1);
       for (int ii = tid; ii < nh; ii += BT) {
         const int h = S.hl[ii];
-        S.bsort[atomicAdd(&L.hist[(S.side[h] * 4 + S.layer[h]) * t.ncit + S.cit[h]], 1)] = h;
+        S.bsort[atomicAdd(&L.u.hist[(S.side[h] * 4 + S.layer[h]) * t.ncit + S.cit[h]], 1)] = h;
       }
       __syncthreads();  // hist[b] is now the end of bucket b; order each bucket by pulse (= hit order)
       for (int b = tid; b < nbk; b += BT) {
-        const int b0 = b ? L.hist[b - 1] : 0, b1 = L.hist[b];
+        const int b0 = b ? L.u.hist[b - 1] : 0, b1 = L.u.hist[b];
         for (int i = b0 + 1; i < b1; ++i) {
           const int v = S.bsort[i];
           int j = i - 1;
@@ -1454,7 +1572,7 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
         for (int d = -1; d <= 1; ++d) {
           if (cnh + d < 0 || cnh + d >= t.ncit) continue;
           const int b = row + cnh + d;
-          for (int k = b ? L.hist[b - 1] : 0; k < L.hist[b]; ++k) {
+          for (int k = b ? L.u.hist[b - 1] : 0; k < L.u.hist[b]; ++k) {
             const int q = S.bsort[k];
             if (fabs(S.t[q] - S.t[h]) > ch.max_dt_adjacent) continue;
             sums[d + 1] += S.pe[q];
@@ -1478,34 +1596,38 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
         __syncthreads();
         // the clusters go to cl2 / off2 as scratch here (the second clustering reuses them later)
         const int ncs = block_clusters(L, S, S.filt + f0, nfilt - f0, S.maxd, tmaxdt, false, 0.0, S.cl2, S.off2);
-        // coincidence check: one thread per cluster, the block for clusters with many combinations
+        // coincidence check: one thread per small cluster, the block for the others
         int nbig = 0;
         for (int c0 = 0; c0 < ncs; c0 += BT) {
           const int c = c0 + tid;
-          int64_t P = 0;
+          bool big = false;
           if (c < ncs) {
             const int o0 = S.off2[c], mn = S.off2[c + 1] - o0;
-            int lay[5], nmin, nmax;
-            bool allk;
-            P = thread_check(p, t, S, S.cl2 + o0, mn, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, allk);
-            S.clmc[c] = allk ? -1 : 0;  // -1: every member kept with its initial distance
-            S.tmp[c] = (int32_t)min<int64_t>(P, INT32_MAX);
-            if (!allk && P <= BIG_COMBOS) combos(t, S, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, 0, 1, false);
+            S.clmc[c] = 1;  // 1: checked by the block
+            if (mn <= SMALL_CLUSTER) {
+              int lay[5], nmin, nmax;
+              bool allk;
+              const int64_t P = thread_check(p, t, S, S.cl2 + o0, mn, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin,
+                                             nmax, allk);
+              if (allk) S.clmc[c] = -1;  // -1: every member kept with its initial distance
+              else if (P <= BIG_COMBOS) {
+                S.clmc[c] = 0;  // 0: checked by this thread
+                combos(t, S, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, 0, 1, false);
+              }
+            }
+            big = S.clmc[c] == 1;
           }
           int tot;
-          const bool big = P > BIG_COMBOS;
           const int r = bscan(L, big, tot);
           if (big) S.sk[nbig + r] = c;
           nbig += tot;
         }
         __syncthreads();
-        for (int bi = 0; bi < nbig; ++bi) {  // the block on each cluster with many combinations
+        for (int bi = 0; bi < nbig; ++bi) {
           const int c = S.sk[bi], o0 = S.off2[c], mn = S.off2[c + 1] - o0;
-          if (!block_tuple_check(L, t, S, o0, mn)) {
-            int lay[5], nmin, nmax;
-            cluster_layout(t, S, S.cl2 + o0, mn, lay, nmin, nmax);
-            combos(t, S, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, tid, BT, true);
-          }
+          const int rc = block_check(L, p, t, S, S.cl2 + o0, mn, S.byl + o0, S.wq + o0, S.kept + o0, S.label + o0,
+                                     S.tbin + o0, S.cdt + o0, S.memf + o0);
+          if (rc == 2 && tid == 0) S.clmc[c] = -1;
           __syncthreads();
         }
         // kept hits per cluster in pulse order, appended in cluster order with their widened distance
@@ -1527,6 +1649,8 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
                 S.coinc[w0 + i] = S.cl2[o0 + i];
                 S.cmaxd[w0 + i] = S.maxd[S.cl2[o0 + i]];
               }
+            } else if (S.clmc[c] > 0) {
+              S.tmp[c] = w0;
             } else {
               int w = w0;
               for (int q = 0; q < mn; ++q)
@@ -1547,6 +1671,26 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
           nco += tot;
         }
         __syncthreads();
+        for (int bi = 0; bi < nbig; ++bi) {  // kept hits of the block-checked clusters: rank by pulse
+          const int c = S.sk[bi];
+          if (S.clmc[c] != 1) continue;
+          const int o0 = S.off2[c], mn = S.off2[c + 1] - o0, w0 = S.tmp[c];
+          const bool lm = mn <= BH;
+          if (lm) {
+            for (int q = tid; q < mn; q += BT) L.u.hist[q] = S.kept[o0 + q] ? S.byl[o0 + q] : INT_MAX;
+            __syncthreads();
+          }
+          for (int q = tid; q < mn; q += BT) {
+            if (!S.kept[o0 + q]) continue;
+            const int h = S.byl[o0 + q];
+            int r = 0;
+            for (int q2 = 0; q2 < mn; ++q2)
+              r += (lm ? L.u.hist[q2] : S.kept[o0 + q2] ? S.byl[o0 + q2] : INT_MAX) < h;
+            S.coinc[w0 + r] = h;
+            S.cmaxd[w0 + r] = S.wq[o0 + q];
+          }
+          __syncthreads();
+        }
       }
       const double dt_opp = 2.0 * half / p.fiber_signal_speed;
       for (int i = tid; i < nco; i += BT) S.wm[S.coinc[i]] = S.cmaxd[i];
@@ -1556,18 +1700,18 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
       // attachment of opposite-side pulses (reco/coinc_cpu.py:11-51), candidates looked up by bar
       int nchosen = 0;
       if (both && p.add_opposite_side_pulses && nitems > 0 && nfilt > 0) {
-        for (int b = tid; b < t.nbars + 1; b += BT) L.hist[b] = 0;
+        for (int b = tid; b < t.nbars + 1; b += BT) L.u.hist[b] = 0;
         for (int k = tid; k < n; k += BT) S.memf[k] = 0;
         __syncthreads();
         for (int c = tid; c < ncl2; c += BT)
           for (int m = S.off2[c]; m < S.off2[c + 1]; ++m) {
             S.clid[m] = c;
             S.memf[S.cl2[m]] = 1;
-            atomicAdd(&L.hist[S.bar[S.cl2[m]]], 1);
+            atomicAdd(&L.u.hist[S.bar[S.cl2[m]]], 1);
           }
         __syncthreads();
         bscan_hist(L, t.nbars + 1);
-        for (int m = tid; m < nitems; m += BT) S.bsort[atomicAdd(&L.hist[S.bar[S.cl2[m]]], 1)] = m;
+        for (int m = tid; m < nitems; m += BT) S.bsort[atomicAdd(&L.u.hist[S.bar[S.cl2[m]]], 1)] = m;
         __syncthreads();  // hist[bar] = end of the bar's items
         const double mdt = tmaxdt + dt_opp;
         for (int ci = tid; ci < nfilt; ci += BT) {
@@ -1576,7 +1720,7 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
           double best_dt = 0.0;
           if (S.seq0[h] && !S.memf[h]) {
             const int hb = S.bar[h], hs = S.side[h];
-            const int k0 = hb ? L.hist[hb - 1] : 0, k1 = L.hist[hb];
+            const int k0 = hb ? L.u.hist[hb - 1] : 0, k1 = L.u.hist[hb];
             for (int k = k0; k < k1; ++k) {
               const int m = S.bsort[k], an = S.cl2[m], c = S.clid[m];
               if (!S.seq0[an] || hs != 1 - S.side[an]) continue;
@@ -1788,13 +1932,13 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     if (c->t.nbars + 1 > BH || 8 * c->t.ncit + 1 > BH)
       return set_error(c, CRV_E_ARG, "clusters: geometry too large for the block finder's bar / column buckets");
     BS b{};
-    double **bd[] = {&b.t, &b.pe, &b.x, &b.y, &b.maxd, &b.wm, &b.cmaxd, &b.wq, &b.cdt};
-    for (int i = 0; i < 9; ++i) *bd[i] = (double *)ws(c, 130 + i, ns * 8);
+    double **bd[] = {&b.t, &b.pe, &b.x, &b.y, &b.maxd, &b.wm, &b.cmaxd, &b.wq, &b.cdt, &b.px, &b.pt, &b.pmd};
+    for (int i = 0; i < 12; ++i) *bd[i] = (double *)ws(c, i < 9 ? 130 + i : 171 + i, ns * 8);
     int32_t **bi[] = {&b.bar, &b.sec, &b.cit, &b.hl, &b.filt, &b.items, &b.off, &b.coinc, &b.byl, &b.cl2, &b.off2,
                       &b.bsort, &b.tsort, &b.label, &b.pass, &b.chk, &b.sk, &b.sc, &b.clmc, &b.tmp, &b.clid, &b.tbin};
     for (int i = 0; i < 22; ++i) *bi[i] = (int32_t *)ws(c, 140 + i, ns * 4);
-    uint8_t **bu[] = {&b.layer, &b.side, &b.sipm, &b.seq0, &b.kept, &b.fpass, &b.memf};
-    for (int i = 0; i < 7; ++i) *bu[i] = (uint8_t *)ws(c, 170 + i, ns + 8);
+    uint8_t **bu[] = {&b.layer, &b.side, &b.sipm, &b.seq0, &b.kept, &b.fpass, &b.memf, &b.pside};
+    for (int i = 0; i < 8; ++i) *bu[i] = (uint8_t *)ws(c, i < 7 ? 170 + i : 183, ns + 8);
     for (auto q : bd) if (!*q) return ws_fail(c);
     for (auto q : bi) if (!*q) return ws_fail(c);
     for (auto q : bu) if (!*q) return ws_fail(c);
