@@ -68,6 +68,7 @@ struct Ctx {
   hipEvent_t kev[8];             // start / end of the main kernel of each stage (sipm, digi, pulse, coinc)
   bool kev_on = false;
   int64_t b_rec_cap = 0, b_run_cap = 0;  // pool sizes of the batched chain (grown on demand)
+  int b_log_factor = 1;  // fired-pixel log capacity factor of the batched chain, doubled on overflow and kept
   int ws_fail_slot = -1;                  // the workspace request that failed last
   size_t ws_fail_bytes = 0;
 };

@@ -952,7 +952,7 @@ __global__ __launch_bounds__(64) void kb_fit(crv_params p, Tables t, const uint6
 }
 
 // fired-pixel log capacity per lane of each group of 64 SiPMs in queue order, from the largest photon count of the
-// group (factor / 2 of it + 16; factor doubles on every re-run); the group takes 64 times that
+// group (3 factor / 4 of it + 16; factor doubles on every re-run); the group takes 64 times that
 __global__ void kb_gcaps(const int64_t *__restrict__ nph, const int32_t *__restrict__ order, int64_t L4,
                          int64_t *__restrict__ gcap, int64_t *__restrict__ gsize, int factor) {
   const int64_t grp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -960,7 +960,7 @@ __global__ void kb_gcaps(const int64_t *__restrict__ nph, const int32_t *__restr
   int64_t m = 0;
   for (int64_t gq = grp * 64; gq < L4 && gq < grp * 64 + 64; ++gq) {
     const int64_t v = nph[order ? order[gq] : gq];
-    m = max(m, v * factor / 2 + 16);  // fired pixels: about 45 % of the photons
+    m = max(m, v * 3 * factor / 4 + 16);  // fired pixels: about 42 % of the photons, binomial spread
   }
   gcap[grp] = m;
   gsize[grp] = m * 64;
@@ -1093,7 +1093,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   int hf[F_NFLAGS];
   // phase 1: photons -> fired-pixel logs (a log that overflows is grown and the phase run again)
   int64_t nlog = 0;
-  for (int attempt = 0, factor = 1;; ++attempt, factor *= 2) {
+  for (int attempt = 0, factor = c->b_log_factor;; ++attempt, factor *= 2) {
     if (L4) kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, sorted ? order : nullptr, L4, gcap, gsize, factor);
     if ((rc = scan_i64(c, gsize, goff, ngrp, 510, s))) return rc;
     if ((rc = read_i64(c, goff + ngrp, &nlog, s))) return rc;
@@ -1117,7 +1117,10 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
       return set_error(c, CRV_E_OVERFLOW,
                        "batched chain: cross-talk cascade wider than 16 pixels or deeper than 31 generations");
     if (hf[F_COUNTER]) return set_error(c, CRV_E_OVERFLOW, "batched chain: more than 65535 photoelectrons from one step");
-    if (!hf[F_OVERFLOW]) break;
+    if (!hf[F_OVERFLOW]) {
+      c->b_log_factor = factor;
+      break;
+    }
     if (attempt >= 8) return set_error(c, CRV_E_OVERFLOW, "batched chain: fired-pixel log still too short");
   }
   // photons drawn and pixels fired (diagnostics)
