@@ -213,12 +213,20 @@ def kernels_fp32(avg, cnt, fires):
     }
 
 
-def roofline(kernels):
+# HBM bytes per event actually moved by a kernel on the cosmic workload: rocprofv3 FETCH_SIZE + WRITE_SIZE passes
+# over 1M events, each scaled by the factor that makes a plain 1 GiB device copy read and write 1 GiB
+# (profiles/chain_fp32_cosmic1m_traffic.txt); used for roofline.traffic of the same workload only
+PMC_BYTES_PER_EVENT = {"cosmic": {"kb_sipm": 200019.2, "kb_wave": 54813.7, "kb_fit": 37109.9}}
+
+
+def roofline(kernels, workload=None, events=0):
     dom = max(kernels, key=lambda k: kernels[k][0])
     kms, kbytes = kernels[dom]
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    per_event = PMC_BYTES_PER_EVENT.get(workload, {}).get(dom)
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": kbytes, "ms_per_launch": kms}
+            "frac": achieved / HBM_PEAK_GBS, "traffic": per_event * events if per_event else None,
+            "bytes_per_launch": kbytes, "ms_per_launch": kms}
 
 
 def main():
@@ -296,10 +304,16 @@ def main():
         ds64 = DeviceSteps(ev64, ds.counter[:cut].cpu().numpy(), ds.edep[:cut].cpu().numpy(),
                            ds.z[:cut].cpu().numpy(), ds.t[:cut].cpu().numpy())
         v64, ms64, avg64, cnt64, _ = measure("fp64", ds64, ev64, n64, args.seed + 1, args.steps, args.warmup, batch)
+        # the FP32 chain on exactly these events (same seed, so the same Philox streams): its count errors
+        _, _, _, c32s, _ = measure("fp32", ds64, ev64, n64, args.seed + 1, 1, 0, batch)
         del ds64
+        fp32_err = {k: (c32s[f"n_{k}"] - cnt64[f"n_{k}"]) / max(cnt64[f"n_{k}"], 1)
+                    for k in ("photons", "charges", "pulses", "clusters", "members")}
         k64 = kernels_fp64(avg64, cnt64)
         fp64 = {"value": v64, "unit": "events/s", "events": n64, "ms_per_step": ms64, "dtype": "f64",
                 "speedup_fp32_over_fp64": v32 / v64, "roofline": roofline(k64),
+                "fp32_rel_count_error": {"events": n64, "note": "photons, fired pixels (integer PE), pulses, "
+                                         "clusters, cluster members: (FP32 - FP64) / FP64 on the same events", **fp32_err},
                 "kernel_ms": {k: v[0] for k, v in k64.items()},
                 "stage_ms": {k: avg64[k] for k in ("charges_ms", "digis_ms", "waveforms_ms", "pulses_ms",
                                                    "clusters_ms", "total_ms")},
@@ -317,7 +331,7 @@ def main():
                    "precision": "fp32 (fp64 times and 3x3 solve)",
                    "chain": "steps->photons->SiPM->waveforms->pulses->clusters", "grid": args.grid,
                    "tile": args.tile or 64, "sort_tasks": args.sort},
-        "roofline": roofline(k32),
+        "roofline": roofline(k32, args.workload, args.chunk or args.events),
         "chain_roofline": {"bytes_per_event": b_event, "achieved": b_event * v32 / 1e9, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": b_event * v32 / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,

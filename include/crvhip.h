@@ -209,8 +209,9 @@ typedef struct crv_batch_opts {
   int32_t overlong, capped, pulse_slots_used, _pad;
 } crv_batch_opts;
 
-/* The whole chain as the FP32 batched pipeline: steps -> pulses in one persistent fused kernel (photons, SiPM,
- * waveform, pulse fit per SiPM lane; no charge or digi tables), then the coincidence finder. FP32 arithmetic; FP64
+/* The whole chain as the FP32 batched pipeline: steps -> pulses in three persistent work-queue kernels (photons and
+ * SiPM pixels into a per-SiPM fired-pixel log; waveform records and pulse runs; pulse fits), no charge or digi
+ * tables, then the coincidence finder. FP32 arithmetic; FP64
  * for absolute times and the 3 x 3 normal-equation solve of the fit. Same outputs and conventions as crv_chain_run
  * (pulse `wf` is the SiPM index of the call), equal to the FP64 chain statistically (same Philox streams, FP32
  * rounding). The reference runs this chain as chain.charges -> digis -> reco_waveforms -> clusters
