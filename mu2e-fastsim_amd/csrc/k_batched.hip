@@ -306,12 +306,23 @@ __device__ __forceinline__ bool bget(const uint32_t *bm, int lane, int pix) {
 }
 __device__ __forceinline__ void bset(uint32_t *bm, int lane, int pix) { bm[(pix >> 5) * 64 + lane] |= 1u << (pix & 31); }
 
-// overvoltage of a pixel at time offset toff: from its last firing (the bitmap says whether it fired in this SiPM)
+// overvoltage of a pixel at time offset toff: recovered from its last firing (the bitmap says whether it fired in this
+// SiPM; the per-pixel last-fire times are read only then, so they need no clearing between SiPMs)
 __device__ __forceinline__ float pixel_v(const uint32_t *bm, int lane, int pix, const float *lastt, int64_t xs,
                                          float toff, float Vov, float inv_tau) {
   if (!bget(bm, lane, pix)) return Vov;
   return Vov * (1.f - __expf(-(toff - lastt[pix * xs]) * inv_tau));
 }
+
+// append a firing to the SiPM's log and to the per-pixel last-fire times
+__device__ __forceinline__ void log_fire(const Out &o, int64_t lb, int64_t &nlog, float *lastt, int64_t xs, int pix,
+                                         float toff, float v) {
+  o.log_t[lb + nlog * 64] = toff;
+  o.log_v[lb + nlog * 64] = v;
+  lastt[pix * xs] = toff;
+  ++nlog;
+}
+
 // kb_sipm, one lane per SiPM: photons of the counter's steps in time order -> pixel, recovery, firing, cross-talk
 // cascade -> the SiPM's fired-pixel log (time, overvoltage, pixel), in firing order.
 __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv, BStep st,
@@ -467,18 +478,15 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           bad = true;
           break;
         }
-        o.log_t[lb + nlog * 64] = toff;
-        o.log_v[lb + nlog * 64] = v;
-        lastt[pix * xs] = toff;
-        ++nlog;
+        log_fire(o, lb, nlog, lastt, xs, pix, toff, v);
         bset(bm, lane, pix);
-        // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order
+        // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order (generation 0 is
+        // the photon's pixel alone, kept in a register)
         int ncur = 1, gnum = 0;
-        xcur[0] = (uint16_t)pix;
         while (ncur > 0 && !bad) {
           int ncand = 0;
           for (int ai = 0; ai < ncur; ++ai) {
-            const int f = xcur[ai * xs];
+            const int f = gnum == 0 ? pix : (int)xcur[ai * xs];
             const int fx = f / CRV_NPIX, fy = f % CRV_NPIX;
             const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)f));
             const uint32_t uw[4] = {ux.x, ux.y, ux.z, ux.w};
@@ -530,10 +538,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
               bad = true;
               break;
             }
-            o.log_t[lb + nlog * 64] = toff;
-            o.log_v[lb + nlog * 64] = vc;
-            lastt[c * xs] = toff;
-            ++nlog;
+            log_fire(o, lb, nlog, lastt, xs, c, toff, vc);
             bset(bm, lane, c);
             xcur[ncur * xs] = (uint16_t)c;
             ++ncur;
