@@ -128,7 +128,10 @@ def chunks_of(evkey, n_ev, chunk):
     return [(int(a), int(b - a)) for a, b in zip(bounds[:-1], bounds[1:]) if b > a]
 
 
-def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0):
+AUTOTUNE_EVENTS = 200_000  # events of the sample the work-queue autotuner runs its candidates on
+
+
+def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, autotune=False):
     """Time `steps` chain passes over the batch (chunk by chunk, steps resident in HBM); returns (events/s, ms/step,
     mean timing fields summed over chunks, last counts summed over chunks, extra)."""
     import torch
@@ -142,8 +145,22 @@ def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0):
     structs = [L.Steps(*[getattr(base, f) + a * sz for f, sz in zip(("event", "counter", "edep", "z", "t"), sizes)],
                        n) for a, n in parts]
     tm = L.Timing()
-    opts = L.BatchOpts(**batch)
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    tuned = None
+    if autotune and precision == "fp32":  # tile / sort order of the work queues, measured on the batch's first events
+        from crvhip import autotune as AT
+        a, n = chunks_of(evkey, n_ev, min(n_ev, AUTOTUNE_EVENTS))[0]
+        sample = L.Steps(*[getattr(base, f) + a * sz for f, sz in zip(("event", "counter", "edep", "z", "t"), sizes)],
+                         n)
+        chosen, rep = AT.tune(ctx, sample, min(n_ev, AUTOTUNE_EVENTS), torch.cuda.get_device_name(0), stream=stream,
+                              cache=False)
+        batch = {**batch, **chosen}
+        tuned = {"sample_events": min(n_ev, AUTOTUNE_EVENTS), "chosen": chosen,
+                 "median_ms": {f"tile {c['opts']['tile']} sort {c['opts']['sort_tasks']}":
+                               round(sorted(c["ms"])[len(c["ms"]) // 2], 2) for c in rep["candidates"]}}
+        if PROGRESS:
+            print(f"[bench] autotune: {chosen}", file=sys.stderr, flush=True)
+    opts = L.BatchOpts(**batch)
 
     def call(st, out):
         if precision == "fp32":
@@ -185,6 +202,8 @@ def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0):
     extra = {"n_waveforms_batched": opts.n_waveforms, "pulse_slots_used": opts.pulse_slots_used} \
         if precision == "fp32" else {}
     extra["chunks"] = len(structs)
+    if tuned:
+        extra["autotune"] = tuned
     ctx.close()
     del out, empty
     torch.cuda.empty_cache()
@@ -241,6 +260,8 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--sort", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep --tile / --sort as given (with --tile 0 the work-queue tile and order are autotuned)")
     ap.add_argument("--chunk", type=int, default=0, help="events per chain call (0: the whole batch in one call)")
     ap.add_argument("--gen", default="auto", choices=("auto", "host", "device"),
                     help="synthetic input generator: numpy on the host or torch on the device (auto: device for "
@@ -293,8 +314,12 @@ def main():
         ds = DeviceSteps(S["evkey"], S["counter"], S["edep"], S["z"], S["t"])
         evkey = S["evkey"]
     batch = {"grid": args.grid, "tile": args.tile, "sort_tasks": args.sort}
+    autotune = args.tile == 0 and not args.no_autotune
     v32, ms32, avg32, cnt32, ex32 = measure("fp32", ds, evkey, args.events, args.seed + 1, args.steps, args.warmup,
-                                            batch, args.chunk)
+                                            batch, args.chunk, autotune)
+    tuned = ex32.pop("autotune", None)
+    if tuned:
+        batch = {**batch, **tuned["chosen"]}
     k32 = kernels_fp32(avg32, cnt32, cnt32["n_charges"])
     fp64 = None
     if args.fp64_events and args.workload == "cosmic":
@@ -330,7 +355,8 @@ def main():
                    "generator": "device (torch)" if gen_device else "host (numpy)",
                    "precision": "fp32 (fp64 times and 3x3 solve)",
                    "chain": "steps->photons->SiPM->waveforms->pulses->clusters", "grid": args.grid,
-                   "tile": args.tile or 64, "sort_tasks": args.sort},
+                   "tile": batch["tile"] or 64, "sort_tasks": batch["sort_tasks"],
+                   "autotune": tuned},
         "roofline": roofline(k32, args.workload, args.chunk or args.events),
         "chain_roofline": {"bytes_per_event": b_event, "achieved": b_event * v32 / 1e9, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": b_event * v32 / 1e9 / HBM_PEAK_GBS},

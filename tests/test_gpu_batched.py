@@ -152,3 +152,27 @@ def test_uneven_showers():
     cond, G = DT.nominal_conditions(), DT.sector_geom()
     S = SY.uneven_events(300, seed=8)
     _compare(CH.run_chain(S, cond, c32, key=3, geom=G), CH.run_chain(S, cond, c64, key=3, geom=G), rtol=5e-3)
+
+
+def test_autotuner_picks_a_candidate_and_results_do_not_change(cosmic, tmp_path):
+    """The work-queue autotuner times every candidate on device-resident steps and returns the fastest; the chain
+    run with its choice gives the same tables as the default launch."""
+    from crvhip import _lib as L, autotune as AT
+    S, cond, G, c64, c32, r64, r32 = cosmic
+    ev, counter, edep, z, t, _ = CH.step_table(S)
+    ds = CH.DeviceSteps(ev, counter, edep, z, t)
+    ctx = CH.context(c32, cond, G)
+    ctx.set_params(c32, 7, ds.evkey_hi, 32)
+    cache = str(tmp_path / "tune.json")
+    chosen, rep = AT.tune(ctx, ds.struct(), 3000, "test-device", tiles=(64, 128, 256), sorts=(1, 0), repeats=1,
+                          cache_file=cache)
+    assert not rep["cached"] and len(rep["candidates"]) == 6
+    assert all(len(c["ms"]) == 1 and c["ms"][0] > 0 for c in rep["candidates"])
+    assert chosen in [c["opts"] for c in rep["candidates"]]
+    again, rep2 = AT.tune(ctx, ds.struct(), 3000, "test-device", cache_file=cache)
+    assert rep2["cached"] and again == chosen
+    r = CH.run_chain(S, cond, c32, key=7, geom=G, batch=chosen)
+    for k, v in r32["pulses"].items():
+        np.testing.assert_array_equal(r["pulses"][k], v, err_msg=k)
+    for k, v in r32["clusters"].items():
+        np.testing.assert_array_equal(r["clusters"][k], v, err_msg=k)
