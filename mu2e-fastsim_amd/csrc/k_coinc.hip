@@ -44,42 +44,25 @@ struct ClSlots {  // per-cluster outputs of the finder kernels (event-local slot
 template <class Acc>
 __device__ bool cluster_emit(const crv_params &p, const Tables &t, const Acc &A, const int32_t *mem, int mn, int type,
                              double P0, double P1, int64_t evkey, int64_t slot, const ClSlots &o) {
-  double tot = 0.0, mincp = INFINITY;
-  for (int i = 0; i < mn; ++i) {
-    int h = mem[i];
-    tot = tot + A.pe(h);
-    mincp = fmin(mincp, cut_of(t, t.sector[A.bar(h)]).min_cluster_pes);
-  }
-  if (tot < mincp) return false;
-  double pos[3];
-  for (int ax = 0; ax < 3; ++ax) {
-    double acc = 0.0;
-    for (int i = 0; i < mn; ++i) {
-      int h = mem[i];
-      acc = acc + t.bpos[A.bar(h) * 3 + ax] * A.pe(h);
-    }
-    pos[ax] = acc / tot;
-  }
-  double sxy = 0.0, sx = 0.0, sy = 0.0, syy = 0.0;
-  double st[2] = {0.0, 0.0}, spe[2] = {0.0, 0.0}, tmn[2] = {0.0, 0.0}, tmx[2] = {0.0, 0.0};
+  // one pass over the members; every sum still runs in member order
+  double tot = 0.0, mincp = INFINITY, acc[3] = {0.0, 0.0, 0.0};
+  double sxy = 0.0, sx = 0.0, sy = 0.0, syy = 0.0, cmin = INFINITY, cmax = -INFINITY;
+  double ps[2] = {0.0, 0.0}, pt[2] = {0.0, 0.0}, tmn[2] = {0.0, 0.0}, tmx[2] = {0.0, 0.0};
   int nhs[2] = {0, 0};
-  double cmin = INFINITY, cmax = -INFINITY;
   for (int i = 0; i < mn; ++i) {
-    int h = mem[i];
-    double pe = A.pe(h), x = t.bx[A.bar(h)], y = t.by[A.bar(h)], tt = A.tm(h);
+    const int h = mem[i], bar = A.bar(h);
+    const double pe = A.pe(h), tt = A.tm(h), x = A.x(h), y = A.y(h);
+    tot = tot + pe;
+    mincp = fmin(mincp, cut_of(t, A.sec(h)).min_cluster_pes);
+    for (int ax = 0; ax < 3; ++ax) acc[ax] = acc[ax] + t.bpos[bar * 3 + ax] * pe;
     sxy = sxy + pe * x * y;
     sx = sx + pe * x;
     sy = sy + pe * y;
     syy = syy + pe * y * y;
     cmin = fmin(cmin, tt);
     cmax = fmax(cmax, tt);
-  }
-  for (int sd = 0; sd < 2; ++sd) {
-    double ps = 0.0, pt = 0.0;
-    for (int i = 0; i < mn; ++i) {
-      int h = mem[i];
-      if (A.sipm(h) % 2 != sd || !(A.pe(h) > 0)) continue;
-      double tt = A.tm(h);
+    const int sd = A.sipm(h) % 2;
+    if (pe > 0) {
       if (nhs[sd] == 0) {
         tmn[sd] = tt;
         tmx[sd] = tt;
@@ -88,14 +71,19 @@ __device__ bool cluster_emit(const crv_params &p, const Tables &t, const Acc &A,
         tmx[sd] = fmax(tmx[sd], tt);
       }
       ++nhs[sd];
-      ps = ps + A.pe(h);
-      pt = pt + tt * A.pe(h);
+      ps[sd] = ps[sd] + pe;
+      pt[sd] = pt[sd] + tt * pe;
     }
-    spe[sd] = ps;
-    st[sd] = ps > 0 ? pt / ps : 0.0;
+  }
+  if (tot < mincp) return false;
+  double pos[3], st[2], spe[2];
+  for (int ax = 0; ax < 3; ++ax) pos[ax] = acc[ax] / tot;
+  for (int sd = 0; sd < 2; ++sd) {
+    spe[sd] = ps[sd];
+    st[sd] = ps[sd] > 0 ? pt[sd] / ps[sd] : 0.0;
   }
   double slope = (tot * sxy - sx * sy) / (tot * syy - sy * sy);
-  int ld = t.ldir[t.sector[A.bar(mem[0])]];
+  int ld = t.ldir[A.sec(mem[0])];
   bool calc = false;
   double avg_t, org;
   const double v = p.fiber_signal_speed;
@@ -156,6 +144,10 @@ struct WS {
   uint8_t sl[WMAX], fpass[WMAX];  // hits grouped by (side, layer) in hit order; filterHits result by hit position
   int16_t gof[9];
   int32_t cl_mc[WMAX];  // members of a second-level cluster (0: dropped)
+  int32_t mlist[WMAX];  // members of the kept clusters, at their event-local member offsets
+  int32_t spre[12];     // layer subsets of a coincidence check: combination prefix counts,
+  int16_t slay[11];     // their layers (4 bits each)
+  int8_t snl[11];       // and sizes
 };
 
 template <int WMAX>
@@ -165,6 +157,9 @@ struct WaveAcc {
   __device__ double tm(int h) const { return S.t[h]; }
   __device__ int bar(int h) const { return S.bar[h]; }
   __device__ int sipm(int h) const { return S.sipm[h]; }
+  __device__ int sec(int h) const { return S.sec[h]; }
+  __device__ double x(int h) const { return S.x[h]; }
+  __device__ double y(int h) const { return S.y[h]; }
 };
 
 __device__ __forceinline__ uint64_t lanes_below() { return (1ull << threadIdx.x) - 1ull; }
@@ -176,16 +171,6 @@ __device__ __forceinline__ int wave_min(int v) {
 __device__ __forceinline__ int wave_max(int v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
   return v;
-}
-
-__device__ __forceinline__ void lds_fmax(double *addr, double v) {
-  unsigned long long *a = (unsigned long long *)addr;
-  unsigned long long old = *a, assumed;
-  do {
-    assumed = old;
-    if (!(v > __longlong_as_double((long long)assumed))) return;
-    old = atomicCAS(a, assumed, (unsigned long long)__double_as_longlong(v));
-  } while (old != assumed);
 }
 
 // greedy findClusters of the pulses list[0..m) (reco/coinc.py:128-152): each pass visits the remaining hits in list
@@ -287,21 +272,126 @@ __device__ int wave_clusters(const Tables &t, WS<WMAX> &S, const uint8_t *list, 
   return ncl;
 }
 
-// checkCoincidence of the cluster mem[0..n) (pulses): kept hits appended to coinc in pulse order with their widened
-// max distance; returns the new coinc length
+// the layer combinations of a by-layer list, every needed layer subset in one index space over the lanes (a cluster
+// of one track has a handful per subset); UNI: every hit has the cuts `cu`
+template <int WMAX, bool UNI>
+__device__ void wave_combos(const Tables &t, WS<WMAX> &S, const int lay_off[5], int nmin, int nmax, const Cut &cu) {
+  const int lane = threadIdx.x;
+  int nsub = 0, total = 0;
+  for (int nl = 2; nl <= 4; ++nl) {
+    const bool need = nl == 2 ? nmin == 2 : nl == 3 ? (nmin <= 3 && 3 <= nmax) : nmax == 4;
+    if (!need) continue;
+    int L[4];
+    for (int i = 0; i < nl; ++i) L[i] = i;
+    for (;;) {
+      int P = 1;
+      for (int i = 0; i < nl; ++i) P *= lay_off[L[i] + 1] - lay_off[L[i]];
+      if (P > 0) {
+        if (lane == 0) {
+          S.spre[nsub] = total;
+          S.snl[nsub] = (int8_t)nl;
+          S.slay[nsub] = (int16_t)(L[0] | L[1] << 4 | (nl > 2 ? L[2] : 0) << 8 | (nl > 3 ? L[3] : 0) << 12);
+        }
+        ++nsub;
+        total += P;
+      }
+      int k = nl - 1;
+      while (k >= 0 && L[k] == 4 - nl + k) --k;
+      if (k < 0) break;
+      ++L[k];
+      for (int i = k + 1; i < nl; ++i) L[i] = L[i - 1] + 1;
+    }
+  }
+  if (lane == 0) S.spre[nsub] = total;
+  __syncthreads();
+  for (int cidx = lane; cidx < total; cidx += 64) {
+    int sb = 0;
+    while (cidx >= S.spre[sb + 1]) ++sb;
+    const int nl = S.snl[sb], lay = S.slay[sb];
+    int idx[4];
+    uint32_t rest = (uint32_t)(cidx - S.spre[sb]);
+    for (int i = nl - 1; i >= 0; --i) {
+      const int l = (lay >> (4 * i)) & 15;
+      const uint32_t sz = (uint32_t)(lay_off[l + 1] - lay_off[l]);
+      idx[i] = lay_off[l] + (int)(rest % sz);
+      rest /= sz;
+    }
+    bool skip = true;
+    double mdt = -INFINITY, lo = INFINITY, hi = -INFINITY, msd = -INFINITY;
+    double tmin = INFINITY, tmax = -INFINITY;
+    for (int i = 0; i < nl; ++i) {
+      const int h = S.byl[idx[i]];
+      const Cut c = UNI ? cu : cut_of(t, S.sec[h]);
+      if (!((int)c.coincidence_layers > nl)) skip = false;
+      mdt = fmax(mdt, c.max_dt);
+      lo = fmin(lo, c.min_slope);
+      hi = fmax(hi, c.max_slope);
+      msd = fmax(msd, c.max_slope_diff);
+      tmin = fmin(tmin, S.t[h]);
+      tmax = fmax(tmax, S.t[h]);
+    }
+    bool pass = !skip && !(tmax - tmin > mdt);
+    double sl[3];
+    for (int i = 0; pass && i + 1 < nl; ++i) {
+      const int a = S.byl[idx[i]], b = S.byl[idx[i + 1]];
+      const double v = (S.x[b] - S.x[a]) / (S.y[b] - S.y[a]);
+      if (v < lo || v > hi) pass = false;
+      sl[i] = v;
+    }
+    if (pass && nl > 2 && fabs(sl[0] - sl[1]) > msd) pass = false;
+    if (pass && nl > 3 && (fabs(sl[0] - sl[2]) > msd || fabs(sl[1] - sl[2]) > msd)) pass = false;
+    if (pass) {
+      int ord[4];
+      for (int i = 0; i < nl; ++i) ord[i] = i;
+      for (int i = 1; i < nl; ++i) {  // stable insertion sort by x
+        const int v = ord[i];
+        const double xv = S.x[S.byl[idx[v]]];
+        int j = i - 1;
+        while (j >= 0 && S.x[S.byl[idx[ord[j]]]] > xv) {
+          ord[j + 1] = ord[j];
+          --j;
+        }
+        ord[j + 1] = v;
+      }
+      for (int i = 0; i < nl; ++i) {
+        const int q = idx[ord[i]];
+        const double x = S.x[S.byl[q]];
+        double w = -INFINITY;
+        if (i > 0) w = fmax(w, x - S.x[S.byl[idx[ord[i - 1]]]]);
+        if (i + 1 < nl) w = fmax(w, S.x[S.byl[idx[ord[i + 1]]]] - x);
+        atomicMax(&S.wq[q], w);  // LDS floating max (the distances are finite)
+        S.kept[q] = 1;
+      }
+    }
+  }
+}
+
+// checkCoincidence of one cluster (mem[0..n), reference reco/coinc.py:184-207): its kept hits appended to the
+// event's coincidence list in pulse order with their widened distance; returns the new list length. A cluster whose
+// hits share one sector (the usual case) reads that sector's cuts once.
 template <int WMAX>
 __device__ int wave_check(const crv_params &p, const Tables &t, WS<WMAX> &S, const uint8_t *mem, int n, int nco) {
   constexpr int WNW = WMAX / 64;
   const int lane = threadIdx.x;
   if (n == 0) return nco;
+  const int sec0 = S.sec[mem[0]];
+  bool same = true;
+  for (int i = lane; i < n; i += 64) same = same && S.sec[mem[i]] == sec0;
+  const bool uni = __ballot(!same) == 0ull;
+  Cut cu{};
   int nmin = 1 << 30, nmax = -1;
-  for (int i = lane; i < n; i += 64) {
-    const int cl = (int)cut_of(t, S.sec[mem[i]]).coincidence_layers;
-    nmin = min(nmin, cl);
-    nmax = max(nmax, cl);
+  if (uni) {
+    cu = cut_of(t, sec0);
+    nmin = nmax = (int)cu.coincidence_layers;
+  } else {
+    for (int i = lane; i < n; i += 64) {
+      const int cl = (int)cut_of(t, S.sec[mem[i]]).coincidence_layers;
+      nmin = min(nmin, cl);
+      nmax = max(nmax, cl);
+    }
+    nmin = wave_min(nmin);
+    nmax = wave_max(nmax);
   }
-  nmin = wave_min(nmin);
-  nmax = wave_max(nmax);
   int lay_off[5];
   int pos = 0;
   for (int l = 0; l < 4; ++l) {
@@ -313,7 +403,7 @@ __device__ int wave_check(const crv_params &p, const Tables &t, WS<WMAX> &S, con
       if (pr) {
         const int q = pos + __popcll(bm & lanes_below());
         S.byl[q] = mem[i];
-        S.wq[q] = cut_of(t, S.sec[mem[i]]).final_max_distance;
+        S.wq[q] = uni ? cu.final_max_distance : cut_of(t, S.sec[mem[i]]).final_max_distance;
         S.kept[q] = 0;
       }
       pos += __popcll(bm);
@@ -332,86 +422,8 @@ __device__ int wave_check(const crv_params &p, const Tables &t, WS<WMAX> &S, con
     __syncthreads();
     return nco + n;
   }
-  for (int nl = 2; nl <= 4; ++nl) {
-    const bool need = nl == 2 ? nmin == 2 : nl == 3 ? (nmin <= 3 && 3 <= nmax) : nmax == 4;
-    if (!need) continue;
-    int L[4];
-    for (int i = 0; i < nl; ++i) L[i] = i;
-    for (;;) {
-      int64_t P = 1;
-      for (int i = 0; i < nl; ++i) P *= lay_off[L[i] + 1] - lay_off[L[i]];
-      for (int64_t cidx = lane; cidx < P; cidx += 64) {
-        int idx[4];
-        if (P <= INT32_MAX) {  // 32-bit decode (64-bit division is a long software sequence)
-          uint32_t rest = (uint32_t)cidx;
-          for (int i = nl - 1; i >= 0; --i) {
-            const uint32_t sz = (uint32_t)(lay_off[L[i] + 1] - lay_off[L[i]]);
-            idx[i] = lay_off[L[i]] + (int)(rest % sz);
-            rest /= sz;
-          }
-        } else {
-          int64_t rest = cidx;
-          for (int i = nl - 1; i >= 0; --i) {
-            const int sz = lay_off[L[i] + 1] - lay_off[L[i]];
-            idx[i] = lay_off[L[i]] + (int)(rest % sz);
-            rest /= sz;
-          }
-        }
-        bool skip = true;
-        double mdt = -INFINITY, lo = INFINITY, hi = -INFINITY, msd = -INFINITY;
-        double tmin = INFINITY, tmax = -INFINITY;
-        for (int i = 0; i < nl; ++i) {
-          const int h = S.byl[idx[i]];
-          const Cut c = cut_of(t, S.sec[h]);
-          if (!((int)c.coincidence_layers > nl)) skip = false;
-          mdt = fmax(mdt, c.max_dt);
-          lo = fmin(lo, c.min_slope);
-          hi = fmax(hi, c.max_slope);
-          msd = fmax(msd, c.max_slope_diff);
-          tmin = fmin(tmin, S.t[h]);
-          tmax = fmax(tmax, S.t[h]);
-        }
-        bool pass = !skip && !(tmax - tmin > mdt);
-        double sl[3];
-        for (int i = 0; pass && i + 1 < nl; ++i) {
-          const int a = S.byl[idx[i]], b = S.byl[idx[i + 1]];
-          const double v = (S.x[b] - S.x[a]) / (S.y[b] - S.y[a]);
-          if (v < lo || v > hi) pass = false;
-          sl[i] = v;
-        }
-        if (pass && nl > 2 && fabs(sl[0] - sl[1]) > msd) pass = false;
-        if (pass && nl > 3 && (fabs(sl[0] - sl[2]) > msd || fabs(sl[1] - sl[2]) > msd)) pass = false;
-        if (pass) {
-          int ord[4];
-          for (int i = 0; i < nl; ++i) ord[i] = i;
-          for (int i = 1; i < nl; ++i) {  // stable insertion sort by x
-            const int v = ord[i];
-            const double xv = S.x[S.byl[idx[v]]];
-            int j = i - 1;
-            while (j >= 0 && S.x[S.byl[idx[ord[j]]]] > xv) {
-              ord[j + 1] = ord[j];
-              --j;
-            }
-            ord[j + 1] = v;
-          }
-          for (int i = 0; i < nl; ++i) {
-            const int q = idx[ord[i]];
-            const double x = S.x[S.byl[q]];
-            double w = -INFINITY;
-            if (i > 0) w = fmax(w, x - S.x[S.byl[idx[ord[i - 1]]]]);
-            if (i + 1 < nl) w = fmax(w, S.x[S.byl[idx[ord[i + 1]]]] - x);
-            lds_fmax(&S.wq[q], w);
-            S.kept[q] = 1;
-          }
-        }
-      }
-      int k = nl - 1;
-      while (k >= 0 && L[k] == 4 - nl + k) --k;
-      if (k < 0) break;
-      ++L[k];
-      for (int i = k + 1; i < nl; ++i) L[i] = L[i - 1] + 1;
-    }
-  }
+  if (uni) wave_combos<WMAX, true>(t, S, lay_off, nmin, nmax, cu);
+  else wave_combos<WMAX, false>(t, S, lay_off, nmin, nmax, cu);
   __syncthreads();
   for (int q = lane; q < pos; q += 64)
     if (S.kept[q]) {
@@ -420,22 +432,19 @@ __device__ int wave_check(const crv_params &p, const Tables &t, WS<WMAX> &S, con
       atomicOr((unsigned long long *)&S.kmask[h >> 6], 1ull << (h & 63));
     }
   __syncthreads();
-  // kept hits in pulse order
+  // kept hits in pulse order: a hit's rank is the number of kept hits below it
+  int before = 0;
   for (int w = 0; w < WNW; ++w) {
-    uint64_t r = S.kmask[w];
-    while (r) {
-      const int b = __ffsll((unsigned long long)r) - 1;
-      r &= r - 1;
-      const int h = w * 64 + b;
-      if (lane == 0) {
-        S.coinc[nco] = (uint8_t)h;
-        S.cmaxd[nco] = S.wm[h];
-      }
-      ++nco;
+    const uint64_t m = S.kmask[w];
+    if ((m >> lane) & 1ull) {
+      const int r = nco + before + __popcll(m & lanes_below());
+      S.coinc[r] = (uint8_t)(w * 64 + lane);
+      S.cmaxd[r] = S.wm[w * 64 + lane];
     }
+    before += __popcll(m);
   }
   __syncthreads();
-  return nco;
+  return nco + before;
 }
 
 template <int WMAX>
@@ -662,9 +671,9 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
         }
         const int mbase = nm + incl - mc, cslot = nclus + kincl - (mc > 0);
         if (mc > 0) {
-          int32_t *mm = mem_e + mbase;
+          int32_t *mm = mem_e + mbase, *ml = S.mlist + mbase;  // output and LDS copy (a pulse is in one cluster)
           int w = 0;
-          for (int m = S.off2[c]; m < S.off2[c + 1]; ++m) mm[w++] = S.cl2[m];
+          for (int m = S.off2[c]; m < S.off2[c + 1]; ++m, ++w) mm[w] = ml[w] = S.cl2[m];
           int lastb = -1, lasts = -1;
           for (int q = 0; q < nchosen; ++q) {
             if (S.sk[q] != c) continue;
@@ -672,9 +681,10 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
             if (S.bar[h] == lastb && S.sipm[h] == lasts) continue;
             lastb = S.bar[h];
             lasts = S.sipm[h];
-            mm[w++] = h;
+            mm[w] = ml[w] = h;
+            ++w;
           }
-          cluster_emit(p, t, WaveAcc<WMAX>{S}, mm, mc, type, P0, P1, evkey, a + cslot, slots);
+          cluster_emit(p, t, WaveAcc<WMAX>{S}, ml, mc, type, P0, P1, evkey, a + cslot, slots);
           cl_moff_e[cslot] = mbase;
         }
         nm += __shfl(incl, 63);
@@ -1457,6 +1467,9 @@ struct BlockAcc {
   __device__ double tm(int h) const { return S.t[h]; }
   __device__ int bar(int h) const { return S.bar[h]; }
   __device__ int sipm(int h) const { return S.sipm[h]; }
+  __device__ int sec(int h) const { return S.sec[h]; }
+  __device__ double x(int h) const { return S.x[h]; }
+  __device__ double y(int h) const { return S.y[h]; }
 };
 
 __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int64_t nev, const int64_t *__restrict__ eoff,
