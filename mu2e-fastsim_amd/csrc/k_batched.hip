@@ -31,6 +31,7 @@ constexpr int DWORDS = 6;   // digi-start mask words: window_samples <= 192
 constexpr float PERIODF = 12.5f;
 constexpr double PERIODD = 12.5;
 constexpr int F_PSLOT = 7;  // a reco waveform produced more pulses than its slots
+constexpr int TMPL_LDS = 1024;  // template samples staged in LDS by the waveform kernel
 
 __device__ __forceinline__ float u32f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * 0x1p-24f; }
 
@@ -737,12 +738,15 @@ __global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv
                                               const int32_t *__restrict__ nfired, Out o, Queue qu,
                                               int *__restrict__ flags) {
   __shared__ int tile_base;
+  __shared__ float tmpl_s[TMPL_LDS];  // the single-PE template, staged once per workgroup
   const int lane = threadIdx.x;
   const double gap = p.single_pe_max_time + 4.0 * PERIODD;
   const Draw dr{(uint32_t)p.seed, p.evkey_hi};
+  for (int i = lane; i < t.ntmpl; i += 64) tmpl_s[i] = t.tmpl_f[i];
+  __syncthreads();
   Lane L;
   L.W = p.window_samples;
-  L.tmpl = t.tmpl_f;
+  L.tmpl = tmpl_s;
   int64_t base = 0;
   while (grab(qu, &tile_base, base)) {
     for (int64_t gq = base + lane; gq < base + qu.tile && gq < qu.n; gq += 64) {
@@ -1039,6 +1043,9 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   if (W > 32 * DWORDS || Sz < 1 || W <= Sz)
     return set_error(c, CRV_E_ARG, "batched chain: window_samples must be in (samples_zs, 192]");
   if (!c->t.tmpl_f) return set_error(c, CRV_E_STATE, "template not uploaded");
+  if (c->t.ntmpl > TMPL_LDS)
+    return set_error(c, CRV_E_ARG, "batched chain: single-PE template longer than " + std::to_string(TMPL_LDS) +
+                                       " samples");
   if (c->d.nreach > RINGB) return set_error(c, CRV_E_ARG, "batched chain: single-PE template reaches over 16 samples");
   const int tile = opt->tile > 0 ? opt->tile : 64;
   if (tile % 64) return set_error(c, CRV_E_ARG, "batched chain: tile must be a multiple of 64");
