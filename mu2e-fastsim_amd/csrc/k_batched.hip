@@ -32,6 +32,7 @@ constexpr float PERIODF = 12.5f;
 constexpr double PERIODD = 12.5;
 constexpr int F_PSLOT = 7;  // a reco waveform produced more pulses than its slots
 constexpr int TMPL_LDS = 1024;  // template samples staged in LDS by the waveform kernel
+constexpr int FWIN = 16;        // fit-window samples staged in LDS per lane by the fit kernel
 
 __device__ __forceinline__ float u32f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * 0x1p-24f; }
 
@@ -150,7 +151,8 @@ __device__ __forceinline__ void fparams(const float lo[3], const float hi[3], co
   }
 }
 
-__device__ __forceinline__ void feval(const float *y, int64_t ys, int fs, int fe, const float lo[3],
+// chi2, J^T J and J^T r of the Gumbel model at u over samples fs..fe; sample q is y[(q - q0) * ys]
+__device__ __forceinline__ void feval(const float *y, int64_t ys, int q0, int fs, int fe, const float lo[3],
                                       const float hi[3], const float u[3], FSum &s) {
   float par[3], dpdu[3];
   fparams(lo, hi, u, par, dpdu);
@@ -165,7 +167,7 @@ __device__ __forceinline__ void feval(const float *y, int64_t ys, int fs, int fe
     const float ez = __expf(-z);
     const float g = __expf(-z - ez);
     const float fv = A * g;
-    const float r = y[q * ys] - fv;
+    const float r = y[(q - q0) * ys] - fv;
     const float dfdz = fv * (ez - 1.f);
     const float J0 = g * dpdu[0], J1 = -dfdz * ib * dpdu[1], J2 = -dfdz * z * ib * dpdu[2];
     s.chi2 += r * r;
@@ -796,9 +798,57 @@ __global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv
 // kb_fit, one lane per reco waveform of the run pool: pedestal, peaks, Gumbel fits, pulses -> the run's slots; the
 // run's place in the output (SiPM's first run + run number) gets its pulse count and pool slot.
 // the pulses of pool run r, with the lane's sample scratch y (stride ys)
+// Levenberg-Marquardt then Gauss-Newton polish of the Gumbel fit over samples fs..fe of y (stride ys) from the start
+// point u (Minuit's sine-mapped parameters; sample q at y[(q - q0) * ys]); u and s end at the best point found
+__device__ __forceinline__ void fit_lmgn(const crv_params &p, const float *y, int64_t ys, int q0, int fs, int fe,
+                                         const float lo[3], const float hi[3], float u[3], FSum &s) {
+  float lam = 1e-3f;
+  FSum sn;
+  feval(y, ys, q0, fs, fe, lo, hi, u, s);
+  for (int it = 0; it < p.lm_iterations; ++it) {
+    const double add[3] = {(double)lam * ((double)s.jtj[0] + 1e-12), (double)lam * ((double)s.jtj[3] + 1e-12),
+                           (double)lam * ((double)s.jtj[5] + 1e-12)};
+    double du[3];
+    if (!fsolve(s.jtj, add, s.jtr, du)) du[0] = du[1] = du[2] = 0.0;
+    const float un[3] = {u[0] + (float)du[0], u[1] + (float)du[1], u[2] + (float)du[2]};
+    feval(y, ys, q0, fs, fe, lo, hi, un, sn);
+    const bool better = sn.chi2 < s.chi2;
+    if (better) {
+      u[0] = un[0];
+      u[1] = un[1];
+      u[2] = un[2];
+      s = sn;
+    }
+    lam = fminf(fmaxf(better ? lam * 0.3f : lam * 10.f, 1e-9f), 1e9f);
+  }
+  const double tiny[3] = {1e-12, 1e-12, 1e-12};
+  for (int it = 0; it < 8; ++it) {
+    double du[3];
+    if (!fsolve(s.jtj, tiny, s.jtr, du)) break;
+    bool moved = false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      du[i] = isfinite(du[i]) ? du[i] : 0.0;
+      moved |= du[i] != 0.0;
+    }
+    const float sc[4] = {1.f, 0.5f, 0.25f, 0.125f};
+    for (int k = 0; k < 4 && moved; ++k) {
+      const float un[3] = {u[0] + sc[k] * (float)du[0], u[1] + sc[k] * (float)du[1], u[2] + sc[k] * (float)du[2]};
+      feval(y, ys, q0, fs, fe, lo, hi, un, sn);
+      if (sn.chi2 < s.chi2) {
+        u[0] = un[0];
+        u[1] = un[1];
+        u[2] = un[2];
+        s = sn;
+        moved = false;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ void fit_run(const crv_params &p, const Tables &t, const uint64_t *__restrict__ task_key,
                                         const int64_t *__restrict__ roff, const Out &o, int64_t r, float *y,
-                                        int64_t ys, int *__restrict__ flags) {
+                                        int64_t ys, float *win, int *__restrict__ flags) {
   const int64_t g = o.run_g[r];
   const int ch = (int)(task_key[g >> 2] & 8191u) * 4 + (int)(g & 3);
   const int n = o.run_n[r];
@@ -860,48 +910,12 @@ __device__ __forceinline__ void fit_run(const crv_params &p, const Tables &t, co
     float u[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) u[i] = asinf(fminf(fmaxf(2.f * (p0[i] - lo[i]) / (hi[i] - lo[i]) - 1.f, -1.f), 1.f));
-    float lam = 1e-3f;
-    FSum s, sn;
-    feval(y, ys, fs, fe, lo, hi, u, s);
-    for (int it = 0; it < p.lm_iterations; ++it) {
-      const double add[3] = {(double)lam * ((double)s.jtj[0] + 1e-12), (double)lam * ((double)s.jtj[3] + 1e-12),
-                             (double)lam * ((double)s.jtj[5] + 1e-12)};
-      double du[3];
-      if (!fsolve(s.jtj, add, s.jtr, du)) du[0] = du[1] = du[2] = 0.0;
-      const float un[3] = {u[0] + (float)du[0], u[1] + (float)du[1], u[2] + (float)du[2]};
-      feval(y, ys, fs, fe, lo, hi, un, sn);
-      const bool better = sn.chi2 < s.chi2;
-      if (better) {
-        u[0] = un[0];
-        u[1] = un[1];
-        u[2] = un[2];
-        s = sn;
-      }
-      lam = fminf(fmaxf(better ? lam * 0.3f : lam * 10.f, 1e-9f), 1e9f);
-    }
+    FSum s;
+    const bool inw = fe - fs + 1 <= FWIN;  // the fit window in LDS ([sample][lane]): every evaluation reads it
+    if (inw)
+      for (int q = fs; q <= fe; ++q) win[(q - fs) * 64] = y[q * ys];
+    fit_lmgn(p, inw ? win : y, inw ? 64 : ys, inw ? fs : 0, fs, fe, lo, hi, u, s);
     const double tiny[3] = {1e-12, 1e-12, 1e-12};
-    for (int it = 0; it < 8; ++it) {
-      double du[3];
-      if (!fsolve(s.jtj, tiny, s.jtr, du)) break;
-      bool moved = false;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        du[i] = isfinite(du[i]) ? du[i] : 0.0;
-        moved |= du[i] != 0.0;
-      }
-      const float sc[4] = {1.f, 0.5f, 0.25f, 0.125f};
-      for (int k = 0; k < 4 && moved; ++k) {
-        const float un[3] = {u[0] + sc[k] * (float)du[0], u[1] + sc[k] * (float)du[1], u[2] + sc[k] * (float)du[2]};
-        feval(y, ys, fs, fe, lo, hi, un, sn);
-        if (sn.chi2 < s.chi2) {
-          u[0] = un[0];
-          u[1] = un[1];
-          u[2] = un[2];
-          s = sn;
-          moved = false;
-        }
-      }
-    }
     double step[3];
     const bool ok = fsolve(s.jtj, tiny, s.jtr, step);
     const double edm = 0.5 * ((double)s.jtr[0] * step[0] + (double)s.jtr[1] * step[1] + (double)s.jtr[2] * step[2]);
@@ -956,8 +970,10 @@ __device__ __forceinline__ void fit_run(const crv_params &p, const Tables &t, co
 __global__ __launch_bounds__(64) void kb_fit(crv_params p, Tables t, const uint64_t *__restrict__ task_key,
                                              const int64_t *__restrict__ roff, Out o, int64_t nruns,
                                              int *__restrict__ flags) {
+  __shared__ float win[FWIN * 64];
   const int64_t lanes = (int64_t)gridDim.x * blockDim.x, gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (int64_t r = gid; r < nruns; r += lanes) fit_run(p, t, task_key, roff, o, r, o.yscr + gid, lanes, flags);
+  for (int64_t r = gid; r < nruns; r += lanes)
+    fit_run(p, t, task_key, roff, o, r, o.yscr + gid, lanes, win + threadIdx.x, flags);
 }
 
 // fired-pixel log capacity per lane of each group of 64 SiPMs in queue order, from the largest photon count of the
