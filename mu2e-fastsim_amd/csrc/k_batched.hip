@@ -309,13 +309,8 @@ __device__ __forceinline__ bool bget(const uint32_t *bm, int lane, int pix) {
 }
 __device__ __forceinline__ void bset(uint32_t *bm, int lane, int pix) { bm[(pix >> 5) * 64 + lane] |= 1u << (pix & 31); }
 
-// overvoltage of a pixel at time offset toff: recovered from its last firing (the bitmap says whether it fired in this
-// SiPM; the per-pixel last-fire times are read only then, so they need no clearing between SiPMs)
-__device__ __forceinline__ float pixel_v(const uint32_t *bm, int lane, int pix, const float *lastt, int64_t xs,
-                                         float toff, float Vov, float inv_tau) {
-  if (!bget(bm, lane, pix)) return Vov;
-  return Vov * (1.f - __expf(-(toff - lastt[pix * xs]) * inv_tau));
-}
+// a pixel whose bit is set in a bitmap that started as the dead mask: dead (lost) or fired before in this SiPM
+__device__ __forceinline__ bool dead_px(const Tables &t, int pix) { return (t.dead[pix >> 5] >> (pix & 31)) & 1u; }
 
 // append a firing to the SiPM's log and to the per-pixel last-fire times
 __device__ __forceinline__ void log_fire(const Out &o, int64_t lb, int64_t &nlog, float *lastt, int64_t xs, int pix,
@@ -362,7 +357,9 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
       }
       const int a = task_off[tk], b = task_off[tk + 1];
       const int nstr = b - a;
-      for (int w = 0; w < CRV_DEAD_WORDS; ++w) bm[w * 64 + lane] = 0u;
+      // the bitmap starts as the dead-pixel mask: a clear bit (the usual case) is a live pixel that has not fired,
+      // so only a set bit needs the dead table (dead: the photon is lost) or the pixel's last firing
+      for (int w = 0; w < CRV_DEAD_WORDS; ++w) bm[w * 64 + lane] = t.dead[w];
       const int64_t lb = o.goff[gq >> 6] + (gq & 63), lcap = o.gcap[gq >> 6];
       const double tbase = st.t[perm[a]];  // log times are FP32 offsets from it (they span microseconds)
       o.tbase[g] = tbase;
@@ -473,8 +470,10 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           py += u32f(u.z) >= 0.5f ? 1 : 0;
         }
         const int pix = px * CRV_NPIX + py;
-        if ((t.dead[pix >> 5] >> (pix & 31)) & 1u) continue;
-        const float v = pixel_v(bm, lane, pix, lastt, xs, toff, Vov, inv_tau);
+        const bool hit_set = bget(bm, lane, pix);
+        if (hit_set && dead_px(t, pix)) continue;
+        float v = Vov;
+        if (hit_set) v = Vov * (1.f - __expf(-(toff - lastt[pix * xs]) * inv_tau));
         const float pf = v == Vov ? pf_full : p1 * (1.f - __expf(-v * inv_p2));
         if (!(u32f(u.w) < pf)) continue;
         if (nlog >= lcap) {
@@ -529,8 +528,10 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           for (int ai = 0; ai < ncand; ++ai) {
             const int c = xcand[ai * xs];
             const int m = xmult[ai * xs];
-            if ((t.dead[c >> 5] >> (c & 31)) & 1u) continue;
-            const float vc = pixel_v(bm, lane, c, lastt, xs, toff, Vov, inv_tau);
+            const bool c_set = bget(bm, lane, c);
+            if (c_set && dead_px(t, c)) continue;
+            float vc = Vov;
+            if (c_set) vc = Vov * (1.f - __expf(-(toff - lastt[c * xs]) * inv_tau));
             const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
             float qm = q1;
             for (int mm = 2; mm <= m; ++mm) qm *= q1;
