@@ -154,6 +154,17 @@ def test_uneven_showers():
     _compare(CH.run_chain(S, cond, c32, key=3, geom=G), CH.run_chain(S, cond, c64, key=3, geom=G), rtol=5e-3)
 
 
+def test_overlay_events():
+    """The overlay workload (~2,800 steps and ~5,500 pulses per event, pile-up in the fits): FP32 against FP64."""
+    c64, c32 = _cfgs()
+    cond, G = DT.nominal_conditions(), DT.sector_geom()
+    S = SY.overlay_events(40, seed=9)
+    r64 = CH.run_chain(S, cond, c64, key=3, geom=G)
+    r32 = CH.run_chain(S, cond, c32, key=3, geom=G)
+    assert len(r64["pulses"]["ev"]) > 100_000
+    _compare(r32, r64)
+
+
 def test_autotuner_picks_a_candidate_and_results_do_not_change(cosmic, tmp_path):
     """The work-queue autotuner times every candidate on device-resident steps and returns the fastest; the chain
     run with its choice gives the same tables as the default launch."""

@@ -204,6 +204,29 @@ def test_chain_edge_cases():
     _pulse_compare(CH.pulses(Wo, cond, cfg, G), OC.pulses(Wo, cond, cfg))
 
 
+def test_overlay_events_stage_by_stage():
+    """Two events of the overlay workload (cosmic mixture + 0.5 background hits per counter, ~2,800 steps each):
+    charges and digis bit-exact, pulses and clusters against the oracle on the same reco waveforms."""
+    cfg = CF.ChainConfig()
+    cond = DT.nominal_conditions()
+    G = DT.sector_geom()
+    tmpl = TP.load(cfg.waveform)
+    S = SY.overlay_events(2, seed=6)
+    Q = CH.charges(S, None, G, cond, cfg, None, key=13)
+    Qo = OC.charges(S, cond, cfg, 13)
+    assert len(Qo["chan"]) > 100_000
+    for k in ("chan", "time", "photon", "pe"):
+        np.testing.assert_array_equal(Q[k], Qo[k], err_msg=k)
+    D = CH.digis(Qo, cond, cfg, None, key=13)
+    Do = OC.digis(Qo, cond, cfg, 13, tmpl)
+    np.testing.assert_array_equal(D["adc"], Do["adc"])
+    Wo = OC.reco_waveforms(Do, cond)
+    _pulse_compare(CH.pulses(Wo, cond, cfg, G), OC.pulses(Wo, cond, cfg))
+    out, _ = CH.clusters(Wo, cond, G, None, cfg, detail=True)
+    ref, _ = OC.clusters(Wo, cond, G, cfg)
+    _cluster_compare(out[0], ref)
+
+
 def test_empty_inputs():
     cfg = CF.ChainConfig()
     cond = DT.nominal_conditions()
