@@ -1256,6 +1256,12 @@ __device__ __noinline__ void tup_leaf(const A &G, const int *g, int nl, const do
     for (int i = 0; i < nl; ++i) G.mark(g[i], w[i]);
     return;
   }
+  double lmax = -INFINITY, umin = INFINITY;  // any choice spans at least max(earliest) - min(latest)
+  for (int i = 0; i < nl; ++i) {
+    lmax = fmax(lmax, G.tmin(g[i]));
+    umin = fmin(umin, G.tmax(g[i]));
+  }
+  if (lmax - umin > mdt) return;  // no choice of hits is within the window
   for (int i = 0; i < nl; ++i)
     for (int k = G.h0(g[i]); k < G.h1(g[i]); ++k)
       if (tup_time_ok(G, g, nl, i, G.ht(k), mdt)) {
@@ -1270,10 +1276,11 @@ __device__ __forceinline__ double tup_slope(const A &G, int ga, int gb) {
 }
 
 // every tuple of the needed layer sets, split over the block by its first two groups and pruned by the widest slope
-// range and slope difference of the cluster (a pruned tuple fails its own, narrower cuts too)
+// range, slope difference and time window of the cluster (a pruned tuple fails its own, narrower cuts too; any choice
+// of hits spans at least the largest earliest time minus the smallest latest time of its groups)
 template <class A>
 __device__ void tup_search(const A &G, const int lgo[5], int nmin, int nmax, double lo, double hi, double msd,
-                           double *wq, uint8_t *kept) {
+                           double mdtw, double *wq, uint8_t *kept) {
   for (int nl = 2; nl <= 4; ++nl) {
     const bool need = nl == 2 ? nmin == 2 : nl == 3 ? (nmin <= 3 && 3 <= nmax) : nmax == 4;
     if (!need) continue;
@@ -1291,6 +1298,9 @@ __device__ void tup_search(const A &G, const int lgo[5], int nmin, int nmax, dou
         g[1] = lgo[Lc[1]] + ci % n1;
         sl[0] = tup_slope(G, g[0], g[1]);
         if (sl[0] < lo || sl[0] > hi) continue;
+        // time bound of the partial tuple: max of the groups' earliest minus min of their latest times
+        const double l01 = fmax(G.tmin(g[0]), G.tmin(g[1])), u01 = fmin(G.tmax(g[0]), G.tmax(g[1]));
+        if (l01 - u01 > mdtw) continue;
         if (nl == 2) {
           tup_leaf(G, g, 2, sl, wq, kept);
           continue;
@@ -1298,6 +1308,8 @@ __device__ void tup_search(const A &G, const int lgo[5], int nmin, int nmax, dou
         for (g[2] = lgo[Lc[2]]; g[2] < lgo[Lc[2] + 1]; ++g[2]) {
           sl[1] = tup_slope(G, g[1], g[2]);
           if (sl[1] < lo || sl[1] > hi || fabs(sl[0] - sl[1]) > msd) continue;
+          const double l2 = fmax(l01, G.tmin(g[2])), u2 = fmin(u01, G.tmax(g[2]));
+          if (l2 - u2 > mdtw) continue;
           if (nl == 3) {
             tup_leaf(G, g, 3, sl, wq, kept);
             continue;
@@ -1305,6 +1317,7 @@ __device__ void tup_search(const A &G, const int lgo[5], int nmin, int nmax, dou
           for (g[3] = lgo[Lc[3]]; g[3] < lgo[Lc[3] + 1]; ++g[3]) {
             sl[2] = tup_slope(G, g[2], g[3]);
             if (sl[2] < lo || sl[2] > hi || fabs(sl[0] - sl[2]) > msd || fabs(sl[1] - sl[2]) > msd) continue;
+            if (fmax(l2, G.tmin(g[3])) - fmin(u2, G.tmax(g[3])) > mdtw) continue;
             tup_leaf(G, g, 4, sl, wq, kept);
           }
         }
@@ -1333,7 +1346,7 @@ __device__ __noinline__ int block_check(BlockLds &L, const crv_params &p, const 
     L.ishare[6] = -1;
   }
   __syncthreads();
-  double lo = INFINITY, hi = -INFINITY, msd = -INFINITY;
+  double lo = INFINITY, hi = -INFINITY, msd = -INFINITY, mdtw = -INFINITY;
   int nmin = 1 << 30, nmax = -1, lmask = 0;
   for (int i = tid; i < n; i += BT) {
     const int h = mem[i];
@@ -1342,6 +1355,7 @@ __device__ __noinline__ int block_check(BlockLds &L, const crv_params &p, const 
     lo = fmin(lo, c.min_slope);
     hi = fmax(hi, c.max_slope);
     msd = fmax(msd, c.max_slope_diff);
+    mdtw = fmax(mdtw, c.max_dt);
     nmin = min(nmin, (int)c.coincidence_layers);
     nmax = max(nmax, (int)c.coincidence_layers);
     lmask |= 1 << S.layer[h];
@@ -1352,6 +1366,7 @@ __device__ __noinline__ int block_check(BlockLds &L, const crv_params &p, const 
   lo = block_min(L, lo);
   hi = block_max(L, hi);
   msd = block_max(L, msd);
+  mdtw = block_max(L, mdtw);
   const bool fin = !L.flag;
   nmin = L.ishare[5];
   nmax = L.ishare[6];
@@ -1440,14 +1455,14 @@ __device__ __noinline__ int block_check(BlockLds &L, const crv_params &p, const 
       T.gk[g] = 0;
     }
     __syncthreads();
-    tup_search(TupL{T}, lgo, nmin, nmax, lo, hi, msd, wq, kept);
+    tup_search(TupL{T}, lgo, nmin, nmax, lo, hi, msd, mdtw, wq, kept);
   } else {
     for (int g = tid; g < ng; g += BT) {
       gw[g] = -INFINITY;
       gk[g] = 0;
     }
     __syncthreads();
-    tup_search(TupG{S, t, byl, gs, gw, gk}, lgo, nmin, nmax, lo, hi, msd, wq, kept);
+    tup_search(TupG{S, t, byl, gs, gw, gk}, lgo, nmin, nmax, lo, hi, msd, mdtw, wq, kept);
   }
   __syncthreads();
   for (int q = tid; q < n; q += BT) {
