@@ -1212,7 +1212,7 @@ __device__ bool tup_time_ok(const A &G, const int *g, int nl, int i, double th, 
 // one tuple of groups g[0..nl) in layer order with its consecutive slopes sl[0..nl-1): the combination tests of
 // reference reco/coinc.py:156-181 with the tuple's cuts, then the widening of every hit that takes part
 template <class A>
-__device__ __noinline__ void tup_leaf(const A &G, const int *g, int nl, const double *sl, double *wq, uint8_t *kept) {
+__device__ __forceinline__ void tup_leaf(const A &G, const int *g, int nl, const double *sl, double *wq, uint8_t *kept) {
   bool skip = true;
   double mdt = -INFINITY, lo = INFINITY, hi = -INFINITY, msd = -INFINITY, m_all = INFINITY, M_all = -INFINITY;
   double xs[4];
