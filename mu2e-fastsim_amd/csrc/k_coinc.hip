@@ -802,8 +802,8 @@ __device__ __forceinline__ double block_max(BlockLds &L, double v) {
 // cluster starts to off[]; returns the number of clusters. Clusters are the link graph's connected components in
 // the order of their first position; a member's join order is (pass, position) with pass(seed) = 1 and
 // pass(p) = min over linked q of pass(q) + [q > p]. The positions' times, x, link distances and sides are staged in
-// LDS (LM) or in the scratch arrays px / pt / pmd / pside; link candidates come from time bins at least one link
-// window wide.
+// LDS (LM) or in the scratch arrays px / pt / pmd / pside; link candidates come from the neighbouring cells of a
+// (time, x) grid whose cells are at least one link window and one largest link distance wide.
 template <bool LM>
 __device__ int block_clusters_t(BlockLds &L, const BS &S, const int32_t *list, int m, const double *md,
                                 double max_dt, bool opposite, double dt_opp, int32_t *out, int32_t *off) {
@@ -834,22 +834,42 @@ __device__ int block_clusters_t(BlockLds &L, const BS &S, const int32_t *list, i
   }
   lo = block_min(L, lo);
   hi = block_max(L, hi);
+  double xlo = INFINITY, xhi = -INFINITY, mdm = -INFINITY;
+  for (int i = tid; i < m; i += BT) {
+    xlo = fmin(xlo, X[i]);
+    xhi = fmax(xhi, X[i]);
+    mdm = fmax(mdm, MD[i]);
+  }
+  xlo = block_min(L, xlo);
+  xhi = block_max(L, xhi);
+  mdm = block_max(L, mdm);
+  // cells of (time, x): at least one link window wide in time and the largest link distance wide in x, so linked
+  // positions lie in neighbouring cells; at most maxb - 2 cells
   const double mt = max_dt + (opposite ? dt_opp : 0.0);
+  const double xspan = xhi - xlo;
+  const bool xfin = xspan >= 0.0 && xspan < 1e300 && mdm >= 0.0 && mdm < 1e300;
+  const double xw0 = xfin ? fmax(mdm, 1e-9) * (1.0 + 1e-12) + 1e-9 : 1.0;
+  const int nbx = xfin ? max(1, min((int)(xspan / xw0) + 1, 32)) : 1;
+  const double xw = xfin ? fmax(xw0, xspan / (double)max(nbx - 1, 1) * (1.0 + 1e-12)) : 1.0;
+  const int tcap = (maxb - 2) / nbx;
   const double span = hi - lo;
-  const bool fin = span >= 0.0 && span < 1e300;  // else one bin: every position is a candidate of every other
-  const double width = fin ? fmax(mt, span / (double)(maxb - 2)) * (1.0 + 1e-12) + 1e-9 : 1.0;
-  const int nb = fin ? min((int)(span / width) + 1, maxb - 1) : 1;
+  const bool fin = span >= 0.0 && span < 1e300;  // else one time bin
+  const double width = fin ? fmax(mt, span / (double)max(tcap - 1, 1)) * (1.0 + 1e-12) + 1e-9 : 1.0;
+  const int nbt = fin ? max(1, min((int)(span / width) + 1, tcap)) : 1;
+  const int nb = nbt * nbx;
   for (int b = tid; b < nb + 1; b += BT) BINS[b] = 0;
   __syncthreads();
   for (int i = tid; i < m; i += BT) {
-    const double fb = fin ? (T[i] - lo) / width : 0.0;
-    const int b = fb >= 0.0 ? min((int)fmin(fb, (double)(nb - 1)), nb - 1) : 0;  // NaN -> bin 0
-    TBI[i] = b;
-    atomicAdd(&BINS[b], 1);
+    const double ft = fin ? (T[i] - lo) / width : 0.0;
+    const int bt = ft >= 0.0 ? min((int)fmin(ft, (double)(nbt - 1)), nbt - 1) : 0;  // NaN -> bin 0
+    const double fx = xfin ? (X[i] - xlo) / xw : 0.0;
+    const int bx = fx >= 0.0 ? min((int)fmin(fx, (double)(nbx - 1)), nbx - 1) : 0;
+    TBI[i] = bt * nbx + bx;
+    atomicAdd(&BINS[TBI[i]], 1);
   }
   __syncthreads();
   bscan_arr(L, BINS, nb + 1);
-  for (int i = tid; i < m; i += BT) TSO[atomicAdd(&BINS[TBI[i]], 1)] = i;  // BINS[b] ends as the end of bin b
+  for (int i = tid; i < m; i += BT) TSO[atomicAdd(&BINS[TBI[i]], 1)] = i;  // BINS[b] ends as the end of cell b
   __syncthreads();
   auto link = [&](int i, int j) {
     const double mdd = fmax(MD[i], MD[j]);
@@ -865,13 +885,15 @@ __device__ int block_clusters_t(BlockLds &L, const BS &S, const int32_t *list, i
   for (;;) {  // every thread leaves together: the vote is a barrier with a block-wide OR
     int changed = 0;
     for (int i = tid; i < m; i += BT) {
-      const int b = TBI[i];
+      const int bt = TBI[i] / nbx, bx = TBI[i] - bt * nbx;
       int best = LAB[i];
-      const int k0 = b > 1 ? BINS[b - 2] : 0, k1 = BINS[min(b + 1, nb - 1)];
-      for (int k = k0; k < k1; ++k) {
-        const int j = TSO[k];
-        const int lj = LAB[j];
-        if (lj < best && j != i && link(i, j)) best = lj;
+      for (int r = max(bt - 1, 0); r <= min(bt + 1, nbt - 1); ++r) {
+        const int c0 = r * nbx + max(bx - 1, 0), c1 = r * nbx + min(bx + 1, nbx - 1);
+        for (int k = c0 ? BINS[c0 - 1] : 0; k < BINS[c1]; ++k) {
+          const int j = TSO[k];
+          const int lj = LAB[j];
+          if (lj < best && j != i && link(i, j)) best = lj;
+        }
       }
       if (best < LAB[i]) {
         atomicMin(&LAB[i], best);
@@ -895,13 +917,15 @@ __device__ int block_clusters_t(BlockLds &L, const BS &S, const int32_t *list, i
     int changed = 0;
     for (int i = tid; i < m; i += BT) {
       if (LAB[i] == i) continue;
-      const int b = TBI[i];
+      const int bt = TBI[i] / nbx, bx = TBI[i] - bt * nbx;
       int best = PAS[i];
-      const int k0 = b > 1 ? BINS[b - 2] : 0, k1 = BINS[min(b + 1, nb - 1)];
-      for (int k = k0; k < k1; ++k) {
-        const int j = TSO[k];
-        const int cand = PAS[j] + (j > i ? 1 : 0);
-        if (cand < best && j != i && link(i, j)) best = cand;
+      for (int r = max(bt - 1, 0); r <= min(bt + 1, nbt - 1); ++r) {
+        const int c0 = r * nbx + max(bx - 1, 0), c1 = r * nbx + min(bx + 1, nbx - 1);
+        for (int k = c0 ? BINS[c0 - 1] : 0; k < BINS[c1]; ++k) {
+          const int j = TSO[k];
+          const int cand = PAS[j] + (j > i ? 1 : 0);
+          if (cand < best && j != i && link(i, j)) best = cand;
+        }
       }
       if (best < PAS[i]) {
         PAS[i] = best;
@@ -1679,21 +1703,15 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
               }
             } else if (S.clmc[c] > 0) {
               S.tmp[c] = w0;
-            } else {
-              int w = w0;
-              for (int q = 0; q < mn; ++q)
-                if (S.kept[o0 + q]) {
-                  const int h = S.byl[o0 + q];
-                  int at = w;  // insertion by pulse
-                  while (at > w0 && S.coinc[at - 1] > h) {
-                    S.coinc[at] = S.coinc[at - 1];
-                    S.cmaxd[at] = S.cmaxd[at - 1];
-                    --at;
-                  }
-                  S.coinc[at] = h;
-                  S.cmaxd[at] = S.wq[o0 + q];
-                  ++w;
-                }
+            } else {  // rank by pulse among the kept hits (independent loads, no shifting)
+              for (int q = 0; q < mn; ++q) {
+                if (!S.kept[o0 + q]) continue;
+                const int h = S.byl[o0 + q];
+                int r = 0;
+                for (int q2 = 0; q2 < mn; ++q2) r += S.kept[o0 + q2] && S.byl[o0 + q2] < h;
+                S.coinc[w0 + r] = h;
+                S.cmaxd[w0 + r] = S.wq[o0 + q];
+              }
             }
           }
           nco += tot;
