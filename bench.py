@@ -149,13 +149,13 @@ def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, aut
     tuned = None
     if autotune and precision == "fp32":  # tile / sort order of the work queues, measured on the batch's first events
         from crvhip import autotune as AT
-        a, n = chunks_of(evkey, n_ev, min(n_ev, AUTOTUNE_EVENTS))[0]
+        n_tune = min(n_ev, AUTOTUNE_EVENTS, chunk or n_ev)  # never more events than one timed chunk holds
+        a, n = chunks_of(evkey, n_ev, n_tune)[0]
         sample = L.Steps(*[getattr(base, f) + a * sz for f, sz in zip(("event", "counter", "edep", "z", "t"), sizes)],
                          n)
-        chosen, rep = AT.tune(ctx, sample, min(n_ev, AUTOTUNE_EVENTS), torch.cuda.get_device_name(0), stream=stream,
-                              cache=False)
+        chosen, rep = AT.tune(ctx, sample, n_tune, torch.cuda.get_device_name(0), stream=stream, cache=False)
         batch = {**batch, **chosen}
-        tuned = {"sample_events": min(n_ev, AUTOTUNE_EVENTS), "chosen": chosen,
+        tuned = {"sample_events": n_tune, "chosen": chosen,
                  "median_ms": {f"tile {c['opts']['tile']} sort {c['opts']['sort_tasks']}":
                                round(sorted(c["ms"])[len(c["ms"]) // 2], 2) for c in rep["candidates"]}}
         if PROGRESS:

@@ -200,13 +200,15 @@ typedef struct crv_batch_opts {
   int32_t tile;        /* SiPMs per work-queue grab, a multiple of 64 (0: 64) */
   int32_t pulse_slots; /* initial pulse slots per reco waveform (0: 4); grown and re-run when one has more */
   int32_t sort_tasks;  /* 1: run the SiPMs longest first (photon count), so the lanes of a wave carry similar work */
+  int64_t log_budget;  /* bytes of fired-pixel log per pass (0: half the device memory free at the call); a call whose
+                        * log is larger runs its SiPMs in several passes of the photon and waveform kernels */
   /* written by the call: the window_samples-long ADC record of every cluster with a digi, in no particular order
    * (label -1: unused slot), in the library's workspace until the next call on the context */
   const uint16_t *record_adc;
   const int64_t *record_label;
   const int32_t *record_tick;
   int64_t n_records, n_waveforms;
-  int32_t overlong, capped, pulse_slots_used, _pad;
+  int32_t overlong, capped, pulse_slots_used, log_passes;
 } crv_batch_opts;
 
 /* The whole chain as the FP32 batched pipeline: steps -> pulses in three persistent work-queue kernels (photons and

@@ -74,7 +74,8 @@ def _empty_outputs():
 
 def measure(ctx, steps, opts, repeats=3, stream=None):
     """Work-queue kernel times (ms: SiPM + waveform + fit kernels) of `repeats` zero-capacity runs of the batched
-    chain over the device steps `steps` (a crv_steps), after one untimed run with the same options."""
+    chain over the device steps `steps` (a crv_steps), after one untimed run with the same options; infinite for a
+    candidate whose workspace does not fit in device memory."""
     P, C = _empty_outputs()
     tm = L.Timing()
     bo = L.BatchOpts(**opts)
@@ -83,7 +84,12 @@ def measure(ctx, steps, opts, repeats=3, stream=None):
         rc = ctx.lib.crv_chain_run_batched(ctx.h, ctypes.byref(steps), ctypes.byref(bo), ctypes.byref(P),
                                            ctypes.byref(C), ctypes.byref(tm), stream)
         if rc not in (0, L.CRV_E_CAPACITY):
-            ctx.check(rc)
+            try:
+                ctx.check(rc)
+            except L.CrvError as e:
+                if "workspace allocation failed" in str(e):  # the candidate does not fit the device: never chosen
+                    return [math.inf] * repeats
+                raise
         if i:
             times.append(tm.k_sipm_ms + tm.k_digi_ms + tm.k_pulse_ms)
     return times

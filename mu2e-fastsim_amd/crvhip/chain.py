@@ -383,7 +383,7 @@ def run_chain(steps, cond, cfg, key=0, geom=None, batch=None, timers=None, recor
     cfg.precision "fp64": the plain FP64 chain (one launch per stage, equal to the oracle bit for bit); "fp32": the
     batched FP32 chain (three fused kernels, FP64 only for absolute times and the fit's 3 x 3 solve), equal to the
     FP64 chain statistically. `batch` holds crv_batch_opts fields for the FP32 chain (grid, tile, pulse_slots,
-    sort_tasks). Returns {"pulses": {...}, "clusters": cluster table, "timing": {...}} with pulses in (event, channel,
+    sort_tasks, log_budget). Returns {"pulses": {...}, "clusters": cluster table, "timing": {...}} with pulses in (event, channel,
     time) order and `wf` the pulse's reco-waveform number; with records=True (FP32 chain) also "records", the ADC
     records of the clusters with digis sorted by (label, FEB tick)."""
     import ctypes
@@ -421,7 +421,7 @@ def run_chain(steps, cond, cfg, key=0, geom=None, batch=None, timers=None, recor
     timing = {f: getattr(tm, f) for f, _ in L.Timing._fields_}
     if cfg.precision == "fp32":
         timing.update(n_records=opts.n_records, n_waveforms=opts.n_waveforms, overlong=opts.overlong,
-                      capped=opts.capped, pulse_slots_used=opts.pulse_slots_used)
+                      capped=opts.capped, pulse_slots_used=opts.pulse_slots_used, log_passes=opts.log_passes)
     if timers is not None:
         timers.update(timing)
     out = {"pulses": P, "clusters": cluster_table(cb, cs.n, cs.n_mem), "timing": timing}

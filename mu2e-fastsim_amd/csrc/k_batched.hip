@@ -15,6 +15,7 @@
 // propagation, window starts, pulse times = TDC grid + fitted offset) and the 3 x 3 normal equations of the fit are
 // solved in FP64. The draws are the Philox streams of the FP64 chain (same keys and counters), so both chains
 // simulate the same photons up to rounding; their tables agree statistically, not bit for bit.
+#include <algorithm>
 #include <hipcub/hipcub.hpp>
 
 #include "crv_internal.h"
@@ -255,6 +256,7 @@ struct Out {
   // fired-pixel log: per group of 64 SiPMs in queue order, entry k of the group's lane l at goff + 64 k + l (the
   // lanes of a wavefront write and read neighbouring words); a group holds gcap entries per lane
   const int64_t *goff, *gcap;
+  int64_t log0;     // goff of the pass's first group: the pass's log starts there
   float *log_t;     // time - tbase of the SiPM
   double *tbase;     // per SiPM: the time of its first step
   float *log_v;
@@ -292,14 +294,15 @@ struct Out {
 struct Queue {
   int *ctr;
   int tile;
-  int64_t n;
+  int64_t n;  // one past the last queue position
   const int32_t *order;
+  int64_t start;  // first queue position (a multiple of 64)
 };
 
 __device__ __forceinline__ bool grab(const Queue &qu, int *tile_base, int64_t &base) {
   if (threadIdx.x == 0) *tile_base = atomicAdd(qu.ctr, qu.tile);
   __syncthreads();
-  base = *tile_base;
+  base = qu.start + *tile_base;
   __syncthreads();
   return base < qu.n;
 }
@@ -360,7 +363,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
       // the bitmap starts as the dead-pixel mask: a clear bit (the usual case) is a live pixel that has not fired,
       // so only a set bit needs the dead table (dead: the photon is lost) or the pixel's last firing
       for (int w = 0; w < CRV_DEAD_WORDS; ++w) bm[w * 64 + lane] = t.dead[w];
-      const int64_t lb = o.goff[gq >> 6] + (gq & 63), lcap = o.gcap[gq >> 6];
+      const int64_t lb = o.goff[gq >> 6] - o.log0 + (gq & 63), lcap = o.gcap[gq >> 6];
       const double tbase = st.t[perm[a]];  // log times are FP32 offsets from it (they span microseconds)
       o.tbase[g] = tbase;
       int64_t nlog = 0;
@@ -761,7 +764,7 @@ __global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv
       L.task = g;
       L.nrun = 0;
       const int nf = nfired[g];
-      const int64_t lb = o.goff[gq >> 6] + (gq & 63);
+      const int64_t lb = o.goff[gq >> 6] - o.log0 + (gq & 63);
       const double tb0 = nf > 0 ? o.tbase[g] : 0.0;
       L.td = 0.0;
       if (nf > 0) {
@@ -1117,40 +1120,140 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     if (!tmp) return ws_fail(c);
     CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, skey, skey2, sidx, order, (int)L4, 0, 32, s));
   }
-  const Queue q_sipm{counters, tile, L4, sorted ? order : nullptr}, q_wave{counters + 1, tile, L4, sorted ? order : nullptr};
   CRV_HIP(c, hipEventRecord(ev[1], s));
   int hf[F_NFLAGS];
-  // phase 1: photons -> fired-pixel logs (a log that overflows is grown and the phase run again)
-  int64_t nlog = 0;
-  for (int attempt = 0, factor = c->b_log_factor;; ++attempt, factor *= 2) {
-    if (L4) kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, sorted ? order : nullptr, L4, gcap, gsize, factor);
-    if ((rc = scan_i64(c, gsize, goff, ngrp, 510, s))) return rc;
-    if ((rc = read_i64(c, goff + ngrp, &nlog, s))) return rc;
-    o.goff = goff;
-    o.gcap = gcap;
-    o.log_t = (float *)ws(c, 538, nlog * 4 + 8);
-    o.log_v = (float *)ws(c, 546, nlog * 4 + 8);
-    o.tbase = (double *)ws(c, 519, L4 * 8 + 8);
-    if (!o.log_t || !o.log_v || !o.tbase) return ws_fail(c);
+  int32_t cnt[2] = {0, 0};
+  float ms_sipm = 0.f, ms_wave = 0.f;  // summed over the passes of the last attempt
+  const int32_t *qorder = sorted ? order : nullptr;
+  auto kernel_ms = [&](int k) {
+    float e = 0.f;
+    (void)hipEventElapsedTime(&e, kev[2 * k], kev[2 * k + 1]);
+    return e;
+  };
+  // kb_sipm over queue positions [t0, t1): 0, 1 when a fired-pixel log overflowed, or an error code
+  auto run_sipm = [&](int64_t t0, int64_t t1) -> int {
     kb_zero<<<1, 64, 0, s>>>(counters, 1);
     kb_zero<<<1, 64, 0, s>>>(c->flags + F_OVERFLOW, 1);
     CRV_HIP(c, hipEventRecord(kev[0], s));
-    if (L4)
-      kb_sipm<<<grid_s, 64, 0, s>>>(p, c->t, c->d, st, g.task_key, g.task_off, g.perm, g.evfirst, nph, o, q_sipm, nfired,
-                                    c->flags);
+    kb_sipm<<<grid_s, 64, 0, s>>>(p, c->t, c->d, st, g.task_key, g.task_off, g.perm, g.evfirst, nph, o,
+                                  Queue{counters, tile, t1, qorder, t0}, nfired, c->flags);
     CRV_HIP(c, hipGetLastError());
     CRV_HIP(c, hipEventRecord(kev[1], s));
     CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
     CRV_HIP(c, hipStreamSynchronize(s));
+    ms_sipm += kernel_ms(0);
     if (hf[F_CASCADE])
       return set_error(c, CRV_E_OVERFLOW,
                        "batched chain: cross-talk cascade wider than 16 pixels or deeper than 31 generations");
     if (hf[F_COUNTER]) return set_error(c, CRV_E_OVERFLOW, "batched chain: more than 65535 photoelectrons from one step");
-    if (!hf[F_OVERFLOW]) {
+    return hf[F_OVERFLOW] ? 1 : 0;
+  };
+  auto run_wave = [&](int64_t t0, int64_t t1) -> int {
+    kb_zero<<<1, 64, 0, s>>>(counters + 1, 1);
+    CRV_HIP(c, hipEventRecord(kev[2], s));
+    kb_wave<<<grid_w, 64, 0, s>>>(p, c->t, c->d, g.task_key, nfired, o, Queue{counters + 1, tile, t1, qorder, t0},
+                                  c->flags);
+    CRV_HIP(c, hipGetLastError());
+    CRV_HIP(c, hipEventRecord(kev[3], s));
+    CRV_HIP(c, hipStreamSynchronize(s));
+    ms_wave += kernel_ms(1);
+    return 0;
+  };
+  // record and run pools at their current capacities
+  auto pools = [&]() -> int {
+    o.rec_cap = std::max<int64_t>(c->b_rec_cap, L4 + 1024);
+    o.run_cap = std::max<int64_t>(c->b_run_cap, L4 + 1024);
+    o.rec_adc = (uint16_t *)ws(c, 570, o.rec_cap * W * 2);
+    o.rec_label = (int64_t *)ws(c, 571, o.rec_cap * 8);
+    o.rec_tick = (int32_t *)ws(c, 572, o.rec_cap * 4);
+    o.run_rec = (int32_t *)ws(c, 573, o.run_cap * 4);
+    o.run_i0 = (int32_t *)ws(c, 574, o.run_cap * 4);
+    o.run_n = (int32_t *)ws(c, 575, o.run_cap * 4);
+    o.run_k = (int32_t *)ws(c, 576, o.run_cap * 4);
+    o.run_g = (int64_t *)ws(c, 577, o.run_cap * 8);
+    o.run_tdc = (int64_t *)ws(c, 578, o.run_cap * 8);
+    if (!o.rec_adc || !o.rec_label || !o.rec_tick || !o.run_rec || !o.run_i0 || !o.run_n || !o.run_k || !o.run_g ||
+        !o.run_tdc)
+      return ws_fail(c);
+    return 0;
+  };
+  // Fired-pixel log budget (bytes of log_t + log_v): the caller's, else half of the device memory free now plus what
+  // the log already holds. When the whole call's log exceeds it, the queue is cut into passes of whole groups whose
+  // logs fit, and each pass runs kb_sipm then kb_wave on its SiPMs (records and runs accumulate over the passes).
+  size_t budget = opt->log_budget > 0 ? (size_t)opt->log_budget : 0;
+  if (!budget) {
+    size_t fr = 0, total = 0;
+    CRV_HIP(c, hipMemGetInfo(&fr, &total));
+    const size_t held = ((int)c->ws_size.size() > 546 ? c->ws_size[538] + c->ws_size[546] : 0);
+    budget = (fr + held) / 2;
+  }
+  std::vector<int64_t> hoff, cuts;  // group offsets on the host; first group of each pass, then ngrp
+  int64_t nlog = 0;
+  int npass = 1;
+  // phases 1 and 2: photons -> fired-pixel logs -> records and reco waveforms. A log overflow grows the logs and runs
+  // everything again; a pool overflow grows the pools and runs kb_wave again (and kb_sipm, when passes reuse the log).
+  for (int attempt = 0, factor = c->b_log_factor;; ++attempt, factor *= 2) {
+    if (attempt > 8) return set_error(c, CRV_E_OVERFLOW, "batched chain: fired-pixel log still too short");
+    if (L4) kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, qorder, L4, gcap, gsize, factor);
+    if ((rc = scan_i64(c, gsize, goff, ngrp, 510, s))) return rc;
+    if ((rc = read_i64(c, goff + ngrp, &nlog, s))) return rc;
+    int64_t log_entries = nlog;
+    cuts.assign({0, ngrp});
+    if ((size_t)nlog * 8 > budget) {
+      hoff.resize(ngrp + 1);
+      CRV_HIP(c, hipMemcpyAsync(hoff.data(), goff, (ngrp + 1) * 8, hipMemcpyDeviceToHost, s));
+      CRV_HIP(c, hipStreamSynchronize(s));
+      const int64_t lim = (int64_t)(budget / 8);
+      cuts.assign(1, 0);
+      log_entries = 0;
+      for (int64_t g0 = 0; g0 < ngrp;) {
+        const int64_t g1 = std::upper_bound(hoff.begin() + g0 + 1, hoff.end(), hoff[g0] + lim) - hoff.begin() - 1;
+        if (g1 <= g0)
+          return set_error(c, CRV_E_ARG, "batched chain: the fired-pixel log of one group of 64 SiPMs (" +
+                                             std::to_string((hoff[g0 + 1] - hoff[g0]) * 8 >> 20) +
+                                             " MiB) exceeds the log budget");
+        log_entries = std::max(log_entries, hoff[g1] - hoff[g0]);
+        cuts.push_back(g1);
+        g0 = g1;
+      }
+    }
+    npass = (int)cuts.size() - 1;
+    o.goff = goff;
+    o.gcap = gcap;
+    o.log_t = (float *)ws(c, 538, log_entries * 4 + 8);
+    o.log_v = (float *)ws(c, 546, log_entries * 4 + 8);
+    o.tbase = (double *)ws(c, 519, L4 * 8 + 8);
+    if (!o.log_t || !o.log_v || !o.tbase) return ws_fail(c);
+    bool grow_log = false;
+    for (int pa = 0;; ++pa) {
+      if ((rc = pools())) return rc;
+      kb_zero<<<1, 64, 0, s>>>(counters + 2, 2);
+      ms_wave = 0.f;
+      if (pa == 0 || npass > 1) ms_sipm = 0.f;
+      for (int pi = 0; pi < npass && L4; ++pi) {
+        const int64_t t0 = cuts[pi] * 64, t1 = std::min<int64_t>(cuts[pi + 1] * 64, L4);
+        o.log0 = npass > 1 ? hoff[cuts[pi]] : 0;
+        if (pa == 0 || npass > 1) {
+          if ((rc = run_sipm(t0, t1)) < 0) return rc;
+          if (rc) {
+            grow_log = true;
+            break;
+          }
+        }
+        if ((rc = run_wave(t0, t1))) return rc;
+      }
+      if (grow_log) break;
+      CRV_HIP(c, hipMemcpyAsync(cnt, counters + 2, 8, hipMemcpyDeviceToHost, s));
+      CRV_HIP(c, hipStreamSynchronize(s));
+      if (cnt[0] <= o.rec_cap && cnt[1] <= o.run_cap) break;
+      if (pa >= 3) return set_error(c, CRV_E_OVERFLOW, "batched chain: record or run pool still too small");
+      c->b_rec_cap = std::max<int64_t>(o.rec_cap, (int64_t)cnt[0] + cnt[0] / 4 + 1024);
+      c->b_run_cap = std::max<int64_t>(o.run_cap, (int64_t)cnt[1] + cnt[1] / 4 + 1024);
+    }
+    if (!grow_log) {
       c->b_log_factor = factor;
       break;
     }
-    if (attempt >= 8) return set_error(c, CRV_E_OVERFLOW, "batched chain: fired-pixel log still too short");
   }
   // photons drawn and pixels fired (diagnostics)
   int64_t *sums = (int64_t *)ws(c, 515, 16);
@@ -1166,35 +1269,6 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, fired64, sums, (int)L4, s));
     CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, sipm_nph, sums + 1, (int)L4, s));
     CRV_HIP(c, hipMemcpyAsync(hsum, sums, 16, hipMemcpyDeviceToHost, s));
-  }
-  // phase 2: fired pixels -> records and reco waveforms (pools grown and the phase run again when they overflow)
-  int32_t cnt[2] = {0, 0};
-  for (int attempt = 0;; ++attempt) {
-    o.rec_cap = std::max<int64_t>(c->b_rec_cap, L4 + 1024);
-    o.run_cap = std::max<int64_t>(c->b_run_cap, L4 + 1024);
-    o.rec_adc = (uint16_t *)ws(c, 570, o.rec_cap * W * 2);
-    o.rec_label = (int64_t *)ws(c, 571, o.rec_cap * 8);
-    o.rec_tick = (int32_t *)ws(c, 572, o.rec_cap * 4);
-    o.run_rec = (int32_t *)ws(c, 573, o.run_cap * 4);
-    o.run_i0 = (int32_t *)ws(c, 574, o.run_cap * 4);
-    o.run_n = (int32_t *)ws(c, 575, o.run_cap * 4);
-    o.run_k = (int32_t *)ws(c, 576, o.run_cap * 4);
-    o.run_g = (int64_t *)ws(c, 577, o.run_cap * 8);
-    o.run_tdc = (int64_t *)ws(c, 578, o.run_cap * 8);
-    if (!o.rec_adc || !o.rec_label || !o.rec_tick || !o.run_rec || !o.run_i0 || !o.run_n || !o.run_k || !o.run_g ||
-        !o.run_tdc)
-      return ws_fail(c);
-    kb_zero<<<1, 64, 0, s>>>(counters + 1, 3);
-    CRV_HIP(c, hipEventRecord(kev[2], s));
-    if (L4) kb_wave<<<grid_w, 64, 0, s>>>(p, c->t, c->d, g.task_key, nfired, o, q_wave, c->flags);
-    CRV_HIP(c, hipGetLastError());
-    CRV_HIP(c, hipEventRecord(kev[3], s));
-    CRV_HIP(c, hipMemcpyAsync(cnt, counters + 2, 8, hipMemcpyDeviceToHost, s));
-    CRV_HIP(c, hipStreamSynchronize(s));
-    if (cnt[0] <= o.rec_cap && cnt[1] <= o.run_cap) break;
-    if (attempt >= 3) return set_error(c, CRV_E_OVERFLOW, "batched chain: record or run pool still too small");
-    c->b_rec_cap = std::max<int64_t>(o.rec_cap, (int64_t)cnt[0] + cnt[0] / 4 + 1024);
-    c->b_run_cap = std::max<int64_t>(o.run_cap, (int64_t)cnt[1] + cnt[1] / 4 + 1024);
   }
   const int64_t nrec = cnt[0], nruns = cnt[1];
   // the runs' places in the output: each SiPM's runs in run-number order
@@ -1272,10 +1346,13 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   opt->overlong = hf[F_OVERLONG];
   opt->capped = hf[F_CAPPED];
   opt->pulse_slots_used = o.pcap;
+  opt->log_passes = npass;
   if (timing) {
     float ms[4], km[3] = {0.f, 0.f, 0.f}, kc = 0.f, tot = 0.f;
     for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
-    for (int i = 0; i < 3; ++i) (void)hipEventElapsedTime(&km[i], kev[2 * i], kev[2 * i + 1]);
+    km[0] = ms_sipm;
+    km[1] = ms_wave;
+    (void)hipEventElapsedTime(&km[2], kev[4], kev[5]);
     if (np && c->kev_on) (void)hipEventElapsedTime(&kc, kev[6], kev[7]);
     (void)hipEventElapsedTime(&tot, ev[0], ev[4]);
     *timing = crv_timing{};
@@ -1284,8 +1361,8 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     timing->pulses_ms = ms[2];    // pulse table
     timing->clusters_ms = ms[3];
     timing->total_ms = tot;
-    timing->k_sipm_ms = km[0];    // kb_sipm (last run)
-    timing->k_digi_ms = km[1];    // kb_wave (last run)
+    timing->k_sipm_ms = km[0];    // kb_sipm (the passes of the last run)
+    timing->k_digi_ms = km[1];    // kb_wave (the passes of the last run)
     timing->k_pulse_ms = km[2];   // kb_fit (last run)
     timing->k_coinc_ms = kc;
     timing->n_steps = n;

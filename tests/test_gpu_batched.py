@@ -107,6 +107,29 @@ def test_launch_shape_does_not_change_results(cosmic):
     assert r["timing"]["pulse_slots_used"] >= 2  # the one-slot run had to grow
 
 
+@pytest.mark.parametrize("sort_tasks", [1, 0])
+def test_log_passes_do_not_change_results(cosmic, sort_tasks):
+    """A fired-pixel log budget far below the call's log cuts the SiPMs into passes of the photon and waveform kernels
+    (records and runs accumulate over the passes); the tables are those of the one-pass run."""
+    S, cond, G, c64, c32, r64, r32 = cosmic
+    r = CH.run_chain(S, cond, c32, key=7, geom=G, batch={"log_budget": 16 << 20, "sort_tasks": sort_tasks})
+    assert r["timing"]["log_passes"] >= 3
+    assert r32["timing"]["log_passes"] == 1
+    for k, v in r32["pulses"].items():
+        np.testing.assert_array_equal(r["pulses"][k], v, err_msg=k)
+    for k, v in r32["clusters"].items():
+        np.testing.assert_array_equal(r["clusters"][k], v, err_msg=k)
+    assert r["timing"]["n_charges"] == r32["timing"]["n_charges"]
+    assert r["timing"]["n_records"] == r32["timing"]["n_records"]
+
+
+def test_log_budget_below_one_group_is_an_error(cosmic):
+    from crvhip import _lib as L
+    S, cond, G, c64, c32, r64, r32 = cosmic
+    with pytest.raises(L.CrvError, match="exceeds the log budget"):
+        CH.run_chain(S, cond, c32, key=7, geom=G, batch={"log_budget": 64})
+
+
 def test_records_match_fp64_records(cosmic):
     S, cond, G, c64, c32, r64, r32 = cosmic
     R = r32["records"]
