@@ -26,6 +26,8 @@ namespace {
 
 constexpr int CMAXB = 16;   // cascade candidates per generation and lane
 constexpr int MAXGEN = 31;  // cascade generations
+// bits of the 7 x 7 cascade window (bit (dx + 3) * 7 + dy + 3) with |dx| = 3 or |dy| = 3
+constexpr uint64_t WIN_BORDER = 0x1FE0C183060FFull;
 constexpr int RINGB = 16;   // open samples of the waveform window
 constexpr int PSLOTS = 4;   // pulse slots per reco waveform (more: the fits run again with more)
 constexpr int DWORDS = 6;   // digi-start mask words: window_samples <= 192
@@ -36,6 +38,31 @@ constexpr int TMPL_LDS = 1024;  // template samples staged in LDS by the wavefor
 constexpr int FWIN = 16;        // fit-window samples staged in LDS per lane by the fit kernel
 
 __device__ __forceinline__ float u32f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * 0x1p-24f; }
+
+// Philox4x32-10 exactly as philox() of crv_math.h, with the round keys formed by scalar adds inside every call. Left
+// to itself the compiler hoists the 20 round keys of each draw kind out of the photon loop, and the scalar registers
+// they take are spilled and reloaded in the loop.
+__device__ __forceinline__ U4 philox_rk(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+      asm volatile("" : "+s"(k0), "+s"(k1));  // keys are wave-uniform: kept in scalar registers, not hoisted
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+    c = U4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
+  }
+  return c;
+}
+
+// Draw with philox_rk (the same values); seed and evhi must be wave-uniform
+struct DrawR {
+  uint32_t seed, evhi;
+  __device__ __forceinline__ U4 operator()(uint32_t kind, uint32_t ev, uint32_t c1, uint32_t c2, uint32_t c3) const {
+    return philox_rk(U4{ev, c1, c2, c3}, seed, kind + 16u * evhi);
+  }
+};
 
 __device__ __forceinline__ float lgam_int_f(float x) {  // log((x - 1)!) for integer x >= 1
   if (x < 10.f) {
@@ -338,7 +365,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
   const float inv_tau = 1.f / (float)p.time_constant, inv_p2 = 1.f / (float)p.p2;
   const float pf_full = p1 * (1.f - __expf(-Vov * inv_p2));
   const float qct = (float)dv.q_ct, tauf = (float)p.tau;
-  const Draw dr{(uint32_t)p.seed, p.evkey_hi};
+  const DrawR dr{(uint32_t)p.seed, p.evkey_hi};
   const float inv_ny = 1.f / (float)t.map_ny;
   const int64_t xb = (int64_t)blockIdx.x * 64 + lane;
   uint16_t *xcur = o.xcur + xb, *xcand = o.xcand + xb;
@@ -485,13 +512,82 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
         }
         log_fire(o, lb, nlog, lastt, xs, pix, toff, v);
         bset(bm, lane, pix);
-        // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order (generation 0 is
-        // the photon's pixel alone, kept in a register)
-        int ncur = 1, gnum = 0;
+        // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order. While a
+        // generation's pixels stay inside the 7 x 7 window around the photon's pixel, the generation, its candidates
+        // and their multiplicities are bit masks over the window in registers (bit (dx + 3) * 7 + dy + 3; ascending
+        // bit = ascending pixel); a generation with a pixel on the window's border continues in the per-lane lists
+        // in global scratch below. Both visit the same draws in the same order.
+        const int cx = pix / CRV_NPIX, cy = pix % CRV_NPIX;
+        uint64_t cur = 1ull << 24;  // generation 0: the photon's pixel at the window's centre
+        int gnum = 0;
+        bool window = true;
+        while (!bad) {
+          if (cur & WIN_BORDER) {
+            window = false;  // neighbours could leave the window
+            break;
+          }
+          uint64_t m0 = 0, m1 = 0, m2 = 0;  // candidate multiplicity, bit planes of a 3-bit count per position
+          for (uint64_t rem = cur; rem; rem &= rem - 1) {
+            const int bq = __ffsll((long long)rem) - 1;
+            const int fx = cx + bq / 7 - 3, fy = cy + bq % 7 - 3;
+            const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j,
+                             pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)(fx * CRV_NPIX + fy)));
+            uint64_t add = 0;
+            if (fx > 0 && u32f(ux.x) < qct) add |= 1ull << (bq - 7);
+            if (fx + 1 < p.n_pixels_x && u32f(ux.y) < qct) add |= 1ull << (bq + 7);
+            if (fy > 0 && u32f(ux.z) < qct) add |= 1ull << (bq - 1);
+            if (fy + 1 < p.n_pixels_y && u32f(ux.w) < qct) add |= 1ull << (bq + 1);
+            const uint64_t c0 = m0 & add;
+            m0 ^= add;
+            const uint64_t c1 = m1 & c0;
+            m1 ^= c0;
+            m2 ^= c1;
+          }
+          const uint64_t cand = m0 | m1 | m2;
+          if (!cand) break;
+          if (__popcll(cand) > CMAXB || gnum + 1 > MAXGEN) {
+            atomicAdd(&flags[F_CASCADE], 1);
+            bad = true;
+            break;
+          }
+          ++gnum;
+          uint64_t nxt = 0;
+          for (uint64_t rem = cand; rem; rem &= rem - 1) {
+            const int bq = __ffsll((long long)rem) - 1;
+            const int c = (cx + bq / 7 - 3) * CRV_NPIX + cy + bq % 7 - 3;
+            const int m = (int)((m0 >> bq) & 1u) + 2 * (int)((m1 >> bq) & 1u) + 4 * (int)((m2 >> bq) & 1u);
+            const bool c_set = bget(bm, lane, c);
+            if (c_set && dead_px(t, c)) continue;
+            float vc = Vov;
+            if (c_set) vc = Vov * (1.f - __expf(-(toff - lastt[c * xs]) * inv_tau));
+            const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
+            float qm = q1;
+            for (int mm = 2; mm <= m; ++mm) qm *= q1;
+            const float uc =
+                u32f(dr(KIND_XTFIRE, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)c)).x);
+            if (!(uc < 1.f - qm)) continue;
+            if (nlog >= lcap) {
+              bad = true;
+              break;
+            }
+            log_fire(o, lb, nlog, lastt, xs, c, toff, vc);
+            bset(bm, lane, c);
+            nxt |= 1ull << bq;
+          }
+          cur = nxt;
+          if (!cur) break;
+        }
+        if (bad) break;
+        if (window) continue;
+        int ncur = 0;
+        for (uint64_t rem = cur; rem; rem &= rem - 1, ++ncur) {
+          const int bq = __ffsll((long long)rem) - 1;
+          xcur[ncur * xs] = (uint16_t)((cx + bq / 7 - 3) * CRV_NPIX + cy + bq % 7 - 3);
+        }
         while (ncur > 0 && !bad) {
           int ncand = 0;
           for (int ai = 0; ai < ncur; ++ai) {
-            const int f = gnum == 0 ? pix : (int)xcur[ai * xs];
+            const int f = (int)xcur[ai * xs];
             const int fx = f / CRV_NPIX, fy = f % CRV_NPIX;
             const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)f));
             const uint32_t uw[4] = {ux.x, ux.y, ux.z, ux.w};
@@ -595,7 +691,7 @@ __device__ __forceinline__ void wave_step(const crv_params &p, const Lane &L, co
   const bool valid = !at_close || J < length;
   if (valid && p.noise_on) {
     if ((J & 1) == 0)
-      w.nz = Draw{(uint32_t)p.seed, p.evkey_hi}(KIND_NOISE, L.ev, (uint32_t)L.ch, (uint32_t)w.kf, (uint32_t)(J >> 1));
+      w.nz = DrawR{(uint32_t)p.seed, p.evkey_hi}(KIND_NOISE, L.ev, (uint32_t)L.ch, (uint32_t)w.kf, (uint32_t)(J >> 1));
     const float u1 = (J & 1) ? u32f(w.nz.z) : u32f(w.nz.x), u2 = (J & 1) ? u32f(w.nz.w) : u32f(w.nz.y);
     v += (float)p.noise * sqrtf(-2.f * __logf(u1)) * __cosf(6.28318531f * u2);
   }
@@ -607,9 +703,9 @@ __device__ __forceinline__ void wave_step(const crv_params &p, const Lane &L, co
   if (i_valid && (w.above & 0x1Fu) != 0u && w.next_i <= i) {
     const double tstart = w.t_start + (double)i * PERIODD;
     if (tstart >= p.window_lo && tstart <= p.window_hi) {  // i < length <= W - samples_zs < 32 * DWORDS
+      const uint32_t bit = 1u << (i & 31);
 #pragma unroll
-      for (int q = 0; q < DWORDS; ++q)
-        if (q == (i >> 5)) w.dmask[q] |= 1u << (i & 31);
+      for (int q = 0; q < DWORDS; ++q) w.dmask[q] |= q == (i >> 5) ? bit : 0u;  // a select per word: stays in VGPRs
       ++w.nd;
       w.next_i = i + p.samples_zs;
     }
@@ -747,7 +843,7 @@ __global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv
   __shared__ float tmpl_s[TMPL_LDS];  // the single-PE template, staged once per workgroup
   const int lane = threadIdx.x;
   const double gap = p.single_pe_max_time + 4.0 * PERIODD;
-  const Draw dr{(uint32_t)p.seed, p.evkey_hi};
+  const DrawR dr{(uint32_t)p.seed, p.evkey_hi};
   for (int i = lane; i < t.ntmpl; i += 64) tmpl_s[i] = t.tmpl_f[i];
   __syncthreads();
   Lane L;
