@@ -280,13 +280,13 @@ __device__ __forceinline__ void feig(const float jtj[6], float &emin, float &ema
 
 
 struct Out {
-  // fired-pixel log: per group of 64 SiPMs in queue order, entry k of the group's lane l at goff + 64 k + l (the
-  // lanes of a wavefront write and read neighbouring words); a group holds gcap entries per lane
+  // fired-pixel log: entries (time - tbase of the SiPM, overvoltage) in chunks of two (16 bytes); per group of 64
+  // SiPMs in queue order, chunk c of the group's lane l at goff + 64 c + l (the lanes of a wavefront write and read
+  // neighbouring chunks); a group holds gcap entries (even) per lane
   const int64_t *goff, *gcap;
   int64_t log0;     // goff of the pass's first group: the pass's log starts there
-  float *log_t;     // time - tbase of the SiPM
+  float4 *flog;
   double *tbase;     // per SiPM: the time of its first step
-  float *log_v;
   float *lastt;      // per persistent lane: time offset of each pixel's last firing, [pixel][grid lane]
   // photon streams of the (step, SiPM) pairs of counters with several steps
   float *gS;
@@ -343,10 +343,15 @@ __device__ __forceinline__ void bset(uint32_t *bm, int lane, int pix) { bm[(pix 
 __device__ __forceinline__ bool dead_px(const Tables &t, int pix) { return (t.dead[pix >> 5] >> (pix & 31)) & 1u; }
 
 // append a firing to the SiPM's log and to the per-pixel last-fire times
-__device__ __forceinline__ void log_fire(const Out &o, int64_t lb, int64_t &nlog, float *lastt, int64_t xs, int pix,
-                                         float toff, float v) {
-  o.log_t[lb + nlog * 64] = toff;
-  o.log_v[lb + nlog * 64] = v;
+// (an even entry waits in registers, pt / pv, and goes out with the next one as one 16-byte chunk)
+__device__ __forceinline__ void log_fire(const Out &o, int64_t lb, int64_t &nlog, float &pt, float &pv, float *lastt,
+                                         int64_t xs, int pix, float toff, float v) {
+  if (nlog & 1) {
+    o.flog[lb + (nlog >> 1) * 64] = make_float4(pt, pv, toff, v);
+  } else {
+    pt = toff;
+    pv = v;
+  }
   lastt[pix * xs] = toff;
   ++nlog;
 }
@@ -394,6 +399,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
       const double tbase = st.t[perm[a]];  // log times are FP32 offsets from it (they span microseconds)
       o.tbase[g] = tbase;
       int64_t nlog = 0;
+      float pend_t = 0.f, pend_v = 0.f;  // the log's pending even entry
       // photon streams: in registers for a counter with one step, in global memory otherwise
       float S1 = 0.f;
       double head1 = 0.0, tb1 = 0.0;
@@ -510,7 +516,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           bad = true;
           break;
         }
-        log_fire(o, lb, nlog, lastt, xs, pix, toff, v);
+        log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, pix, toff, v);
         bset(bm, lane, pix);
         // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order. While a
         // generation's pixels stay inside the 7 x 7 window around the photon's pixel, the generation, its candidates
@@ -570,7 +576,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
               bad = true;
               break;
             }
-            log_fire(o, lb, nlog, lastt, xs, c, toff, vc);
+            log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, c, toff, vc);
             bset(bm, lane, c);
             nxt |= 1ull << bq;
           }
@@ -641,7 +647,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
               bad = true;
               break;
             }
-            log_fire(o, lb, nlog, lastt, xs, c, toff, vc);
+            log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, c, toff, vc);
             bset(bm, lane, c);
             xcur[ncur * xs] = (uint16_t)c;
             ++ncur;
@@ -649,6 +655,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
         }
         if (bad) break;
       }
+      if (nlog & 1) o.flog[lb + (nlog >> 1) * 64] = make_float4(pend_t, pend_v, 0.f, 0.f);
       if (bad) atomicAdd(&flags[F_OVERFLOW], 1);  // the batch is run again with larger logs
       nfired[g] = (int32_t)nlog;
     }
@@ -836,7 +843,7 @@ __device__ __forceinline__ void wave_close(const crv_params &p, const Tables &t,
 
 // kb_wave, one lane per SiPM: its fired pixels in time order -> charge clusters -> waveform window (template sums,
 // noise, ADC, zero suppression) -> ADC records and reco waveforms of the clusters with digis.
-__global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv, const uint64_t *__restrict__ task_key,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void kb_wave(crv_params p, Tables t, Derived dv, const uint64_t *__restrict__ task_key,
                                               const int32_t *__restrict__ nfired, Out o, Queue qu,
                                               int *__restrict__ flags) {
   __shared__ int tile_base;
@@ -872,14 +879,20 @@ __global__ __launch_bounds__(64) void kb_wave(crv_params p, Tables t, Derived dv
       wv.overlong = 0;
       wv.rec = -1;
       int q = 0;
+      float4 ck = make_float4(0.f, 0.f, 0.f, 0.f);  // the log chunk holding entry q
+      if (nf > 0) ck = o.flog[lb];
       while (q < nf) {
         // the charges of one time (a fired pixel and its cross-talk)
-        const float tqf = o.log_t[lb + (int64_t)q * 64];
+        const float tqf = (q & 1) ? ck.z : ck.x;
         const double tq = tb0 + (double)tqf;
-        float vsum = o.log_v[lb + (int64_t)q * 64];
+        float vsum = (q & 1) ? ck.w : ck.y;
         int n = 1;
-        while (q + n < nf && o.log_t[lb + (int64_t)(q + n) * 64] == tqf) {
-          vsum += o.log_v[lb + (int64_t)(q + n) * 64];
+        for (;;) {
+          const int e = q + n;
+          if (e >= nf) break;
+          if ((e & 1) == 0) ck = o.flog[lb + (int64_t)(e >> 1) * 64];
+          if (((e & 1) ? ck.z : ck.x) != tqf) break;
+          vsum += (e & 1) ? ck.w : ck.y;
           ++n;
         }
         q += n;
@@ -1087,8 +1100,9 @@ __global__ void kb_gcaps(const int64_t *__restrict__ nph, const int32_t *__restr
     const int64_t v = nph[order ? order[gq] : gq];
     m = max(m, v * 3 * factor / 4 + 16);  // fired pixels: about 42 % of the photons, binomial spread
   }
+  m = (m + 1) & ~(int64_t)1;
   gcap[grp] = m;
-  gsize[grp] = m * 64;
+  gsize[grp] = m / 2 * 64;  // chunks of two entries
 }
 
 __global__ void kb_zero(int *__restrict__ p, int n) {
@@ -1273,14 +1287,14 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
       return ws_fail(c);
     return 0;
   };
-  // Fired-pixel log budget (bytes of log_t + log_v): the caller's, else half of the device memory free now plus what
+  // Fired-pixel log budget (bytes): the caller's, else half of the device memory free now plus what
   // the log already holds. When the whole call's log exceeds it, the queue is cut into passes of whole groups whose
   // logs fit, and each pass runs kb_sipm then kb_wave on its SiPMs (records and runs accumulate over the passes).
   size_t budget = opt->log_budget > 0 ? (size_t)opt->log_budget : 0;
   if (!budget) {
     size_t fr = 0, total = 0;
     CRV_HIP(c, hipMemGetInfo(&fr, &total));
-    const size_t held = ((int)c->ws_size.size() > 546 ? c->ws_size[538] + c->ws_size[546] : 0);
+    const size_t held = ((int)c->ws_size.size() > 538 ? c->ws_size[538] : 0);
     budget = (fr + held) / 2;
   }
   std::vector<int64_t> hoff, cuts;  // group offsets on the host; first group of each pass, then ngrp
@@ -1293,22 +1307,22 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     if (L4) kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, qorder, L4, gcap, gsize, factor);
     if ((rc = scan_i64(c, gsize, goff, ngrp, 510, s))) return rc;
     if ((rc = read_i64(c, goff + ngrp, &nlog, s))) return rc;
-    int64_t log_entries = nlog;
+    int64_t log_chunks = nlog;  // goff and nlog count 16-byte chunks
     cuts.assign({0, ngrp});
-    if ((size_t)nlog * 8 > budget) {
+    if ((size_t)nlog * 16 > budget) {
       hoff.resize(ngrp + 1);
       CRV_HIP(c, hipMemcpyAsync(hoff.data(), goff, (ngrp + 1) * 8, hipMemcpyDeviceToHost, s));
       CRV_HIP(c, hipStreamSynchronize(s));
-      const int64_t lim = (int64_t)(budget / 8);
+      const int64_t lim = (int64_t)(budget / 16);
       cuts.assign(1, 0);
-      log_entries = 0;
+      log_chunks = 0;
       for (int64_t g0 = 0; g0 < ngrp;) {
         const int64_t g1 = std::upper_bound(hoff.begin() + g0 + 1, hoff.end(), hoff[g0] + lim) - hoff.begin() - 1;
         if (g1 <= g0)
           return set_error(c, CRV_E_ARG, "batched chain: the fired-pixel log of one group of 64 SiPMs (" +
-                                             std::to_string((hoff[g0 + 1] - hoff[g0]) * 8 >> 20) +
+                                             std::to_string((hoff[g0 + 1] - hoff[g0]) * 16 >> 20) +
                                              " MiB) exceeds the log budget");
-        log_entries = std::max(log_entries, hoff[g1] - hoff[g0]);
+        log_chunks = std::max(log_chunks, hoff[g1] - hoff[g0]);
         cuts.push_back(g1);
         g0 = g1;
       }
@@ -1316,10 +1330,9 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     npass = (int)cuts.size() - 1;
     o.goff = goff;
     o.gcap = gcap;
-    o.log_t = (float *)ws(c, 538, log_entries * 4 + 8);
-    o.log_v = (float *)ws(c, 546, log_entries * 4 + 8);
+    o.flog = (float4 *)ws(c, 538, log_chunks * 16 + 16);
     o.tbase = (double *)ws(c, 519, L4 * 8 + 8);
-    if (!o.log_t || !o.log_v || !o.tbase) return ws_fail(c);
+    if (!o.flog || !o.tbase) return ws_fail(c);
     bool grow_log = false;
     for (int pa = 0;; ++pa) {
       if ((rc = pools())) return rc;
