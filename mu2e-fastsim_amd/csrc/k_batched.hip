@@ -931,6 +931,8 @@ __device__ __forceinline__ void fit_lmgn(const crv_params &p, const float *y, in
       u[1] = un[1];
       u[2] = un[2];
       s = sn;
+    } else if (lam >= 1e9f) {
+      break;  // u, s and lam stay as they are: every remaining iteration would repeat this one
     }
     lam = fminf(fmaxf(better ? lam * 0.3f : lam * 10.f, 1e-9f), 1e9f);
   }
@@ -945,6 +947,7 @@ __device__ __forceinline__ void fit_lmgn(const crv_params &p, const float *y, in
       moved |= du[i] != 0.0;
     }
     const float sc[4] = {1.f, 0.5f, 0.25f, 0.125f};
+    bool improved = false;
     for (int k = 0; k < 4 && moved; ++k) {
       const float un[3] = {u[0] + sc[k] * (float)du[0], u[1] + sc[k] * (float)du[1], u[2] + sc[k] * (float)du[2]};
       feval(y, ys, q0, fs, fe, lo, hi, un, sn);
@@ -954,8 +957,10 @@ __device__ __forceinline__ void fit_lmgn(const crv_params &p, const float *y, in
         u[2] = un[2];
         s = sn;
         moved = false;
+        improved = true;
       }
     }
+    if (!improved) break;  // u and s unchanged: the remaining iterations would repeat this one
   }
 }
 
