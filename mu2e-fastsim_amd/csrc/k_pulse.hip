@@ -180,6 +180,8 @@ __device__ void fit_gumbel(const crv_params &pp, const FitIn &f, const double p0
       u[1] = un[1];
       u[2] = un[2];
       s = sn;
+    } else if (lam >= 1e9) {
+      break;  // u, s and lam stay as they are: every remaining iteration would repeat this one
     }
     lam = fmin(fmax(better ? lam * 0.3 : lam * 10.0, 1e-9), 1e9);
   }
@@ -191,6 +193,7 @@ __device__ void fit_gumbel(const crv_params &pp, const FitIn &f, const double p0
 #pragma unroll
     for (int i = 0; i < 3; ++i) du[i] = (isfinite(du[i]) && !stopped) ? du[i] : 0.0;
     const double scales[4] = {1.0, 0.5, 0.25, 0.125};
+    bool improved = false;
     for (int k = 0; k < 4; ++k) {
       if (stopped || (du[0] == 0.0 && du[1] == 0.0 && du[2] == 0.0)) break;  // u + 0 cannot lower chi2
       double un[3] = {u[0] + scales[k] * du[0], u[1] + scales[k] * du[1], u[2] + scales[k] * du[2]};
@@ -201,8 +204,10 @@ __device__ void fit_gumbel(const crv_params &pp, const FitIn &f, const double p0
         u[2] = un[2];
         s = sn;
         du[0] = du[1] = du[2] = 0.0;
+        improved = true;
       }
     }
+    if (!improved) break;  // u and s unchanged: the remaining iterations would repeat this one
   }
   double step[3];
   bool ok = solve3(s.jtj, tiny, s.jtr, step);
