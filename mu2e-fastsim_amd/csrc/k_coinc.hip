@@ -1966,11 +1966,14 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
   Ev e{bar, sipm, seq};
   CRV_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int) * F_NFLAGS, s));
   kmark(c, 3, 0, s);
-  {  // size classes: wavefront kernels for events of <= 128 and <= 256 pulses, one lane per larger event
+  {  // size classes: wavefront kernels for events of <= 64, <= 128 and <= 256 pulses, one lane per larger event
     const ClSlots slots{cl_ev, cl_type, cl_pc, cl_sh, cl_d};
     const int grid = (int)std::min<int64_t>(nev, 16384);
+    k_coinc_wave<64><<<grid, 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff, mem,
+                                         (int64_t)0, c->flags);
+    CRV_HIP(c, hipGetLastError());
     k_coinc_wave<128><<<grid, 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff, mem,
-                                          (int64_t)0, c->flags);
+                                          (int64_t)65, c->flags);
     CRV_HIP(c, hipGetLastError());
     k_coinc_wave<256><<<std::min(grid, 4096), 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots,
                                                           cl_moff, mem, (int64_t)129, c->flags);
