@@ -279,6 +279,13 @@ __device__ __forceinline__ void feig(const float jtj[6], float &emin, float &ema
 }
 
 
+// a pulse slot of a pool run: the fit's results, read back whole by the pulse-table kernel
+struct alignas(16) Slot {
+  double tp, mu;  // pulse time without the DB offset, fitted Gumbel location (absolute, ns)
+  float pe, height, beta, chi2, ped, fa, fb;
+  uint32_t sf;  // seq | failed << 16 | zerondf << 17 | pedfromdb << 18
+};
+
 struct Out {
   // fired-pixel log: entries (time - tbase of the SiPM, overvoltage) in chunks of two (16 bytes); per group of 64
   // SiPMs in queue order, chunk c of the group's lane l at goff + 64 c + l (the lanes of a wavefront write and read
@@ -308,11 +315,9 @@ struct Out {
   int32_t *run_count;
   int64_t run_cap;
   int32_t *nrun;  // per SiPM
-  // pulse slots per pool run, and per final run the pulse count and the pool run
+  // pulse slots per pool run (one 48-byte record each), and per final run the pulse count and the pool run
   int pcap;
-  double *tp, *mu;  // pulse time without the DB offset, fitted Gumbel location (absolute, ns)
-  float *pe, *height, *beta, *chi2, *ped, *fa, *fb;
-  uint32_t *sf;  // seq | failed << 16 | zerondf << 17 | pedfromdb << 18
+  Slot *slot;
   int32_t *npr, *rmap;
   float *yscr;  // fit samples [sample][pool run]
 };
@@ -1057,16 +1062,18 @@ __device__ __forceinline__ void fit_run(const crv_params &p, const Tables &t, co
       if (np < o.pcap) {
         const int64_t q = r * o.pcap + np;
         const double t0 = (double)st * PERIODD;
-        o.tp[q] = t0 + (double)(failed ? t_peak : mu);
-        o.mu[q] = t0 + (double)mu;
-        o.pe[q] = (failed ? ye * (float)p.default_beta : A * bb) / (float)t.area[ch];
-        o.height[q] = failed ? yps : A / 2.71828183f;
-        o.beta[q] = failed ? (float)p.default_beta : bb;
-        o.chi2[q] = (failed || ndf <= 0) ? -1.f : s.chi2 / (float)max(ndf, 1);
-        o.ped[q] = ped;
-        o.fa[q] = A;
-        o.fb[q] = bb;
-        o.sf[q] = (uint32_t)seq | (fseq ? 1u << 16 : 0u) | (ndf <= 0 ? 1u << 17 : 0u) | (!und ? 1u << 18 : 0u);
+        Slot sv;
+        sv.tp = t0 + (double)(failed ? t_peak : mu);
+        sv.mu = t0 + (double)mu;
+        sv.pe = (failed ? ye * (float)p.default_beta : A * bb) / (float)t.area[ch];
+        sv.height = failed ? yps : A / 2.71828183f;
+        sv.beta = failed ? (float)p.default_beta : bb;
+        sv.chi2 = (failed || ndf <= 0) ? -1.f : s.chi2 / (float)max(ndf, 1);
+        sv.ped = ped;
+        sv.fa = A;
+        sv.fb = bb;
+        sv.sf = (uint32_t)seq | (fseq ? 1u << 16 : 0u) | (ndf <= 0 ? 1u << 17 : 0u) | (!und ? 1u << 18 : 0u);
+        o.slot[q] = sv;
       }
       ++np;
     }
@@ -1136,24 +1143,25 @@ __global__ void kb_pulses_out(crv_params p, Tables t, int64_t nruns, const uint6
   const double toff = p.use_time_offset_db ? t.toff[ch] * p.time_offset_scale : 0.0;
   const double hdb = t.height[ch];
   for (int64_t k = 0; k < n; ++k) {
-    const int64_t a = r * o.pcap + k, b = dst + k;
-    const uint32_t sf = o.sf[a];
-    const double tp = o.tp[a], beta = o.beta[a], height = o.height[a];
+    const int64_t b = dst + k;
+    const Slot sv = o.slot[r * o.pcap + k];
+    const uint32_t sf = sv.sf;
+    const double tp = sv.tp, beta = sv.beta, height = sv.height;
     P.wf[b] = fr;
     P.seq[b] = sf & 0xFFFFu;
     P.ev[b] = evkey;
     P.chan[b] = ch;
-    P.pe[b] = o.pe[a];
+    P.pe[b] = sv.pe;
     P.peph[b] = height / hdb;
     P.time[b] = tp + toff;
     P.height[b] = height;
     P.beta[b] = beta;
-    P.chi2[b] = o.chi2[a];
+    P.chi2[b] = sv.chi2;
     P.letime[b] = tp - p.le_time_factor * beta;
-    P.ped[b] = o.ped[a];
-    P.fit_a[b] = o.fa[a];
-    P.fit_mu[b] = o.mu[a];
-    P.fit_b[b] = o.fb[a];
+    P.ped[b] = sv.ped;
+    P.fit_a[b] = sv.fa;
+    P.fit_mu[b] = sv.mu;
+    P.fit_b[b] = sv.fb;
     P.failed[b] = (sf >> 16) & 1u;
     P.zerondf[b] = (sf >> 17) & 1u;
     P.pedfromdb[b] = (sf >> 18) & 1u;
@@ -1399,13 +1407,8 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   o.pcap = opt->pulse_slots > 0 ? opt->pulse_slots : PSLOTS;
   for (int attempt = 0;; ++attempt) {
     const int64_t ns = nruns * o.pcap + 8;
-    o.tp = (double *)ws(c, 520, ns * 8);
-    o.mu = (double *)ws(c, 521, ns * 8);
-    float **fp[] = {&o.pe, &o.height, &o.beta, &o.chi2, &o.ped, &o.fa, &o.fb};
-    bool okp = o.tp && o.mu;
-    for (int i = 0; i < 7; ++i) okp = okp && (*fp[i] = (float *)ws(c, 522 + i, ns * 4));
-    o.sf = (uint32_t *)ws(c, 529, ns * 4);
-    if (!okp || !o.sf) return ws_fail(c);
+    o.slot = (Slot *)ws(c, 520, ns * sizeof(Slot));
+    if (!o.slot) return ws_fail(c);
     kb_zero<<<1, 64, 0, s>>>(c->flags + F_PSLOT, 1);
     CRV_HIP(c, hipEventRecord(kev[4], s));
     if (nruns) kb_fit<<<fit_blocks, 64, 0, s>>>(p, c->t, g.task_key, roff, o, nruns, c->flags);
