@@ -679,6 +679,7 @@ struct Wave {
   int32_t rec;             // reserved record slot (-1: none)
   bool open;
   int32_t overlong;
+  uint32_t ab[4];          // ADC words of the record's current 8-sample chunk
 };
 
 struct Lane {
@@ -691,6 +692,23 @@ struct Lane {
   int W;
   const float *tmpl;
 };
+
+// ADC word a of sample J of the lane's record, gathered in registers and written eight samples (16 bytes) at a time;
+// a record's samples come in order from 0 (records whose length is not a multiple of 8: word by word)
+__device__ __forceinline__ void put_adc(const Out &o, const Lane &L, Wave &w, int J, uint32_t a) {
+  if ((L.W & 7) != 0) {
+    o.rec_adc[(int64_t)w.rec * L.W + J] = (uint16_t)a;
+    return;
+  }
+  const int k = (J & 7) >> 1;
+  const uint32_t bits = (a & 0xFFFFu) << ((J & 1) * 16);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) w.ab[q] |= q == k ? bits : 0u;
+  if ((J & 7) == 7) {
+    *(uint4 *)(o.rec_adc + (int64_t)w.rec * L.W + (J & ~7)) = make_uint4(w.ab[0], w.ab[1], w.ab[2], w.ab[3]);
+    w.ab[0] = w.ab[1] = w.ab[2] = w.ab[3] = 0u;
+  }
+}
 
 // finalise sample E: noise, ADC word, zero-suppression decision for sample E - 2
 __device__ __forceinline__ void wave_step(const crv_params &p, const Lane &L, const Out &o, Wave &w, bool at_close,
@@ -707,7 +725,7 @@ __device__ __forceinline__ void wave_step(const crv_params &p, const Lane &L, co
     const float u1 = (J & 1) ? u32f(w.nz.z) : u32f(w.nz.x), u2 = (J & 1) ? u32f(w.nz.w) : u32f(w.nz.y);
     v += (float)p.noise * sqrtf(-2.f * __logf(u1)) * __cosf(6.28318531f * u2);
   }
-  if (J < L.W && w.rec >= 0) o.rec_adc[(int64_t)w.rec * L.W + J] = (uint16_t)adc_f(p, valid ? v : 0.f);
+  if (J < L.W && w.rec >= 0) put_adc(o, L, w, J, (uint32_t)adc_f(p, valid ? v : 0.f));
   const bool ab = valid && v > (float)p.min_voltage;
   w.above = (w.above << 1) | (ab ? 1u : 0u);
   const int i = J - 2;  // zero suppression of sample i needs samples i - 2 .. i + 2
@@ -733,6 +751,7 @@ __device__ __forceinline__ void wave_open(const Lane &L, const Out &o, Wave &w, 
   w.nq = 0;
   w.used = 0;
   w.E = 0;
+  w.ab[0] = w.ab[1] = w.ab[2] = w.ab[3] = 0u;
   w.above = 0u;
   w.next_i = 0;
   w.nd = 0;
@@ -820,7 +839,7 @@ __device__ __forceinline__ void wave_close(const crv_params &p, const Tables &t,
   while (w.E < length + 2) wave_step(p, L, o, w, true, length);
   if (w.nd == 0 || w.rec < 0) return;  // no digis (the slot stays reserved), or the record pool is full
   const int pedw = adc_f(p, 0.f);
-  for (int J = w.E; J < W; ++J) o.rec_adc[(int64_t)w.rec * W + J] = (uint16_t)pedw;
+  for (int J = w.E; J < W; ++J) put_adc(o, L, w, J, (uint32_t)pedw);
   o.rec_label[w.rec] = L.evkey * CRV_CHAN_MULT + L.ch;
   o.rec_tick[w.rec] = w.kf;
   // reco waveforms: runs of continuing digis (start TDC = window tick + first sample), in start order
@@ -1108,9 +1127,15 @@ __global__ void kb_gcaps(const int64_t *__restrict__ nph, const int32_t *__restr
   const int64_t grp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (grp >= (L4 + 63) / 64) return;
   int64_t m = 0;
-  for (int64_t gq = grp * 64; gq < L4 && gq < grp * 64 + 64; ++gq) {
-    const int64_t v = nph[order ? order[gq] : gq];
-    m = max(m, v * 3 * factor / 4 + 16);  // fired pixels: about 42 % of the photons, binomial spread
+  // sorted longest first (by the count clamped to 32 bits): the group's first SiPM has its largest count
+  const int64_t first = order ? nph[order[grp * 64]] : -1;
+  if (first >= 0 && first < 0xFFFFFFFFll) {
+    m = first * 3 * factor / 4 + 16;
+  } else {
+    for (int64_t gq = grp * 64; gq < L4 && gq < grp * 64 + 64; ++gq) {
+      const int64_t v = nph[order ? order[gq] : gq];
+      m = max(m, v * 3 * factor / 4 + 16);  // fired pixels: about 42 % of the photons, binomial spread
+    }
   }
   m = (m + 1) & ~(int64_t)1;
   gcap[grp] = m;
