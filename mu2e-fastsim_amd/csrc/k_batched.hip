@@ -414,7 +414,10 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
       for (int k = a; k < b; ++k) {
         const int64_t gi = (int64_t)k * 4 + s;
         const int N = nph[gi];
-        if (nstr > 1) o.gR[gi] = 0;
+        if (nstr > 1) {
+          o.gR[gi] = 0;
+          if (N <= 0) o.gHead[gi] = INFINITY;  // an empty or exhausted stream's head is +inf
+        }
         if (N <= 0) continue;
         const int32_t i = perm[k];
         const uint32_t j = (uint32_t)(i - evfirst[i]);
@@ -446,12 +449,32 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
             kb = a;
             tph = head1;
           }
-        } else {
-          for (int k = a; k < b; ++k) {
-            const int64_t gi = (int64_t)k * 4 + s;
-            if (o.gR[gi] >= nph[gi]) continue;
-            const double h = o.gHead[gi];
-            if (kb < 0 || h < tph) {
+        } else {  // one load per stream, four streams' loads in flight at a time
+          tph = INFINITY;
+          int k = a;
+          for (; k + 4 <= b; k += 4) {
+            const double h0 = o.gHead[(int64_t)k * 4 + s], h1 = o.gHead[(int64_t)(k + 1) * 4 + s];
+            const double h2 = o.gHead[(int64_t)(k + 2) * 4 + s], h3 = o.gHead[(int64_t)(k + 3) * 4 + s];
+            if (h0 < tph) {
+              kb = k;
+              tph = h0;
+            }
+            if (h1 < tph) {
+              kb = k + 1;
+              tph = h1;
+            }
+            if (h2 < tph) {
+              kb = k + 2;
+              tph = h2;
+            }
+            if (h3 < tph) {
+              kb = k + 3;
+              tph = h3;
+            }
+          }
+          for (; k < b; ++k) {
+            const double h = o.gHead[(int64_t)k * 4 + s];
+            if (h < tph) {
               kb = k;
               tph = h;
             }
@@ -486,6 +509,8 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
             const float S = o.gS[gi] + (-__logf(u32f(x))) / (float)(N - rn);
             o.gS[gi] = S;
             o.gHead[gi] = o.gTb[gi] + (double)(tauf * S);
+          } else {
+            o.gHead[gi] = INFINITY;
           }
           o.gR[gi] = r + 1;
         }
