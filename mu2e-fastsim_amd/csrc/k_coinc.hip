@@ -1142,6 +1142,36 @@ __device__ void combos(const Tables &t, const BS &S, const int32_t *byl, double 
   }
 }
 
+// combination li of a cluster's by-layer lists in combos()'s order (layer subsets by size, then lexicographic):
+// its by-layer positions idx[0..nl); returns nl (0 past the last combination)
+__device__ __forceinline__ int combo_at(int li, const int lay_off[5], int nmin, int nmax, int idx[4]) {
+  for (int nl = 2; nl <= 4; ++nl) {
+    const bool need = nl == 2 ? nmin == 2 : nl == 3 ? (nmin <= 3 && 3 <= nmax) : nmax == 4;
+    if (!need) continue;
+    int Lc[4];
+    for (int i = 0; i < nl; ++i) Lc[i] = i;
+    for (;;) {
+      int P = 1;
+      for (int i = 0; i < nl; ++i) P *= lay_off[Lc[i] + 1] - lay_off[Lc[i]];
+      if (li < P) {
+        for (int i = nl - 1; i >= 0; --i) {
+          const int sz = lay_off[Lc[i] + 1] - lay_off[Lc[i]];
+          idx[i] = lay_off[Lc[i]] + li % sz;
+          li /= sz;
+        }
+        return nl;
+      }
+      li -= P;
+      int k = nl - 1;
+      while (k >= 0 && Lc[k] == 4 - nl + k) --k;
+      if (k < 0) break;
+      ++Lc[k];
+      for (int i = k + 1; i < nl; ++i) Lc[i] = Lc[i - 1] + 1;
+    }
+  }
+  return 0;
+}
+
 // atomic max of a double in global memory (no-op unless w is larger; NaN never stored)
 __device__ __forceinline__ void gmax_double(double *a, double w) {
   unsigned long long *q = (unsigned long long *)a;
@@ -1648,11 +1678,14 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
         __syncthreads();
         // the clusters go to cl2 / off2 as scratch here (the second clustering reuses them later)
         const int ncs = block_clusters(L, S, S.filt + f0, nfilt - f0, S.maxd, tmaxdt, false, 0.0, S.cl2, S.off2);
-        // coincidence check: one thread per small cluster, the block for the others
-        int nbig = 0;
+        // coincidence check: a small cluster's by-layer lists are built by one thread and its combinations spread
+        // over the block with the other small clusters' (one flat index space: S.sc holds each cluster's first
+        // combination, S.tmp its packed layer offsets); the block checks the others cluster by cluster
+        int nbig = 0, nflat = 0;
         for (int c0 = 0; c0 < ncs; c0 += BT) {
           const int c = c0 + tid;
           bool big = false;
+          int pc = 0;
           if (c < ncs) {
             const int o0 = S.off2[c], mn = S.off2[c + 1] - o0;
             S.clmc[c] = 1;  // 1: checked by the block
@@ -1663,8 +1696,9 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
                                              nmax, allk);
               if (allk) S.clmc[c] = -1;  // -1: every member kept with its initial distance
               else if (P <= BIG_COMBOS) {
-                S.clmc[c] = 0;  // 0: checked by this thread
-                combos(t, S, S.byl + o0, S.wq + o0, S.kept + o0, lay, nmin, nmax, 0, 1, false);
+                S.clmc[c] = 0;  // 0: checked in the flat combination space
+                pc = (int)P;
+                S.tmp[c] = lay[1] | lay[2] << 5 | lay[3] << 10 | lay[4] << 15 | nmin << 20 | nmax << 23;
               }
             }
             big = S.clmc[c] == 1;
@@ -1673,6 +1707,23 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
           const int r = bscan(L, big, tot);
           if (big) S.sk[nbig + r] = c;
           nbig += tot;
+          const int rp = bscan(L, pc, tot);
+          if (c < ncs) S.sc[c] = nflat + rp;
+          nflat += tot;
+        }
+        __syncthreads();
+        for (int f = tid; f < nflat; f += BT) {
+          int lo = 0, hi = ncs - 1;  // the last cluster whose first combination is <= f (it has combinations)
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S.sc[mid] <= f) lo = mid;
+            else hi = mid - 1;
+          }
+          const int c = lo, o0 = S.off2[c], pk = S.tmp[c];
+          const int lay[5] = {0, pk & 31, (pk >> 5) & 31, (pk >> 10) & 31, (pk >> 15) & 31};
+          int idx[4];
+          const int nl = combo_at(f - S.sc[c], lay, (pk >> 20) & 7, (pk >> 23) & 7, idx);
+          if (nl) combo_eval(t, S, S.byl + o0, S.wq + o0, S.kept + o0, idx, nl, true);
         }
         __syncthreads();
         for (int bi = 0; bi < nbig; ++bi) {
