@@ -713,6 +713,7 @@ constexpr int BT = 256;      // threads of a block-finder block
 constexpr int BH = 12288;    // LDS histogram bins: (side, layer, counter column) buckets, time bins, bars
 constexpr int BIG_COMBOS = 512;
 constexpr int SMALL_CLUSTER = 24;  // larger clusters are checked by the whole block
+static_assert(SMALL_CLUSTER < 64, "a small cluster's layer offsets are packed in 6 bits");
 constexpr int INF_PASS = 1 << 29;
 constexpr int CM = 1024, CB = 1024;  // clustering staged in LDS: positions, time bins
 constexpr int TN = 1536, TG = 384;   // coincidence check staged in LDS: hits, bar groups
@@ -1698,7 +1699,7 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
               else if (P <= BIG_COMBOS) {
                 S.clmc[c] = 0;  // 0: checked in the flat combination space
                 pc = (int)P;
-                S.tmp[c] = lay[1] | lay[2] << 5 | lay[3] << 10 | lay[4] << 15 | nmin << 20 | nmax << 23;
+                S.tmp[c] = lay[1] | lay[2] << 6 | lay[3] << 12 | lay[4] << 18 | nmin << 24 | nmax << 27;
               }
             }
             big = S.clmc[c] == 1;
@@ -1720,9 +1721,9 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
             else hi = mid - 1;
           }
           const int c = lo, o0 = S.off2[c], pk = S.tmp[c];
-          const int lay[5] = {0, pk & 31, (pk >> 5) & 31, (pk >> 10) & 31, (pk >> 15) & 31};
+          const int lay[5] = {0, pk & 63, (pk >> 6) & 63, (pk >> 12) & 63, (pk >> 18) & 63};
           int idx[4];
-          const int nl = combo_at(f - S.sc[c], lay, (pk >> 20) & 7, (pk >> 23) & 7, idx);
+          const int nl = combo_at(f - S.sc[c], lay, (pk >> 24) & 7, (pk >> 27) & 7, idx);
           if (nl) combo_eval(t, S, S.byl + o0, S.wq + o0, S.kept + o0, idx, nl, true);
         }
         __syncthreads();
