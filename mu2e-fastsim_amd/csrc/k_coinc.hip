@@ -53,8 +53,8 @@ __device__ bool cluster_emit(const crv_params &p, const Tables &t, const Acc &A,
     const int h = mem[i], bar = A.bar(h);
     const double pe = A.pe(h), tt = A.tm(h), x = A.x(h), y = A.y(h);
     tot = tot + pe;
-    mincp = fmin(mincp, cut_of(t, A.sec(h)).min_cluster_pes);
-    for (int ax = 0; ax < 3; ++ax) acc[ax] = acc[ax] + t.bpos[bar * 3 + ax] * pe;
+    mincp = fmin(mincp, A.mincp(t, h));
+    for (int ax = 0; ax < 3; ++ax) acc[ax] = acc[ax] + A.bp(t, h, ax) * pe;
     sxy = sxy + pe * x * y;
     sx = sx + pe * x;
     sy = sy + pe * y;
@@ -150,9 +150,12 @@ struct WS {
   int8_t snl[11];       // and sizes
 };
 
+// (bar position and the sector's min_cluster_pes by pulse: staged in LDS arrays that are free by then)
 template <int WMAX>
 struct WaveAcc {
   const WS<WMAX> &S;
+  __device__ double bp(const Tables &, int h, int ax) const { return ax == 0 ? S.wq[h] : ax == 1 ? S.cmaxd[h] : S.wm[h]; }
+  __device__ double mincp(const Tables &, int h) const { return S.cdt[h]; }
   __device__ double pe(int h) const { return S.pe[h]; }
   __device__ double tm(int h) const { return S.t[h]; }
   __device__ int bar(int h) const { return S.bar[h]; }
@@ -631,6 +634,16 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
         }
         __syncthreads();
       }
+      // stage the bar position (wq, cmaxd, wm) and min_cluster_pes (cdt) of each hit of this sector type (every
+      // cluster member is one; other pulses' sector and bar slots are not valid here): those arrays are done with
+      for (int i = lane; i < nh; i += 64) {
+        const int h = S.hl[i], b = S.bar[h];
+        S.wq[h] = t.bpos[b * 3];
+        S.cmaxd[h] = t.bpos[b * 3 + 1];
+        S.wm[h] = t.bpos[b * 3 + 2];
+        S.cdt[h] = cut_of(t, S.sec[h]).min_cluster_pes;
+      }
+      __syncthreads();
       // members of each second-level cluster (items, then its attachments, one per (bar, sipm)); min-PE test
       for (int base = 0; base < ncl2; base += 64) {
         const int c = base + lane;
@@ -640,7 +653,7 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
           for (int m = S.off2[c]; m < S.off2[c + 1]; ++m) {
             const int h = S.cl2[m];
             tot = tot + S.pe[h];
-            mincp = fmin(mincp, cut_of(t, S.sec[h]).min_cluster_pes);
+            mincp = fmin(mincp, S.cdt[h]);
           }
           int lastb = -1, lasts = -1;
           for (int q = 0; q < nchosen; ++q) {
@@ -651,7 +664,7 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
             lasts = S.sipm[h];
             ++mc;
             tot = tot + S.pe[h];
-            mincp = fmin(mincp, cut_of(t, S.sec[h]).min_cluster_pes);
+            mincp = fmin(mincp, S.cdt[h]);
           }
           S.cl_mc[c] = tot < mincp ? 0 : mc;
         }
@@ -1540,6 +1553,8 @@ struct BlockAcc {
   __device__ int sec(int h) const { return S.sec[h]; }
   __device__ double x(int h) const { return S.x[h]; }
   __device__ double y(int h) const { return S.y[h]; }
+  __device__ double bp(const Tables &t, int h, int ax) const { return t.bpos[S.bar[h] * 3 + ax]; }
+  __device__ double mincp(const Tables &t, int h) const { return cut_of(t, S.sec[h]).min_cluster_pes; }
 };
 
 __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int64_t nev, const int64_t *__restrict__ eoff,
