@@ -211,45 +211,20 @@ __device__ __forceinline__ void feval(const float *y, int64_t ys, int q0, int fs
   }
 }
 
-// (S + diag(add)) x = b in FP64 with partial pivoting; false on a zero pivot
+// (S + diag(add)) x = b in FP64 for the symmetric 3 x 3 S: cofactors and one reciprocal of the determinant (the FP64
+// chain eliminates with partial pivoting, as the reference's solve does; here the inputs carry FP32 rounding, far
+// above what the two methods' FP64 differences reach); false on a zero determinant
 __device__ __forceinline__ bool fsolve(const float jtj[6], const double add[3], const float bv[3], double x[3]) {
-  double a[3][3] = {{(double)jtj[0] + add[0], (double)jtj[1], (double)jtj[2]},
-                    {(double)jtj[1], (double)jtj[3] + add[1], (double)jtj[4]},
-                    {(double)jtj[2], (double)jtj[4], (double)jtj[5] + add[2]}};
-  double v[3] = {bv[0], bv[1], bv[2]};
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    int piv = c;
-    double best = fabs(a[c][c]);
-#pragma unroll
-    for (int r = c + 1; r < 3; ++r)
-      if (fabs(a[r][c]) > best) {
-        piv = r;
-        best = fabs(a[r][c]);
-      }
-    if (piv != c) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const double tmp = a[c][k];
-        a[c][k] = a[piv][k];
-        a[piv][k] = tmp;
-      }
-      const double tv = v[c];
-      v[c] = v[piv];
-      v[piv] = tv;
-    }
-    if (a[c][c] == 0.0) return false;
-#pragma unroll
-    for (int r = c + 1; r < 3; ++r) {
-      const double l = a[r][c] / a[c][c];
-#pragma unroll
-      for (int cc = c + 1; cc < 3; ++cc) a[r][cc] -= l * a[c][cc];
-      v[r] -= l * v[c];
-    }
-  }
-  x[2] = v[2] / a[2][2];
-  x[1] = (v[1] - a[1][2] * x[2]) / a[1][1];
-  x[0] = (v[0] - a[0][1] * x[1] - a[0][2] * x[2]) / a[0][0];
+  const double a00 = (double)jtj[0] + add[0], a01 = jtj[1], a02 = jtj[2], a11 = (double)jtj[3] + add[1];
+  const double a12 = jtj[4], a22 = (double)jtj[5] + add[2];
+  const double c00 = a11 * a22 - a12 * a12, c01 = a02 * a12 - a01 * a22, c02 = a01 * a12 - a02 * a11;
+  const double det = a00 * c00 + a01 * c01 + a02 * c02;
+  if (det == 0.0) return false;
+  const double c11 = a00 * a22 - a02 * a02, c12 = a01 * a02 - a00 * a12, c22 = a00 * a11 - a01 * a01;
+  const double id = 1.0 / det, b0 = bv[0], b1 = bv[1], b2 = bv[2];
+  x[0] = (c00 * b0 + c01 * b1 + c02 * b2) * id;
+  x[1] = (c01 * b0 + c11 * b1 + c12 * b2) * id;
+  x[2] = (c02 * b0 + c12 * b1 + c22 * b2) * id;
   return true;
 }
 
