@@ -28,6 +28,10 @@ struct Tables {
   int32_t map_uniform = 0;  // every map bin is 1: the CDF lookup is floor(x)
   double cdf_total = 0.0;
   const uint32_t *dead = nullptr;  // CRV_DEAD_WORDS words, bit pix = x * 41 + y
+  // the dead mask's nonzero words when they lie within 8 consecutive words (dead_w0 >= 0; else -1): kernels test a
+  // pixel against these in registers instead of loading the mask
+  int32_t dead_w0 = -1;
+  uint32_t dead_win[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // finder geometry
   int32_t nbars = 0, nsectors = 0, ntypes = 0;
   int32_t ncit = 0;  // 1 + the largest counter_in_type

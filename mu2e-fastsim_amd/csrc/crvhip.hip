@@ -327,6 +327,15 @@ int crv_upload_photon_map(crv_ctx *ctx, const double *pmap, int32_t nx, int32_t 
   int rc;
   if ((rc = upload(c, cdf.data(), nx * ny, &c->t.cdf)) || (rc = upload(c, dead.data(), CRV_DEAD_WORDS, &c->t.dead)))
     return rc;
+  int w_lo = CRV_DEAD_WORDS, w_hi = -1;
+  for (int w = 0; w < CRV_DEAD_WORDS; ++w)
+    if (dead[w]) {
+      w_lo = std::min(w_lo, w);
+      w_hi = w;
+    }
+  if (w_hi < 0) w_lo = 0;
+  c->t.dead_w0 = w_hi - w_lo < 8 ? w_lo : -1;
+  for (int k = 0; k < 8; ++k) c->t.dead_win[k] = c->t.dead_w0 >= 0 && w_lo + k < CRV_DEAD_WORDS ? dead[w_lo + k] : 0u;
   c->t.ncdf = nx * ny;
   c->t.map_ny = ny;
   c->t.map_uniform = 1;

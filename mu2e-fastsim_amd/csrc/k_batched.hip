@@ -320,7 +320,14 @@ __device__ __forceinline__ bool bget(const uint32_t *bm, int lane, int pix) {
 __device__ __forceinline__ void bset(uint32_t *bm, int lane, int pix) { bm[(pix >> 5) * 64 + lane] |= 1u << (pix & 31); }
 
 // a pixel whose bit is set in a bitmap that started as the dead mask: dead (lost) or fired before in this SiPM
-__device__ __forceinline__ bool dead_px(const Tables &t, int pix) { return (t.dead[pix >> 5] >> (pix & 31)) & 1u; }
+__device__ __forceinline__ bool dead_px(const Tables &t, int pix) {
+  if (t.dead_w0 < 0) return (t.dead[pix >> 5] >> (pix & 31)) & 1u;
+  const int k = (pix >> 5) - t.dead_w0;  // the mask's nonzero words are in registers: no load
+  uint32_t w = 0u;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w = k == i ? t.dead_win[i] : w;
+  return (w >> (pix & 31)) & 1u;
+}
 
 // append a firing to the SiPM's log and to the per-pixel last-fire times
 // (an even entry waits in registers, pt / pv, and goes out with the next one as one 16-byte chunk)
