@@ -15,6 +15,18 @@
 namespace crv {
 
 // Read-only device tables uploaded once per context (conditions, template, photon map, finder geometry).
+// what the coincidence finders read of a bar, with its sector's fields, in one 64-byte record (one request per hit
+// instead of a chain of dependent loads bar -> sector -> sector fields)
+struct BarInfo {
+  double x, y;      // bar centre (the finder's x and y)
+  double ps0, ps1;  // its sector's side-0 and side-1 positions
+  double imd;       // its sector's initial_max_distance
+  int32_t sec;      // sector
+  int32_t type;     // sector type when the sector has cuts, else INT32_MIN (matches no type)
+  int32_t layer, cit;
+  int32_t pad_[2];
+};
+
 struct Tables {
   int32_t n_channels = 0;
   const int32_t *status = nullptr, *roc = nullptr, *feb = nullptr;
@@ -41,6 +53,7 @@ struct Tables {
   const double *ps0 = nullptr, *ps1 = nullptr;
   const uint8_t *sside0 = nullptr, *sside1 = nullptr, *hascuts = nullptr;
   const double *cuts = nullptr;  // [nsectors * 10] SectorCuts fields in declaration order
+  const BarInfo *binfo = nullptr;  // [nbars]
   // per sector type (sorted type ids): p0, p1, half, max_dt, side0, side1
   const int32_t *type_id = nullptr;
   const double *tp0 = nullptr, *tp1 = nullptr, *thalf = nullptr, *tmaxdt = nullptr;

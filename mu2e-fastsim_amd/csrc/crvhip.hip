@@ -406,6 +406,13 @@ int crv_upload_geometry(crv_ctx *ctx, int32_t nbars, const int32_t *sector, cons
       (rc = upload(c, ts0.data(), (int64_t)ts0.size(), &c->t.tside0)) ||
       (rc = upload(c, ts1.data(), (int64_t)ts1.size(), &c->t.tside1)))
     return rc;
+  std::vector<BarInfo> bi(nbars);
+  for (int i = 0; i < nbars; ++i) {
+    const int sc = sector[i];
+    bi[i] = BarInfo{x[i], y[i], pos_side0[sc], pos_side1[sc], cuts[sc * 10 + 8], sc,
+                    has_cuts[sc] ? sector_type[sc] : INT32_MIN, layer[i], counter_in_type[i], {0, 0}};
+  }
+  if ((rc = upload(c, bi.data(), nbars, &c->t.binfo))) return rc;
   c->t.nbars = nbars;
   c->t.nsectors = nsectors;
   c->t.ntypes = (int32_t)tid.size();

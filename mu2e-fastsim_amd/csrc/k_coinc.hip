@@ -488,22 +488,22 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
           if (bar < 0 || bar >= t.nbars || sp < 0 || sp > 3 || bar * 4 + 3 >= t.n_channels) {
             atomicAdd(&flags[F_BADCHAN], 1);
           } else {
-            const int sec = t.sector[bar];
-            if (t.hascuts[sec] && t.stype[sec] == type) {
+            const BarInfo bi = t.binfo[bar];
+            if (bi.type == type) {
               double tt = ptime[a + k];
-              if (sp % 2 == 0) tt += (t.ps0[sec] - P0) / p.fiber_signal_speed;
-              else tt += (P1 - t.ps1[sec]) / p.fiber_signal_speed;
+              if (sp % 2 == 0) tt += (bi.ps0 - P0) / p.fiber_signal_speed;
+              else tt += (P1 - bi.ps1) / p.fiber_signal_speed;
               double pe = ppe[a + k];
               const int other = bar * 4 + sp + (sp < 2 ? 2 : -2);
               if (t.status[other] & p.compensate_mask) pe *= 2.0;
               S.t[k] = tt;
               S.pe[k] = pe;
-              S.maxd[k] = cut_of(t, sec).initial_max_distance;
-              S.x[k] = t.bx[bar];
-              S.y[k] = t.by[bar];
-              S.sec[k] = (int16_t)sec;
-              S.layer[k] = (uint8_t)t.layer[bar];
-              S.cit[k] = (int16_t)t.cit[bar];
+              S.maxd[k] = bi.imd;
+              S.x[k] = bi.x;
+              S.y[k] = bi.y;
+              S.sec[k] = (int16_t)bi.sec;
+              S.layer[k] = (uint8_t)bi.layer;
+              S.cit[k] = (int16_t)bi.cit;
               S.side[k] = (uint8_t)(sp % 2);
               hit = true;
             }
@@ -1606,22 +1606,22 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
           if (bar < 0 || bar >= t.nbars || sp < 0 || sp > 3 || bar * 4 + 3 >= t.n_channels) {
             atomicAdd(&flags[F_BADCHAN], 1);
           } else {
-            const int sec = t.sector[bar];
-            if (t.hascuts[sec] && t.stype[sec] == type) {
+            const BarInfo bi = t.binfo[bar];
+            if (bi.type == type) {
               double tt = ptime[a + k];
-              if (sp % 2 == 0) tt += (t.ps0[sec] - P0) / p.fiber_signal_speed;
-              else tt += (P1 - t.ps1[sec]) / p.fiber_signal_speed;
+              if (sp % 2 == 0) tt += (bi.ps0 - P0) / p.fiber_signal_speed;
+              else tt += (P1 - bi.ps1) / p.fiber_signal_speed;
               double pe = ppe[a + k];
               const int other = bar * 4 + sp + (sp < 2 ? 2 : -2);
               if (t.status[other] & p.compensate_mask) pe *= 2.0;
               S.t[k] = tt;
               S.pe[k] = pe;
-              S.maxd[k] = cut_of(t, sec).initial_max_distance;
-              S.x[k] = t.bx[bar];
-              S.y[k] = t.by[bar];
-              S.sec[k] = sec;
-              S.layer[k] = (uint8_t)t.layer[bar];
-              S.cit[k] = t.cit[bar];
+              S.maxd[k] = bi.imd;
+              S.x[k] = bi.x;
+              S.y[k] = bi.y;
+              S.sec[k] = bi.sec;
+              S.layer[k] = (uint8_t)bi.layer;
+              S.cit[k] = bi.cit;
               S.side[k] = (uint8_t)(sp % 2);
               hit = true;
             }
