@@ -239,6 +239,12 @@ int crv_ctx_create(int device, crv_ctx **out) {
   }
   for (auto &e : ctx->c.ev) (void)hipEventCreate(&e);
   for (auto &e : ctx->c.kev) (void)hipEventCreate(&e);
+  for (auto &e : ctx->c.fork) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+  for (auto &q : ctx->c.aux)
+    if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) {
+      (void)crv_ctx_destroy(ctx);
+      return set_error(nullptr, CRV_E_HIP, "hipStreamCreate failed");
+    }
   *out = ctx;
   return CRV_OK;
 }
@@ -253,6 +259,9 @@ int crv_ctx_destroy(crv_ctx *ctx) {
   if (ctx->c.flags) (void)hipFree(ctx->c.flags);
   for (auto &e : ctx->c.ev) (void)hipEventDestroy(e);
   for (auto &e : ctx->c.kev) (void)hipEventDestroy(e);
+  for (auto &e : ctx->c.fork) (void)hipEventDestroy(e);
+  for (auto &q : ctx->c.aux)
+    if (q) (void)hipStreamDestroy(q);
   delete ctx;
   return CRV_OK;
 }

@@ -2033,20 +2033,13 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
   Ev e{bar, sipm, seq};
   CRV_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int) * F_NFLAGS, s));
   kmark(c, 3, 0, s);
-  {  // size classes: wavefront kernels for events of <= 64, <= 128 and <= 256 pulses, one lane per larger event
-    const ClSlots slots{cl_ev, cl_type, cl_pc, cl_sh, cl_d};
-    const int grid = (int)std::min<int64_t>(nev, 16384);
-    k_coinc_wave<64><<<grid, 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff, mem,
-                                         (int64_t)0, c->flags);
-    CRV_HIP(c, hipGetLastError());
-    k_coinc_wave<128><<<grid, 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff, mem,
-                                          (int64_t)65, c->flags);
-    CRV_HIP(c, hipGetLastError());
-    k_coinc_wave<256><<<std::min(grid, 4096), 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots,
-                                                          cl_moff, mem, (int64_t)129, c->flags);
-    CRV_HIP(c, hipGetLastError());
+  {  // size classes: wavefront kernels for events of <= 64, <= 128 and <= 256 pulses, a block per larger event.
+    // Each event belongs to one class, so the 128-pulse class and the 256-pulse class + block finder run on two side
+    // streams beside the 64-pulse class and are joined back to `s` before anything reads their outputs.
     if (c->t.nbars + 1 > BH || 8 * c->t.ncit + 1 > BH)
       return set_error(c, CRV_E_ARG, "clusters: geometry too large for the block finder's bar / column buckets");
+    const ClSlots slots{cl_ev, cl_type, cl_pc, cl_sh, cl_d};
+    const int grid = (int)std::min<int64_t>(nev, 16384);
     BS b{};
     double **bd[] = {&b.t, &b.pe, &b.x, &b.y, &b.maxd, &b.wm, &b.cmaxd, &b.wq, &b.cdt, &b.px, &b.pt, &b.pmd};
     for (int i = 0; i < 12; ++i) *bd[i] = (double *)ws(c, i < 9 ? 130 + i : 171 + i, ns * 8);
@@ -2058,9 +2051,31 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     for (auto q : bd) if (!*q) return ws_fail(c);
     for (auto q : bi) if (!*q) return ws_fail(c);
     for (auto q : bu) if (!*q) return ws_fail(c);
-    k_coinc_block<<<(int)std::min<int64_t>(nev, 2048), BT, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, b, ncl,
-                                                                  nmem, slots, cl_moff, mem, (int64_t)257, c->flags);
-    CRV_HIP(c, hipGetLastError());
+    CRV_HIP(c, hipEventRecord(c->fork[0], s));
+    CRV_HIP(c, hipStreamWaitEvent(c->aux[0], c->fork[0], 0));
+    CRV_HIP(c, hipStreamWaitEvent(c->aux[1], c->fork[0], 0));
+    k_coinc_wave<64><<<grid, 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff, mem,
+                                         (int64_t)0, c->flags);
+    const hipError_t e0 = hipGetLastError();
+    k_coinc_wave<128><<<grid, 64, 0, c->aux[0]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff,
+                                                  mem, (int64_t)65, c->flags);
+    const hipError_t e1 = hipGetLastError();
+    k_coinc_wave<256><<<std::min(grid, 4096), 64, 0, c->aux[1]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem,
+                                                                  slots, cl_moff, mem, (int64_t)129, c->flags);
+    const hipError_t e2 = hipGetLastError();
+    k_coinc_block<<<(int)std::min<int64_t>(nev, 2048), BT, 0, c->aux[1]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, b,
+                                                                           ncl, nmem, slots, cl_moff, mem,
+                                                                           (int64_t)257, c->flags);
+    const hipError_t e3 = hipGetLastError();
+    // join before any error return, so that nothing of this call is left running on a side stream
+    CRV_HIP(c, hipEventRecord(c->fork[1], c->aux[0]));
+    CRV_HIP(c, hipEventRecord(c->fork[2], c->aux[1]));
+    CRV_HIP(c, hipStreamWaitEvent(s, c->fork[1], 0));
+    CRV_HIP(c, hipStreamWaitEvent(s, c->fork[2], 0));
+    CRV_HIP(c, e0);
+    CRV_HIP(c, e1);
+    CRV_HIP(c, e2);
+    CRV_HIP(c, e3);
   }
   kmark(c, 3, 1, s);
   if ((rc = scan_i32_to_i64(c, ncl, clo, nev, 124, s))) return rc;
