@@ -259,6 +259,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--pipeline", type=int, default=0,
+                    help="FP32 chain: at least this many SiPM passes, waveforms of one beside photons of the next")
     ap.add_argument("--sort", type=int, default=1)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep --tile / --sort as given (with --tile 0 the work-queue tile and order are autotuned)")
@@ -313,7 +315,7 @@ def main():
     else:
         ds = DeviceSteps(S["evkey"], S["counter"], S["edep"], S["z"], S["t"])
         evkey = S["evkey"]
-    batch = {"grid": args.grid, "tile": args.tile, "sort_tasks": args.sort}
+    batch = {"grid": args.grid, "tile": args.tile, "sort_tasks": args.sort, "pipeline": args.pipeline}
     autotune = args.tile == 0 and not args.no_autotune
     v32, ms32, avg32, cnt32, ex32 = measure("fp32", ds, evkey, args.events, args.seed + 1, args.steps, args.warmup,
                                             batch, args.chunk, autotune)

@@ -209,6 +209,9 @@ typedef struct crv_batch_opts {
   const int32_t *record_tick;
   int64_t n_records, n_waveforms;
   int32_t overlong, capped, pulse_slots_used, log_passes;
+  int32_t pipeline; /* >= 2: the SiPMs run in at least this many passes, the waveform kernel of a pass beside the
+                     * photon kernel of the next on a side stream (two fired-pixel log buffers); 0: one pass */
+  int32_t _pad;
 } crv_batch_opts;
 
 /* The whole chain as the FP32 batched pipeline: steps -> pulses in three persistent work-queue kernels (photons and

@@ -89,7 +89,8 @@ class Timing(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [("grid", i32), ("tile", i32), ("pulse_slots", i32), ("sort_tasks", i32), ("log_budget", i64),
                 ("record_adc", P), ("record_label", P), ("record_tick", P), ("n_records", i64), ("n_waveforms", i64),
-                ("overlong", i32), ("capped", i32), ("pulse_slots_used", i32), ("log_passes", i32)]
+                ("overlong", i32), ("capped", i32), ("pulse_slots_used", i32), ("log_passes", i32),
+                ("pipeline", i32), ("_pad", i32)]
 
 
 EXPORTS = ("crv_ctx_create", "crv_ctx_destroy", "crv_set_params", "crv_last_error", "crv_upload_conditions",

@@ -383,7 +383,7 @@ def run_chain(steps, cond, cfg, key=0, geom=None, batch=None, timers=None, recor
     cfg.precision "fp64": the plain FP64 chain (one launch per stage, equal to the oracle bit for bit); "fp32": the
     batched FP32 chain (three fused kernels, FP64 only for absolute times and the fit's 3 x 3 solve), equal to the
     FP64 chain statistically. `batch` holds crv_batch_opts fields for the FP32 chain (grid, tile, pulse_slots,
-    sort_tasks, log_budget). Returns {"pulses": {...}, "clusters": cluster table, "timing": {...}} with pulses in (event, channel,
+    sort_tasks, log_budget, pipeline). Returns {"pulses": {...}, "clusters": cluster table, "timing": {...}} with pulses in (event, channel,
     time) order and `wf` the pulse's reco-waveform number; with records=True (FP32 chain) also "records", the ADC
     records of the clusters with digis sorted by (label, FEB tick)."""
     import ctypes

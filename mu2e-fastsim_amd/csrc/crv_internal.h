@@ -84,6 +84,7 @@ struct Ctx {
   crv_digis last_digis{};
   hipEvent_t kev[8];             // start / end of the main kernel of each stage (sipm, digi, pulse, coinc)
   hipStream_t aux[2] = {nullptr, nullptr};  // side streams of the finder's size classes (default priority)
+  std::vector<hipEvent_t> pev;               // per-pass events of the pipelined batched chain (grown on demand)
   hipEvent_t fork[3];                       // fork / join events of the side streams
   bool kev_on = false;
   int64_t b_rec_cap = 0, b_run_cap = 0;  // pool sizes of the batched chain (grown on demand)

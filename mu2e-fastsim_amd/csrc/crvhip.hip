@@ -260,6 +260,7 @@ int crv_ctx_destroy(crv_ctx *ctx) {
   for (auto &e : ctx->c.ev) (void)hipEventDestroy(e);
   for (auto &e : ctx->c.kev) (void)hipEventDestroy(e);
   for (auto &e : ctx->c.fork) (void)hipEventDestroy(e);
+  for (auto &e : ctx->c.pev) (void)hipEventDestroy(e);
   for (auto &q : ctx->c.aux)
     if (q) (void)hipStreamDestroy(q);
   delete ctx;

@@ -1354,11 +1354,13 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     if ((rc = read_i64(c, goff + ngrp, &nlog, s))) return rc;
     int64_t log_chunks = nlog;  // goff and nlog count 16-byte chunks
     cuts.assign({0, ngrp});
-    if ((size_t)nlog * 16 > budget) {
+    const bool pipe = opt->pipeline >= 2 && ngrp >= 2;  // two log buffers, each pass within half the budget
+    if (pipe || (size_t)nlog * 16 > budget) {
       hoff.resize(ngrp + 1);
       CRV_HIP(c, hipMemcpyAsync(hoff.data(), goff, (ngrp + 1) * 8, hipMemcpyDeviceToHost, s));
       CRV_HIP(c, hipStreamSynchronize(s));
-      const int64_t lim = (int64_t)(budget / 16);
+      int64_t lim = (int64_t)(budget / 16);
+      if (pipe) lim = std::min<int64_t>(lim / 2, (nlog + opt->pipeline - 1) / opt->pipeline);
       cuts.assign(1, 0);
       log_chunks = 0;
       for (int64_t g0 = 0; g0 < ngrp;) {
@@ -1378,7 +1380,82 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     o.flog = (float4 *)ws(c, 538, log_chunks * 16 + 16);
     o.tbase = (double *)ws(c, 519, L4 * 8 + 8);
     if (!o.flog || !o.tbase) return ws_fail(c);
+    float4 *flog2 = nullptr;
+    if (pipe && npass > 1) {
+      flog2 = (float4 *)ws(c, 547, log_chunks * 16 + 16);
+      if (!flog2) return ws_fail(c);
+    }
     bool grow_log = false;
+    if (flog2) {  // pipelined passes: kb_sipm of pass i + 1 on `s` beside kb_wave of pass i on a side stream
+      const hipStream_t s2 = c->aux[0];
+      while ((int)c->pev.size() < 6 * npass + 2) {
+        hipEvent_t e;
+        CRV_HIP(c, hipEventCreate(&e));
+        c->pev.push_back(e);
+      }
+      hipEvent_t *S = c->pev.data(), *Wv = S + npass, *t0s = S + 2 * npass, *t1s = S + 3 * npass,
+                 *t0w = S + 4 * npass, *t1w = S + 5 * npass, fk = S[6 * npass], jn = S[6 * npass + 1];
+      for (int pa = 0;; ++pa) {
+        if ((rc = pools())) return rc;
+        kb_zero<<<1, 64, 0, s>>>(counters + 2, 2);
+        kb_zero<<<1, 64, 0, s>>>(c->flags + F_OVERFLOW, 1);
+        CRV_HIP(c, hipEventRecord(fk, s));
+        CRV_HIP(c, hipStreamWaitEvent(s2, fk, 0));
+        for (int pi = 0; pi < npass; ++pi) {
+          const int64_t q0 = cuts[pi] * 64, q1 = std::min<int64_t>(cuts[pi + 1] * 64, L4);
+          Out op = o;
+          op.flog = (pi & 1) ? flog2 : o.flog;
+          op.log0 = hoff[cuts[pi]];
+          if (pi >= 2) CRV_HIP(c, hipStreamWaitEvent(s, Wv[pi - 2], 0));  // its log buffer has been read
+          kb_zero<<<1, 64, 0, s>>>(counters, 1);
+          CRV_HIP(c, hipEventRecord(t0s[pi], s));
+          kb_sipm<<<grid_s, 64, 0, s>>>(p, c->t, c->d, st, g.task_key, g.task_off, g.perm, g.evfirst, nph, op,
+                                        Queue{counters, tile, q1, qorder, q0}, nfired, c->flags);
+          CRV_HIP(c, hipGetLastError());
+          CRV_HIP(c, hipEventRecord(t1s[pi], s));
+          CRV_HIP(c, hipEventRecord(S[pi], s));
+          CRV_HIP(c, hipStreamWaitEvent(s2, S[pi], 0));
+          kb_zero<<<1, 64, 0, s2>>>(counters + 1, 1);
+          CRV_HIP(c, hipEventRecord(t0w[pi], s2));
+          kb_wave<<<grid_w, 64, 0, s2>>>(p, c->t, c->d, g.task_key, nfired, op, Queue{counters + 1, tile, q1, qorder, q0},
+                                         c->flags);
+          CRV_HIP(c, hipGetLastError());
+          CRV_HIP(c, hipEventRecord(t1w[pi], s2));
+          CRV_HIP(c, hipEventRecord(Wv[pi], s2));
+        }
+        CRV_HIP(c, hipEventRecord(jn, s2));
+        CRV_HIP(c, hipStreamWaitEvent(s, jn, 0));
+        CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
+        CRV_HIP(c, hipMemcpyAsync(cnt, counters + 2, 8, hipMemcpyDeviceToHost, s));
+        CRV_HIP(c, hipStreamSynchronize(s));
+        ms_sipm = ms_wave = 0.f;
+        for (int pi = 0; pi < npass; ++pi) {
+          float e1 = 0.f, e2 = 0.f;
+          (void)hipEventElapsedTime(&e1, t0s[pi], t1s[pi]);
+          (void)hipEventElapsedTime(&e2, t0w[pi], t1w[pi]);
+          ms_sipm += e1;
+          ms_wave += e2;
+        }
+        if (hf[F_CASCADE])
+          return set_error(c, CRV_E_OVERFLOW,
+                           "batched chain: cross-talk cascade wider than 16 pixels or deeper than 31 generations");
+        if (hf[F_COUNTER])
+          return set_error(c, CRV_E_OVERFLOW, "batched chain: more than 65535 photoelectrons from one step");
+        if (hf[F_OVERFLOW]) {
+          grow_log = true;
+          break;
+        }
+        if (cnt[0] <= o.rec_cap && cnt[1] <= o.run_cap) break;
+        if (pa >= 3) return set_error(c, CRV_E_OVERFLOW, "batched chain: record or run pool still too small");
+        c->b_rec_cap = std::max<int64_t>(o.rec_cap, (int64_t)cnt[0] + cnt[0] / 4 + 1024);
+        c->b_run_cap = std::max<int64_t>(o.run_cap, (int64_t)cnt[1] + cnt[1] / 4 + 1024);
+      }
+      if (!grow_log) {
+        c->b_log_factor = factor;
+        break;
+      }
+      continue;
+    }
     for (int pa = 0;; ++pa) {
       if ((rc = pools())) return rc;
       kb_zero<<<1, 64, 0, s>>>(counters + 2, 2);

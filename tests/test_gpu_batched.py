@@ -123,6 +123,18 @@ def test_log_passes_do_not_change_results(cosmic, sort_tasks):
     assert r["timing"]["n_records"] == r32["timing"]["n_records"]
 
 
+@pytest.mark.parametrize("pipeline", [2, 5])
+def test_pipelined_passes_do_not_change_results(cosmic, pipeline):
+    """Pipelined passes (the waveform kernel of a pass on a side stream beside the photon kernel of the next, two log
+    buffers) give the tables of the one-pass run."""
+    S, cond, G, c64, c32, r64, r32 = cosmic
+    r = CH.run_chain(S, cond, c32, key=7, geom=G, batch={"pipeline": pipeline}, records=True)
+    assert r["timing"]["log_passes"] >= pipeline
+    for tab in ("pulses", "clusters", "records"):
+        for k, v in r32[tab].items():
+            np.testing.assert_array_equal(r[tab][k], v, err_msg=f"{tab} {k}")
+
+
 def test_log_budget_below_one_group_is_an_error(cosmic):
     from crvhip import _lib as L
     S, cond, G, c64, c32, r64, r32 = cosmic
