@@ -1749,15 +1749,19 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
           if (rc == 2 && tid == 0) S.clmc[c] = -1;
           __syncthreads();
         }
-        // kept hits per cluster in pulse order, appended in cluster order with their widened distance
+        // kept hits per cluster in pulse order, appended in cluster order with their widened distance; the hits of
+        // the small clusters are ranked one per thread over all of them (S.tmp: each cluster's first output slot,
+        // S.sc: its first hit in that flat list)
+        int nfh = 0;
         for (int c0 = 0; c0 < ncs; c0 += BT) {
           const int c = c0 + tid;
-          int cnt = 0;
+          int cnt = 0, fh = 0;
           if (c < ncs) {
             const int o0 = S.off2[c], mn = S.off2[c + 1] - o0;
             if (S.clmc[c] < 0) cnt = mn;
             else
               for (int q = 0; q < mn; ++q) cnt += S.kept[o0 + q];
+            if (S.clmc[c] == 0 && cnt) fh = mn;
           }
           int tot;
           const int r = bscan(L, cnt, tot);
@@ -1768,20 +1772,30 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
                 S.coinc[w0 + i] = S.cl2[o0 + i];
                 S.cmaxd[w0 + i] = S.maxd[S.cl2[o0 + i]];
               }
-            } else if (S.clmc[c] > 0) {
+            } else {
               S.tmp[c] = w0;
-            } else {  // rank by pulse among the kept hits (independent loads, no shifting)
-              for (int q = 0; q < mn; ++q) {
-                if (!S.kept[o0 + q]) continue;
-                const int h = S.byl[o0 + q];
-                int r = 0;
-                for (int q2 = 0; q2 < mn; ++q2) r += S.kept[o0 + q2] && S.byl[o0 + q2] < h;
-                S.coinc[w0 + r] = h;
-                S.cmaxd[w0 + r] = S.wq[o0 + q];
-              }
             }
           }
           nco += tot;
+          const int rf = bscan(L, fh, tot);
+          if (c < ncs) S.sc[c] = nfh + rf;
+          nfh += tot;
+        }
+        __syncthreads();
+        for (int f = tid; f < nfh; f += BT) {  // rank by pulse among the cluster's kept hits
+          int lo = 0, hi = ncs - 1;  // the last cluster whose first flat hit is <= f (it has hits in the list)
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S.sc[mid] <= f) lo = mid;
+            else hi = mid - 1;
+          }
+          const int c = lo, o0 = S.off2[c], mn = S.off2[c + 1] - o0, q = f - S.sc[c];
+          if (!S.kept[o0 + q]) continue;
+          const int h = S.byl[o0 + q];
+          int rk = 0;
+          for (int q2 = 0; q2 < mn; ++q2) rk += S.kept[o0 + q2] && S.byl[o0 + q2] < h;
+          S.coinc[S.tmp[c] + rk] = h;
+          S.cmaxd[S.tmp[c] + rk] = S.wq[o0 + q];
         }
         __syncthreads();
         for (int bi = 0; bi < nbig; ++bi) {  // kept hits of the block-checked clusters: rank by pulse
