@@ -1355,15 +1355,17 @@ __device__ void tup_search(const A &G, const int lgo[5], int nmin, int nmax, dou
     int Lc[4];
     for (int i = 0; i < nl; ++i) Lc[i] = i;
     for (;;) {
-      const int n1 = lgo[Lc[1] + 1] - lgo[Lc[1]];
-      const int P01 = (lgo[Lc[0] + 1] - lgo[Lc[0]]) * n1;
+      // one work item per choice of the first three groups (two for a 2-layer subset), spread over the block
+      const int n1 = lgo[Lc[1] + 1] - lgo[Lc[1]], n2 = nl >= 3 ? lgo[Lc[2] + 1] - lgo[Lc[2]] : 1;
+      const int P = (lgo[Lc[0] + 1] - lgo[Lc[0]]) * n1 * n2;
       bool empty = false;
       for (int i = 2; i < nl; ++i) empty |= lgo[Lc[i] + 1] == lgo[Lc[i]];
-      for (int ci = empty ? P01 : (int)threadIdx.x; ci < P01; ci += BT) {
+      for (int ci = empty ? P : (int)threadIdx.x; ci < P; ci += BT) {
         int g[4];
         double sl[3];
-        g[0] = lgo[Lc[0]] + ci / n1;
-        g[1] = lgo[Lc[1]] + ci % n1;
+        const int rest = ci % (n1 * n2);
+        g[0] = lgo[Lc[0]] + ci / (n1 * n2);
+        g[1] = lgo[Lc[1]] + rest / n2;
         sl[0] = tup_slope(G, g[0], g[1]);
         if (sl[0] < lo || sl[0] > hi) continue;
         // time bound of the partial tuple: max of the groups' earliest minus min of their latest times
@@ -1373,21 +1375,20 @@ __device__ void tup_search(const A &G, const int lgo[5], int nmin, int nmax, dou
           tup_leaf(G, g, 2, sl, wq, kept);
           continue;
         }
-        for (g[2] = lgo[Lc[2]]; g[2] < lgo[Lc[2] + 1]; ++g[2]) {
-          sl[1] = tup_slope(G, g[1], g[2]);
-          if (sl[1] < lo || sl[1] > hi || fabs(sl[0] - sl[1]) > msd) continue;
-          const double l2 = fmax(l01, G.tmin(g[2])), u2 = fmin(u01, G.tmax(g[2]));
-          if (l2 - u2 > mdtw) continue;
-          if (nl == 3) {
-            tup_leaf(G, g, 3, sl, wq, kept);
-            continue;
-          }
-          for (g[3] = lgo[Lc[3]]; g[3] < lgo[Lc[3] + 1]; ++g[3]) {
-            sl[2] = tup_slope(G, g[2], g[3]);
-            if (sl[2] < lo || sl[2] > hi || fabs(sl[0] - sl[2]) > msd || fabs(sl[1] - sl[2]) > msd) continue;
-            if (fmax(l2, G.tmin(g[3])) - fmin(u2, G.tmax(g[3])) > mdtw) continue;
-            tup_leaf(G, g, 4, sl, wq, kept);
-          }
+        g[2] = lgo[Lc[2]] + rest % n2;
+        sl[1] = tup_slope(G, g[1], g[2]);
+        if (sl[1] < lo || sl[1] > hi || fabs(sl[0] - sl[1]) > msd) continue;
+        const double l2 = fmax(l01, G.tmin(g[2])), u2 = fmin(u01, G.tmax(g[2]));
+        if (l2 - u2 > mdtw) continue;
+        if (nl == 3) {
+          tup_leaf(G, g, 3, sl, wq, kept);
+          continue;
+        }
+        for (g[3] = lgo[Lc[3]]; g[3] < lgo[Lc[3] + 1]; ++g[3]) {
+          sl[2] = tup_slope(G, g[2], g[3]);
+          if (sl[2] < lo || sl[2] > hi || fabs(sl[0] - sl[2]) > msd || fabs(sl[1] - sl[2]) > msd) continue;
+          if (fmax(l2, G.tmin(g[3])) - fmin(u2, G.tmax(g[3])) > mdtw) continue;
+          tup_leaf(G, g, 4, sl, wq, kept);
         }
       }
       int k = nl - 1;
