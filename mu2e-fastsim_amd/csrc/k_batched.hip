@@ -955,8 +955,10 @@ __device__ __forceinline__ void fit_lmgn(const crv_params &p, const float *y, in
     double du[3];
     if (!fsolve(s.jtj, add, s.jtr, du)) du[0] = du[1] = du[2] = 0.0;
     const float un[3] = {u[0] + (float)du[0], u[1] + (float)du[1], u[2] + (float)du[2]};
-    feval(y, ys, q0, fs, fe, lo, hi, un, sn);
-    const bool better = sn.chi2 < s.chi2;
+    // a step that leaves u unchanged (the usual case once the damping is large) evaluates to s itself: not better
+    const bool same = un[0] == u[0] && un[1] == u[1] && un[2] == u[2];
+    if (!same) feval(y, ys, q0, fs, fe, lo, hi, un, sn);
+    const bool better = !same && sn.chi2 < s.chi2;
     if (better) {
       u[0] = un[0];
       u[1] = un[1];
@@ -981,6 +983,7 @@ __device__ __forceinline__ void fit_lmgn(const crv_params &p, const float *y, in
     bool improved = false;
     for (int k = 0; k < 4 && moved; ++k) {
       const float un[3] = {u[0] + sc[k] * (float)du[0], u[1] + sc[k] * (float)du[1], u[2] + sc[k] * (float)du[2]};
+      if (un[0] == u[0] && un[1] == u[1] && un[2] == u[2]) continue;  // evaluates to s itself: not better
       feval(y, ys, q0, fs, fe, lo, hi, un, sn);
       if (sn.chi2 < s.chi2) {
         u[0] = un[0];
