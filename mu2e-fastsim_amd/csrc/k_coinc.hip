@@ -198,6 +198,7 @@ __device__ int wave_clusters(const Tables &t, WS<WMAX> &S, const uint8_t *list, 
       const int h = list[i];
       const double th = S.t[h], xh = S.x[h], mh = md[h];
       const int sh = S.side[h];
+#pragma unroll 4  // the LDS reads of later positions overlap the tests of earlier ones
       for (int j = 0; j < m; ++j) {
         const int c = list[j];
         const double mdd = fmax(mh, md[c]);
