@@ -126,6 +126,7 @@ __global__ void kb_count(crv_params p, Tables t, Derived dv, BStep st, const int
   const int ch = (int)(key & 8191u) * 4 + s;
   const int a = task_off[tk], b = task_off[tk + 1];
   const bool live = ch < t.n_channels && (t.status[ch] & 0b101) == 0;
+  if (ch >= t.n_channels) atomicAdd(&flags[F_BADCHAN], 1);  // the host turns it into an error, as the FP64 chain
   const Draw dr{(uint32_t)p.seed, p.evkey_hi};
   const float ys = live ? (float)(dv.yscale * t.dev[ch]) : 0.f;
   const float H = (float)p.half_length;
@@ -1354,7 +1355,10 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     if (attempt > 8) return set_error(c, CRV_E_OVERFLOW, "batched chain: fired-pixel log still too short");
     if (L4) kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, qorder, L4, gcap, gsize, factor);
     if ((rc = scan_i64(c, gsize, goff, ngrp, 510, s))) return rc;
-    if ((rc = read_i64(c, goff + ngrp, &nlog, s))) return rc;
+    if (attempt == 0) CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
+    if ((rc = read_i64(c, goff + ngrp, &nlog, s))) return rc;  // (synchronises the stream)
+    if (attempt == 0 && hf[F_BADCHAN])
+      return set_error(c, CRV_E_ARG, "batched chain: channel beyond the uploaded conditions");
     int64_t log_chunks = nlog;  // goff and nlog count 16-byte chunks
     cuts.assign({0, ngrp});
     const bool pipe = opt->pipeline >= 2 && ngrp >= 2;  // two log buffers, each pass within half the budget

@@ -142,6 +142,22 @@ def test_log_budget_below_one_group_is_an_error(cosmic):
         CH.run_chain(S, cond, c32, key=7, geom=G, batch={"log_budget": 64})
 
 
+def test_counter_beyond_the_conditions_is_an_error():
+    """A step on a counter whose channels the conditions do not hold is an error in both chains (the reference
+    indexes its status table with the channel, chain.py:200, and fails there)."""
+    from crvhip import _lib as L
+    c64, c32 = _cfgs()
+    cond, G = DT.nominal_conditions(), DT.sector_geom()
+    S = {k: np.array(v) for k, v in SY.cosmic_events(20, seed=3).items()}
+    S["counter"][len(S["counter"]) // 2] = len(cond.status) // 4 + 2
+    for cfg in (c64, c32):
+        with pytest.raises(L.CrvError, match="beyond the uploaded conditions"):
+            CH.run_chain(S, cond, cfg, key=1, geom=G)
+    # the context stays usable: the valid steps run afterwards
+    S = SY.cosmic_events(20, seed=3)
+    assert len(CH.run_chain(S, cond, c32, key=1, geom=G)["pulses"]["ev"]) > 0
+
+
 def test_records_match_fp64_records(cosmic):
     S, cond, G, c64, c32, r64, r32 = cosmic
     R = r32["records"]
