@@ -131,9 +131,10 @@ def chunks_of(evkey, n_ev, chunk):
 AUTOTUNE_EVENTS = 200_000  # events of the sample the work-queue autotuner runs its candidates on
 
 
-def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, autotune=False):
+def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, autotune=False, tables=False):
     """Time `steps` chain passes over the batch (chunk by chunk, steps resident in HBM); returns (events/s, ms/step,
-    mean timing fields summed over chunks, last counts summed over chunks, extra)."""
+    mean timing fields summed over chunks, last counts summed over chunks, extra). With `tables` (one chunk) extra
+    holds the last pass's pulse columns ev, chan, time, pe on the host."""
     import torch
     from crvhip import _lib as L, config as CF, detector as DT, template as TP
     cfg = CF.ChainConfig(precision=precision)
@@ -204,10 +205,34 @@ def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, aut
     extra["chunks"] = len(structs)
     if tuned:
         extra["autotune"] = tuned
+    if tables:
+        assert len(structs) == 1, "pulse tables of a one-chunk run only"
+        extra["pulses"] = {k: out.keep[0][k][:out.P.n].cpu().numpy() for k in ("ev", "chan", "time", "pe")}
     ctx.close()
     del out, empty
     torch.cuda.empty_cache()
     return n_ev * steps / dt, dt / steps * 1e3, avg, runs[-1], extra
+
+
+def pulse_errors(P32, P64):
+    """FP32 pulse time and PE against the FP64 chain's on the same events: the pulses of each (event, channel) whose
+    pulse count is the same in both tables, paired in order (both tables are in (event, channel, time) order)."""
+    k32 = P32["ev"].astype(np.int64) * 32768 + P32["chan"]
+    k64 = P64["ev"].astype(np.int64) * 32768 + P64["chan"]
+    u32, n32 = np.unique(k32, return_counts=True)
+    u64, n64 = np.unique(k64, return_counts=True)
+    both, i32, i64 = np.intersect1d(u32, u64, assume_unique=True, return_indices=True)
+    keys = both[n32[i32] == n64[i64]]
+    m32, m64 = np.isin(k32, keys), np.isin(k64, keys)
+    dt = np.abs(P32["time"][m32] - P64["time"][m64])
+    dpe = np.abs(P32["pe"][m32] - P64["pe"][m64]) / np.maximum(np.abs(P64["pe"][m64]), 1e-9)
+    q = lambda a, f: float(np.quantile(a, f)) if len(a) else None  # noqa: E731
+    return {"note": "pulses of the (event, channel) pairs with the same pulse count in both chains, paired in time "
+                    "order; |dt| in ns, |dPE| relative",
+            "paired_fraction": float(m64.sum() / max(len(k64), 1)),
+            "pe_sum_rel": float((P32["pe"].sum() - P64["pe"].sum()) / max(abs(P64["pe"].sum()), 1e-9)),
+            "dt_median_ns": q(dt, 0.5), "dt_p99_ns": q(dt, 0.99), "dt_max_ns": float(dt.max()) if len(dt) else None,
+            "dpe_median": q(dpe, 0.5), "dpe_p99": q(dpe, 0.99)}
 
 
 def kernels_fp64(avg, cnt):
@@ -330,17 +355,20 @@ def main():
         ev64 = evkey[:cut] if isinstance(evkey, np.ndarray) else evkey[:cut].cpu().numpy().astype(np.int64)
         ds64 = DeviceSteps(ev64, ds.counter[:cut].cpu().numpy(), ds.edep[:cut].cpu().numpy(),
                            ds.z[:cut].cpu().numpy(), ds.t[:cut].cpu().numpy())
-        v64, ms64, avg64, cnt64, _ = measure("fp64", ds64, ev64, n64, args.seed + 1, args.steps, args.warmup, batch)
+        v64, ms64, avg64, cnt64, ex64 = measure("fp64", ds64, ev64, n64, args.seed + 1, args.steps, args.warmup,
+                                                batch, tables=True)
         # the FP32 chain on exactly these events (same seed, so the same Philox streams): its count errors
-        _, _, _, c32s, _ = measure("fp32", ds64, ev64, n64, args.seed + 1, 1, 0, batch)
+        _, _, _, c32s, ex32s = measure("fp32", ds64, ev64, n64, args.seed + 1, 1, 0, batch, tables=True)
         del ds64
         fp32_err = {k: (c32s[f"n_{k}"] - cnt64[f"n_{k}"]) / max(cnt64[f"n_{k}"], 1)
                     for k in ("photons", "charges", "pulses", "clusters", "members")}
+        fp32_pulse_err = pulse_errors(ex32s.pop("pulses"), ex64.pop("pulses"))
         k64 = kernels_fp64(avg64, cnt64)
         fp64 = {"value": v64, "unit": "events/s", "events": n64, "ms_per_step": ms64, "dtype": "f64",
                 "speedup_fp32_over_fp64": v32 / v64, "roofline": roofline(k64),
                 "fp32_rel_count_error": {"events": n64, "note": "photons, fired pixels (integer PE), pulses, "
                                          "clusters, cluster members: (FP32 - FP64) / FP64 on the same events", **fp32_err},
+                "fp32_pulse_error": fp32_pulse_err,
                 "kernel_ms": {k: v[0] for k, v in k64.items()},
                 "stage_ms": {k: avg64[k] for k in ("charges_ms", "digis_ms", "waveforms_ms", "pulses_ms",
                                                    "clusters_ms", "total_ms")},
