@@ -766,6 +766,7 @@ struct BlockLds {
   double dshare[4];
   int ishare[8];
   int flag;
+  int qpos;  // the block's current queue position
 };
 
 __device__ __forceinline__ int bscan(BlockLds &L, int v, int &total) {
@@ -1565,12 +1566,21 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
                                                     int32_t *__restrict__ ncl, int32_t *__restrict__ nmem,
                                                     ClSlots slots, int32_t *__restrict__ cl_moff,
                                                     int32_t *__restrict__ mem, int64_t min_n,
+                                                    const int32_t *__restrict__ qorder,
+                                                    const uint32_t *__restrict__ qkey, int *__restrict__ qctr,
                                                     int *__restrict__ flags) {
   __shared__ BlockLds L;
   const int tid = threadIdx.x;
-  for (int64_t g = blockIdx.x; g < nev; g += gridDim.x) {
+  // a work queue over the events in decreasing pulse count (the largest first, so that no block is left with a large
+  // event at the end); the queue ends at the first event of fewer than min_n pulses
+  for (;;) {
+    if (tid == 0) L.qpos = atomicAdd(qctr, 1);
+    __syncthreads();
+    const int64_t k = L.qpos;
+    __syncthreads();  // every thread has read the position before thread 0 takes the next
+    if (k >= nev || (int64_t)qkey[k] < min_n) break;
+    const int64_t g = qorder[k];
     const int64_t a = eoff[g], n64 = eoff[g + 1] - a;
-    if (n64 < min_n) continue;
     const int n = (int)n64;
     const int64_t evkey = evk[g];
     const int64_t sb = a + g;
@@ -1963,6 +1973,16 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
   }
 }
 
+// queue keys of the block finder: the pulse count (at most 65535) of an event of at least min_n pulses, else 0
+__global__ void k_block_keys(int64_t nev, const int64_t *__restrict__ eoff, int64_t min_n, uint32_t *__restrict__ key,
+                             int32_t *__restrict__ idx) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nev) return;
+  const int64_t n = eoff[g + 1] - eoff[g];
+  key[g] = n >= min_n ? (uint32_t)(n < 65535 ? n : 65535) : 0u;
+  idx[g] = (int32_t)g;
+}
+
 __global__ void k_coinc_out(int64_t nev, const int64_t *__restrict__ eoff, const int64_t *__restrict__ clo,
                             const int64_t *__restrict__ mo, const int32_t *__restrict__ ncl,
                             const int64_t *__restrict__ cl_ev, const int32_t *__restrict__ cl_type,
@@ -2067,6 +2087,14 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     for (auto q : bd) if (!*q) return ws_fail(c);
     for (auto q : bi) if (!*q) return ws_fail(c);
     for (auto q : bu) if (!*q) return ws_fail(c);
+    // the block finder's queue: events by decreasing pulse count (16-bit keys), sorted on its side stream
+    uint32_t *qk = (uint32_t *)ws(c, 186, nev * 4 + 4), *qk2 = (uint32_t *)ws(c, 187, nev * 4 + 4);
+    int32_t *qi = (int32_t *)ws(c, 188, nev * 4 + 4), *qo = (int32_t *)ws(c, 189, nev * 4 + 4);
+    int *qctr = (int *)ws(c, 190, 64);
+    size_t qtb = 0;
+    CRV_HIP(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, qtb, qk, qk2, qi, qo, (int)nev, 0, 16, c->aux[1]));
+    void *qtmp = ws(c, 191, qtb);
+    if (!qk || !qk2 || !qi || !qo || !qctr || !qtmp) return ws_fail(c);
     CRV_HIP(c, hipEventRecord(c->fork[0], s));
     CRV_HIP(c, hipStreamWaitEvent(c->aux[0], c->fork[0], 0));
     CRV_HIP(c, hipStreamWaitEvent(c->aux[1], c->fork[0], 0));
@@ -2079,10 +2107,17 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     k_coinc_wave<256><<<std::min(grid, 4096), 64, 0, c->aux[1]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem,
                                                                   slots, cl_moff, mem, (int64_t)129, c->flags);
     const hipError_t e2 = hipGetLastError();
-    k_coinc_block<<<(int)std::min<int64_t>(nev, 2048), BT, 0, c->aux[1]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, b,
-                                                                           ncl, nmem, slots, cl_moff, mem,
-                                                                           (int64_t)257, c->flags);
-    const hipError_t e3 = hipGetLastError();
+    k_block_keys<<<nblocks(nev, 256), 256, 0, c->aux[1]>>>(nev, eoff, (int64_t)257, qk, qi);
+    hipError_t e3 = hipGetLastError();
+    if (e3 == hipSuccess)
+      e3 = hipcub::DeviceRadixSort::SortPairsDescending(qtmp, qtb, qk, qk2, qi, qo, (int)nev, 0, 16, c->aux[1]);
+    if (e3 == hipSuccess) e3 = hipMemsetAsync(qctr, 0, 4, c->aux[1]);
+    if (e3 == hipSuccess) {
+      k_coinc_block<<<(int)std::min<int64_t>(nev, 2048), BT, 0, c->aux[1]>>>(c->p, c->t, nev, eoff, evk, time, pe, e,
+                                                                             b, ncl, nmem, slots, cl_moff, mem,
+                                                                             (int64_t)257, qo, qk2, qctr, c->flags);
+      e3 = hipGetLastError();
+    }
     // join before any error return, so that nothing of this call is left running on a side stream
     CRV_HIP(c, hipEventRecord(c->fork[1], c->aux[0]));
     CRV_HIP(c, hipEventRecord(c->fork[2], c->aux[1]));
