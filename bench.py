@@ -286,7 +286,9 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--pipeline", type=int, default=0,
                     help="FP32 chain: at least this many SiPM passes, waveforms of one beside photons of the next")
-    ap.add_argument("--sort", type=int, default=1)
+    ap.add_argument("--sort", type=int, default=1, choices=(0, 1, 2, 3),
+                    help="work-queue SiPM order: 0 as given, longest first by photon count (1), its octave (2) or "
+                         "quarter octave (3)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep --tile / --sort as given (with --tile 0 the work-queue tile and order are autotuned)")
     ap.add_argument("--chunk", type=int, default=0, help="events per chain call (0: the whole batch in one call)")

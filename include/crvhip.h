@@ -199,7 +199,9 @@ typedef struct crv_batch_opts {
   int32_t grid;        /* persistent workgroups of 64 lanes of the SiPM kernels (0: all that fit on the device) */
   int32_t tile;        /* SiPMs per work-queue grab, a multiple of 64 (0: 64) */
   int32_t pulse_slots; /* initial pulse slots per reco waveform (0: 4); grown and re-run when one has more */
-  int32_t sort_tasks;  /* 1: run the SiPMs longest first (photon count), so the lanes of a wave carry similar work */
+  int32_t sort_tasks;  /* SiPM order of the work queues: 0 as given (event, counter, SiPM); longest first, so that the
+                        * lanes of a wave carry similar work, 1 by photon count, 2 by its octave and 3 by its quarter
+                        * octave (SiPMs of one bucket in the given order, which keeps neighbours that share steps) */
   int64_t log_budget;  /* bytes of fired-pixel log per pass (0: half the device memory free at the call); a call whose
                         * log is larger runs its SiPMs in several passes of the photon and waveform kernels */
   /* written by the call: the window_samples-long ADC record of every cluster with a digi, in no particular order

@@ -1,8 +1,9 @@
 """Work-queue autotuner of the FP32 batched chain.
 
 The SiPM and waveform kernels of crv_chain_run_batched are persistent grids that take tiles of SiPMs from an atomic
-work queue, optionally longest first. The best tile depends on the workload's spread of photon counts (cosmic
-events: tens of photons per SiPM; dense showers: thousands on a few SiPMs) and on the device, so it is measured:
+work queue, optionally longest first (by photon count, or by its octave or quarter octave, which keeps SiPMs that
+share steps together). The best tile and order depend on the workload's spread of photon counts (cosmic events: tens
+of photons per SiPM; dense showers: thousands on a few SiPMs) and on the device, so they are measured:
 every candidate (tile, sort order, grid) runs the whole chain on a sample of the workload with zero-capacity outputs
 (the library then runs every kernel, reports its timing and writes nothing), and the fastest by the median of the
 work-queue kernels' times is kept. Results do not depend on the choice (the draws are keyed by event, channel, step
@@ -18,8 +19,8 @@ import os
 
 from . import _lib as L
 
-TILES = (64, 128, 256, 512, 1024)
-SORTS = (1, 0)
+TILES = (64, 128, 256, 512)
+SORTS = (1, 3, 2, 0)  # crv_batch_opts.sort_tasks: photon count, its quarter octave, its octave, as given
 GRIDS = (0,)
 
 

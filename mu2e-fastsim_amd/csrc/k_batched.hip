@@ -110,12 +110,16 @@ struct BStep {  // the steps of the batch (device SoA)
   const double *edep, *z, *t;
 };
 
+// key bits of a work-queue order (crv_batch_opts.sort_tasks 1, 2, 3): photon count, its octave (a bucket of at most
+// 32), its quarter octave (at most 128)
+__host__ __device__ constexpr int sort_key_bits(int mode) { return mode == 1 ? 32 : mode == 2 ? 6 : 8; }
+
 // photoelectrons per (step, SiPM), and per SiPM in total
 __global__ void kb_count(crv_params p, Tables t, Derived dv, BStep st, const int32_t *__restrict__ Tdev,
                          const uint64_t *__restrict__ task_key, const int32_t *__restrict__ task_off,
                          const int32_t *__restrict__ perm, const int32_t *__restrict__ evfirst,
                          int32_t *__restrict__ nph, int64_t *__restrict__ sipm_nph, uint32_t *__restrict__ sort_key,
-                         int32_t *__restrict__ sort_idx, int *__restrict__ flags) {
+                         int32_t *__restrict__ sort_idx, int sort_mode, int *__restrict__ flags) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t L4 = (int64_t)(*Tdev) * 4;
   if (g >= L4) return;
@@ -151,8 +155,22 @@ __global__ void kb_count(crv_params p, Tables t, Derived dv, BStep st, const int
     total += n;
   }
   sipm_nph[g] = total;
-  // longest first: descending photon count, ties in SiPM order (stable radix sort of the complement)
-  sort_key[g] = 0xFFFFFFFFu - (uint32_t)(total < 0xFFFFFFFFll ? total : 0xFFFFFFFFll);
+  // longest first, ties in SiPM order (a stable radix sort of the complement): mode 1 by photon count, modes 2 and 3
+  // by its octave or quarter octave (fewer key bits, and the SiPMs of one bucket keep their order, so that
+  // neighbouring SiPMs, which share steps, stay neighbours in the queue)
+  const uint32_t v = (uint32_t)(total < 0xFFFFFFFFll ? total : 0xFFFFFFFFll);
+  if (sort_mode == 1) {
+    sort_key[g] = 0xFFFFFFFFu - v;
+  } else {
+    const int mb = sort_mode == 3 ? 2 : 0;  // bits of the count below its leading one that the bucket keeps
+    uint32_t bucket = 0;                    // 0 for no photons, else 1 + (octave << mb) + those bits
+    if (v) {
+      const int e = 31 - __clz(v);
+      const uint32_t m = (e >= mb ? v >> (e - mb) : v << (mb - e)) & ((1u << mb) - 1u);
+      bucket = 1u + ((uint32_t)e << mb) + m;
+    }
+    sort_key[g] = ((1u << sort_key_bits(sort_mode)) - 1u) - bucket;
+  }
   sort_idx[g] = (int32_t)g;
 }
 
@@ -1228,6 +1246,8 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   if (c->d.nreach > RINGB) return set_error(c, CRV_E_ARG, "batched chain: single-PE template reaches over 16 samples");
   const int tile = opt->tile > 0 ? opt->tile : 64;
   if (tile % 64) return set_error(c, CRV_E_ARG, "batched chain: tile must be a multiple of 64");
+  if (opt->sort_tasks < 0 || opt->sort_tasks > 3)
+    return set_error(c, CRV_E_ARG, "batched chain: sort_tasks must be 0, 1, 2 or 3");
   const int grid_s = opt->grid > 0 ? opt->grid : resident_blocks(c, (const void *)kb_sipm);
   const int grid_w = opt->grid > 0 ? opt->grid : resident_blocks(c, (const void *)kb_wave);
   hipEvent_t *ev = c->ev, *kev = c->kev;
@@ -1269,15 +1289,16 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   const BStep st{g.edep, g.z, g.t};
   if (L4)
     kb_count<<<nblocks(L4, 256), 256, 0, s>>>(p, c->t, c->d, st, Tdev, g.task_key, g.task_off, g.perm, g.evfirst,
-                                              nph, sipm_nph, skey, sidx, c->flags);
+                                              nph, sipm_nph, skey, sidx, opt->sort_tasks, c->flags);
   CRV_HIP(c, hipGetLastError());
   const bool sorted = opt->sort_tasks != 0 && L4 > 0;
+  const int sbits = sort_key_bits(opt->sort_tasks);
   if (sorted) {
     size_t tb = 0;
-    CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, skey, skey2, sidx, order, (int)L4, 0, 32, s));
+    CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, skey, skey2, sidx, order, (int)L4, 0, sbits, s));
     void *tmp = ws(c, 509, tb);
     if (!tmp) return ws_fail(c);
-    CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, skey, skey2, sidx, order, (int)L4, 0, 32, s));
+    CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, skey, skey2, sidx, order, (int)L4, 0, sbits, s));
   }
   CRV_HIP(c, hipEventRecord(ev[1], s));
   int hf[F_NFLAGS];

@@ -98,7 +98,8 @@ def test_pulses_ordered_by_event_channel_time(cosmic):
 
 def test_launch_shape_does_not_change_results(cosmic):
     S, cond, G, c64, c32, r64, r32 = cosmic
-    for batch in ({"grid": 96, "tile": 128}, {"sort_tasks": 1, "tile": 64}, {"pulse_slots": 1, "grid": 300}):
+    for batch in ({"grid": 96, "tile": 128}, {"sort_tasks": 1, "tile": 64}, {"sort_tasks": 2, "tile": 128},
+                  {"sort_tasks": 3, "tile": 64}, {"pulse_slots": 1, "grid": 300}):
         r = CH.run_chain(S, cond, c32, key=7, geom=G, batch=batch)
         for k, v in r32["pulses"].items():
             np.testing.assert_array_equal(r["pulses"][k], v, err_msg=f"{batch} {k}")
@@ -140,6 +141,8 @@ def test_log_budget_below_one_group_is_an_error(cosmic):
     S, cond, G, c64, c32, r64, r32 = cosmic
     with pytest.raises(L.CrvError, match="exceeds the log budget"):
         CH.run_chain(S, cond, c32, key=7, geom=G, batch={"log_budget": 64})
+    with pytest.raises(L.CrvError, match="sort_tasks must be"):
+        CH.run_chain(S, cond, c32, key=7, geom=G, batch={"sort_tasks": 4})
 
 
 def test_counter_beyond_the_conditions_is_an_error():
