@@ -1560,7 +1560,7 @@ struct BlockAcc {
   __device__ double mincp(const Tables &t, int h) const { return cut_of(t, S.sec[h]).min_cluster_pes; }
 };
 
-__global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int64_t nev, const int64_t *__restrict__ eoff,
+__global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int64_t nq, const int64_t *__restrict__ eoff,
                                                     const int64_t *__restrict__ evk, const double *__restrict__ ptime,
                                                     const double *__restrict__ ppe, Ev e, BS s0,
                                                     int32_t *__restrict__ ncl, int32_t *__restrict__ nmem,
@@ -1571,14 +1571,14 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
                                                     int *__restrict__ flags) {
   __shared__ BlockLds L;
   const int tid = threadIdx.x;
-  // a work queue over the events in decreasing pulse count (the largest first, so that no block is left with a large
-  // event at the end); the queue ends at the first event of fewer than min_n pulses
+  // a work queue of the nq events of at least min_n pulses in decreasing pulse count (the largest first, so that no
+  // block is left with a large event at the end)
   for (;;) {
     if (tid == 0) L.qpos = atomicAdd(qctr, 1);
     __syncthreads();
     const int64_t k = L.qpos;
     __syncthreads();  // every thread has read the position before thread 0 takes the next
-    if (k >= nev || (int64_t)qkey[k] < min_n) break;
+    if (k >= nq || (int64_t)qkey[k] < min_n) break;
     const int64_t g = qorder[k];
     const int64_t a = eoff[g], n64 = eoff[g + 1] - a;
     const int n = (int)n64;
@@ -1973,14 +1973,17 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
   }
 }
 
-// queue keys of the block finder: the pulse count (at most 65535) of an event of at least min_n pulses, else 0
+// the block finder's events: each event of at least min_n pulses appended (in no particular order) with its pulse
+// count, at most 65535, as the queue's sort key
 __global__ void k_block_keys(int64_t nev, const int64_t *__restrict__ eoff, int64_t min_n, uint32_t *__restrict__ key,
-                             int32_t *__restrict__ idx) {
+                             int32_t *__restrict__ idx, int *__restrict__ count) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nev) return;
   const int64_t n = eoff[g + 1] - eoff[g];
-  key[g] = n >= min_n ? (uint32_t)(n < 65535 ? n : 65535) : 0u;
-  idx[g] = (int32_t)g;
+  if (n < min_n) return;
+  const int k = atomicAdd(count, 1);
+  key[k] = (uint32_t)(n < 65535 ? n : 65535);
+  idx[k] = (int32_t)g;
 }
 
 __global__ void k_coinc_out(int64_t nev, const int64_t *__restrict__ eoff, const int64_t *__restrict__ clo,
@@ -2087,14 +2090,24 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     for (auto q : bd) if (!*q) return ws_fail(c);
     for (auto q : bi) if (!*q) return ws_fail(c);
     for (auto q : bu) if (!*q) return ws_fail(c);
-    // the block finder's queue: events by decreasing pulse count (16-bit keys), sorted on its side stream
+    // the block finder's queue: the events of more than 256 pulses, compacted here (their count read back), then
+    // sorted by decreasing pulse count (16-bit keys) on the block finder's side stream
     uint32_t *qk = (uint32_t *)ws(c, 186, nev * 4 + 4), *qk2 = (uint32_t *)ws(c, 187, nev * 4 + 4);
     int32_t *qi = (int32_t *)ws(c, 188, nev * 4 + 4), *qo = (int32_t *)ws(c, 189, nev * 4 + 4);
-    int *qctr = (int *)ws(c, 190, 64);
+    int *qctr = (int *)ws(c, 190, 64);  // queue position, then the number of block-finder events
+    if (!qk || !qk2 || !qi || !qo || !qctr) return ws_fail(c);
+    CRV_HIP(c, hipMemsetAsync(qctr, 0, 8, s));
+    k_block_keys<<<nblocks(nev, 256), 256, 0, s>>>(nev, eoff, (int64_t)257, qk, qi, qctr + 1);
+    CRV_HIP(c, hipGetLastError());
+    int nq = 0;
+    CRV_HIP(c, hipMemcpyAsync(&nq, qctr + 1, 4, hipMemcpyDeviceToHost, s));
+    CRV_HIP(c, hipStreamSynchronize(s));
     size_t qtb = 0;
-    CRV_HIP(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, qtb, qk, qk2, qi, qo, (int)nev, 0, 16, c->aux[1]));
-    void *qtmp = ws(c, 191, qtb);
-    if (!qk || !qk2 || !qi || !qo || !qctr || !qtmp) return ws_fail(c);
+    void *qtmp = nullptr;
+    if (nq > 0) {
+      CRV_HIP(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, qtb, qk, qk2, qi, qo, nq, 0, 16, c->aux[1]));
+      if (!(qtmp = ws(c, 191, qtb))) return ws_fail(c);
+    }
     CRV_HIP(c, hipEventRecord(c->fork[0], s));
     CRV_HIP(c, hipStreamWaitEvent(c->aux[0], c->fork[0], 0));
     CRV_HIP(c, hipStreamWaitEvent(c->aux[1], c->fork[0], 0));
@@ -2107,16 +2120,15 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     k_coinc_wave<256><<<std::min(grid, 4096), 64, 0, c->aux[1]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem,
                                                                   slots, cl_moff, mem, (int64_t)129, c->flags);
     const hipError_t e2 = hipGetLastError();
-    k_block_keys<<<nblocks(nev, 256), 256, 0, c->aux[1]>>>(nev, eoff, (int64_t)257, qk, qi);
-    hipError_t e3 = hipGetLastError();
-    if (e3 == hipSuccess)
-      e3 = hipcub::DeviceRadixSort::SortPairsDescending(qtmp, qtb, qk, qk2, qi, qo, (int)nev, 0, 16, c->aux[1]);
-    if (e3 == hipSuccess) e3 = hipMemsetAsync(qctr, 0, 4, c->aux[1]);
-    if (e3 == hipSuccess) {
-      k_coinc_block<<<(int)std::min<int64_t>(nev, 2048), BT, 0, c->aux[1]>>>(c->p, c->t, nev, eoff, evk, time, pe, e,
-                                                                             b, ncl, nmem, slots, cl_moff, mem,
-                                                                             (int64_t)257, qo, qk2, qctr, c->flags);
-      e3 = hipGetLastError();
+    hipError_t e3 = hipSuccess;
+    if (nq > 0) {
+      e3 = hipcub::DeviceRadixSort::SortPairsDescending(qtmp, qtb, qk, qk2, qi, qo, nq, 0, 16, c->aux[1]);
+      if (e3 == hipSuccess) {
+        k_coinc_block<<<std::min(nq, 2048), BT, 0, c->aux[1]>>>(c->p, c->t, (int64_t)nq, eoff, evk, time, pe, e, b, ncl,
+                                                               nmem, slots, cl_moff, mem, (int64_t)257, qo, qk2, qctr,
+                                                               c->flags);
+        e3 = hipGetLastError();
+      }
     }
     // join before any error return, so that nothing of this call is left running on a side stream
     CRV_HIP(c, hipEventRecord(c->fork[1], c->aux[0]));
