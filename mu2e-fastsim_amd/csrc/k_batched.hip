@@ -453,6 +453,17 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
         } else {  // one load per stream, four streams' loads in flight at a time
           tph = INFINITY;
           int k = a;
+          for (; k + 8 <= b; k += 8) {
+            double h[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) h[q] = o.gHead[(int64_t)(k + q) * 4 + s];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+              if (h[q] < tph) {
+                kb = k + q;
+                tph = h[q];
+              }
+          }
           for (; k + 4 <= b; k += 4) {
             const double h0 = o.gHead[(int64_t)k * 4 + s], h1 = o.gHead[(int64_t)(k + 1) * 4 + s];
             const double h2 = o.gHead[(int64_t)(k + 2) * 4 + s], h3 = o.gHead[(int64_t)(k + 3) * 4 + s];
