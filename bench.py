@@ -260,7 +260,7 @@ def kernels_fp32(avg, cnt, fires):
 # HBM bytes per event actually moved by a kernel on the cosmic workload: rocprofv3 FETCH_SIZE + WRITE_SIZE passes
 # over 1M events, each scaled by the factor that makes a plain 1 GiB device copy read and write 1 GiB
 # (profiles/chain_fp32_cosmic1m_traffic.txt); used for roofline.traffic of the same workload only
-PMC_BYTES_PER_EVENT = {"cosmic": {"kb_sipm": 111710.4, "kb_wave": 47951.4, "kb_fit": 33254.8}}
+PMC_BYTES_PER_EVENT = {"cosmic": {"kb_sipm": 88487.6, "kb_wave": 37401.1, "kb_fit": 24707.7}}
 
 
 def roofline(kernels, workload=None, events=0):
