@@ -292,7 +292,7 @@ struct Out {
   // photon streams of the (step, SiPM) pairs of counters with several steps
   float *gS;
   double *gHead, *gTb;
-  int32_t *gR;
+  uint64_t *gRJ;  // per stream: step ordinal in its event << 32 | photon rank (ranks stay below 65536)
   // cascade lists per persistent lane [slot][grid lane]
   uint16_t *xcur, *xcand;
   uint8_t *xmult;
@@ -415,9 +415,9 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
       for (int k = a; k < b; ++k) {
         const int64_t gi = (int64_t)k * 4 + s;
         const int N = nph[gi];
-        if (nstr > 1) {
-          o.gR[gi] = 0;
-          if (N <= 0) o.gHead[gi] = INFINITY;  // an empty or exhausted stream's head is +inf
+        if (nstr > 1 && N <= 0) {
+          o.gRJ[gi] = 0;
+          o.gHead[gi] = INFINITY;  // an empty or exhausted stream's head is +inf
         }
         if (N <= 0) continue;
         const int32_t i = perm[k];
@@ -438,6 +438,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           o.gS[gi] = S;
           o.gTb[gi] = tb;
           o.gHead[gi] = tb + (double)(tauf * S);
+          o.gRJ[gi] = (uint64_t)j << 32;
         }
       }
       bool bad = false;
@@ -510,9 +511,9 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           r1 += 1;
         } else {
           const int64_t gi = (int64_t)kb * 4 + s;
-          const int32_t row = perm[kb];
-          j = (uint32_t)(row - evfirst[row]);
-          r = o.gR[gi];
+          const uint64_t rj = o.gRJ[gi];  // one load for the step ordinal and the rank
+          j = (uint32_t)(rj >> 32);
+          r = (int)(uint32_t)rj;
           const int N = nph[gi];
           if (r + 1 < N) {
             const int rn = r + 1;
@@ -524,7 +525,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           } else {
             o.gHead[gi] = INFINITY;
           }
-          o.gR[gi] = r + 1;
+          o.gRJ[gi] = rj + 1u;
         }
         // pixel of the photon (photon-map CDF), firing with the recovered overvoltage
         const U4 u = dr(KIND_PIXEL, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, 0u, 0u));
@@ -1287,10 +1288,10 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   o.gS = (float *)ws(c, 534, n * 16 + 16);
   o.gHead = (double *)ws(c, 535, n * 32 + 16);
   o.gTb = (double *)ws(c, 536, n * 32 + 16);
-  o.gR = (int32_t *)ws(c, 537, n * 16 + 16);
+  o.gRJ = (uint64_t *)ws(c, 537, n * 32 + 16);
   o.nrun = (int32_t *)ws(c, 513, L4 * 4 + 8);
   if (!Tdev || !nph || !sipm_nph || !gcap || !gsize || !goff || !skey || !skey2 || !sidx || !order || !nfired || !counters ||
-      !o.xcur || !o.xcand || !o.xmult || !o.lastt || !o.gS || !o.gHead || !o.gTb || !o.gR || !o.nrun)
+      !o.xcur || !o.xcand || !o.xmult || !o.lastt || !o.gS || !o.gHead || !o.gTb || !o.gRJ || !o.nrun)
     return ws_fail(c);
   o.rec_count = counters + 2;
   o.run_count = counters + 3;
