@@ -89,11 +89,13 @@ struct Ctx {
   bool kev_on = false;
   int64_t b_rec_cap = 0, b_run_cap = 0;  // pool sizes of the batched chain (grown on demand)
   int b_log_factor = 1;  // fired-pixel log capacity factor of the batched chain, doubled on overflow and kept
+  std::vector<int> ws_spare;  // slots the running call no longer reads: freed to make room when an allocation fails
   int ws_fail_slot = -1;                  // the workspace request that failed last
   size_t ws_fail_bytes = 0;
 };
 
-// workspace slot of at least `bytes` bytes (grown, never shrunk)
+// workspace slot of at least `bytes` bytes (grown, never shrunk; when the device has no room, the slots listed in
+// ws_spare are freed once, after the device has drained, and the allocation is tried again)
 void *ws(Ctx *c, int slot, size_t bytes);
 
 // device flag words

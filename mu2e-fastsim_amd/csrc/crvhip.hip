@@ -35,9 +35,24 @@ void *ws(Ctx *c, int slot, size_t bytes) {
     size_t b = bytes + bytes / 8;
     if (hipMalloc(&c->ws[slot], b) != hipSuccess) {
       (void)hipGetLastError();
-      c->ws_fail_slot = slot;
-      c->ws_fail_bytes = b;
-      return nullptr;
+      c->ws[slot] = nullptr;
+      bool freed = false;
+      for (const int sp : c->ws_spare)
+        if (sp != slot && sp < (int)c->ws.size() && c->ws[sp]) {
+          if (!freed) (void)hipDeviceSynchronize();  // no kernel still reads a spare slot
+          (void)hipFree(c->ws[sp]);
+          c->ws[sp] = nullptr;
+          c->ws_size[sp] = 0;
+          freed = true;
+        }
+      c->ws_spare.clear();
+      if (!freed || hipMalloc(&c->ws[slot], b) != hipSuccess) {
+        (void)hipGetLastError();
+        c->ws[slot] = nullptr;
+        c->ws_fail_slot = slot;
+        c->ws_fail_bytes = b;
+        return nullptr;
+      }
     }
     c->ws_size[slot] = b;
   }

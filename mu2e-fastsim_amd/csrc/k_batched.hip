@@ -1247,6 +1247,7 @@ int resident_blocks(Ctx *c, const void *kernel) {
 
 // ---- host side ----
 int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timing *timing, hipStream_t s) {
+  c->ws_spare.clear();
   const crv_params &p = c->p;
   const int W = p.window_samples, Sz = p.samples_zs;
   if (W > 32 * DWORDS || Sz < 1 || W <= Sz)
@@ -1526,6 +1527,9 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
       break;
     }
   }
+  // the photon and waveform kernels are done: their fired-pixel logs, last-fire times, photon streams and cascade
+  // lists may be freed if the fit or the finder needs the room (they are allocated again by the next call)
+  c->ws_spare = {534, 535, 536, 537, 538, 539, 543, 544, 545, 547};
   // photons drawn and pixels fired (diagnostics)
   int64_t *sums = (int64_t *)ws(c, 515, 16);
   int64_t *fired64 = (int64_t *)ws(c, 516, L4 * 8 + 8);
