@@ -205,7 +205,15 @@ def test_uneven_showers():
     c64, c32 = _cfgs()
     cond, G = DT.nominal_conditions(), DT.sector_geom()
     S = SY.uneven_events(300, seed=8)
-    _compare(CH.run_chain(S, cond, c32, key=3, geom=G), CH.run_chain(S, cond, c64, key=3, geom=G), rtol=5e-3)
+    r32 = CH.run_chain(S, cond, c32, key=3, geom=G)
+    _compare(r32, CH.run_chain(S, cond, c64, key=3, geom=G), rtol=5e-3)
+    # counters of many steps (merged photon streams) and events for the block finder: the same tables in every
+    # queue order and tile
+    for batch in ({"sort_tasks": 2, "tile": 64}, {"sort_tasks": 3, "tile": 128}, {"sort_tasks": 1, "tile": 256}):
+        r = CH.run_chain(S, cond, c32, key=3, geom=G, batch=batch)
+        for tab in ("pulses", "clusters"):
+            for k, v in r32[tab].items():
+                np.testing.assert_array_equal(r[tab][k], v, err_msg=f"{batch} {tab} {k}")
 
 
 def test_overlay_events():
