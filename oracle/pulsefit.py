@@ -274,8 +274,9 @@ PULSE_KEYS = ("wf", "seq", "pe", "peph", "time", "height", "beta", "chi2", "leti
               "pedfromdb", "fit_A", "fit_mu", "fit_b")
 
 
-def reconstruct(adc, n, start_tdc, pedestal_db, calib_area, calib_height, cfg=RecoConfig()):
-    """Reference reco/pulsefit.py:236-316."""
+def reconstruct(adc, n, start_tdc, pedestal_db, calib_area, calib_height, cfg=RecoConfig(), windows=None):
+    """Reference reco/pulsefit.py:236-316. `windows`, if a list, receives each round's fit inputs
+    (pulse rows, t, y, mask), in the order pulses are produced, for checks of the fit's end point."""
     adc = np.asarray(adc)
     W = adc.shape[0]
     L = adc.shape[1] if adc.ndim == 2 else 0
@@ -320,6 +321,8 @@ def reconstruct(adc, n, start_tdc, pedestal_db, calib_area, calib_height, cfg=Re
         tt = tgrid[k[:, None], idxc]
         yy = y[k[:, None], idxc]
         par, chi2, conv, hok = fit_gumbel(tt, yy, mask, p0, lo, hi, cfg.lm_iterations)
+        if windows is not None:
+            windows.append((k, seq, tt, yy, mask))
         npts = mask.sum(1)
         ndf = npts - 3
         span = hi - lo
@@ -354,9 +357,9 @@ def reconstruct(adc, n, start_tdc, pedestal_db, calib_area, calib_height, cfg=Re
     return res
 
 
-def reconstruct_full(adc, n, start_tdc, pedestal_db, calib_area, calib_height, cfg=RecoConfig()):
+def reconstruct_full(adc, n, start_tdc, pedestal_db, calib_area, calib_height, cfg=RecoConfig(), windows=None):
     return reconstruct(adc, n, start_tdc, pedestal_db, calib_area, calib_height,
-                       replace(cfg, max_pulses=max(cfg.max_pulses, int(np.max(n, initial=0)))))
+                       replace(cfg, max_pulses=max(cfg.max_pulses, int(np.max(n, initial=0)))), windows)
 
 
 def filter_channels(pulses, channels, status, cfg=RecoConfig()):

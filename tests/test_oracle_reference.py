@@ -2,8 +2,16 @@
 import numpy as np
 import pytest
 
-from oracle import coinc as OCo, pulsefit as OPF, response as ORS, waveform as OWF
+from oracle import coinc as OCo, fmath as OFM, pulsefit as OPF, response as ORS, waveform as OWF
 from crvhip import config as CF, detector as DT
+
+import fitcheck
+
+try:
+    from numpy._core._multiarray_umath import __cpu_features__ as _CPU
+    _NUMPY_AVX512 = bool(_CPU.get("AVX512F"))
+except ImportError:
+    _NUMPY_AVX512 = False
 
 
 def _sipm_cfg(golden, name):
@@ -59,25 +67,88 @@ def test_concat_waveforms_matches_reference(golden):
 
 
 PF_CASES = ["synth12", "edge36", "double36", "train252", "chain"]
+PF_ARGS = ("adc", "n", "start", "ped", "area", "height")
 
 
 @pytest.mark.parametrize("name", PF_CASES)
 def test_pulse_reco_matches_reference(golden, name):
-    args = [golden[f"pf_{name}_in_{k}"] for k in ("adc", "n", "start", "ped", "area", "height")]
-    P = OPF.reconstruct_full(*args)
+    """Flags exact, times within 1e-9 relative; PE, height, beta within 1e-9 relative or, for the few pulses whose
+    fit stopped elsewhere in chi2's double-precision flat region, chi2-equivalent to the reference's end point."""
+    args = [golden[f"pf_{name}_in_{k}"] for k in PF_ARGS]
+    win = []
+    P = OPF.reconstruct_full(*args, windows=win)
     g = lambda k: golden[f"pf_{name}_{k}"]
     for k in ("wf", "seq", "failed", "zerondf", "pedfromdb", "wf_near_threshold"):
         np.testing.assert_array_equal(P[k], g(k), err_msg=k)
-    # The reference's fit stops after a fixed 40 LM + 8 GN iterations, converged to ~5e-9 relative; its LAPACK solve,
-    # pairwise sums and libm move that end point by a few 1e-9 against the oracle's fixed arithmetic. Flags above are
-    # exact; values within 2e-8 relative (times within 1e-6 ns).
-    for k in ("pe", "height", "beta", "fit_A", "fit_b", "peph", "ped"):
-        np.testing.assert_allclose(P[k], g(k), rtol=2e-8, atol=0, err_msg=k)
-    for k in ("time", "letime", "fit_mu"):
-        np.testing.assert_allclose(P[k], g(k), rtol=0, atol=1e-6, err_msg=k)
+    ref = {k: g(k) for k in P if k != "wf_near_threshold"}
+    worst, _ = fitcheck.check_fit_values(P, ref, win)
+    np.testing.assert_array_equal(P["ped"], g("ped"))
     ok = g("chi2") > 0
     np.testing.assert_allclose(P["chi2"][ok], g("chi2")[ok], rtol=1e-7, atol=1e-9)
     np.testing.assert_array_equal(P["chi2"] < 0, g("chi2") < 0)
+
+
+def _reference_arithmetic(monkeypatch, exp=np.exp):
+    """The oracle's fit with the reference's arithmetic: numpy's exp, sin, cos, arcsin and LAPACK's solve."""
+    def solve(A, b):
+        return np.linalg.solve(A, b[..., None])[..., 0], np.zeros(len(A), bool)
+    monkeypatch.setattr(OFM, "exp", exp)
+    monkeypatch.setattr(OFM, "sin", np.sin)
+    monkeypatch.setattr(OFM, "cos", np.cos)
+    monkeypatch.setattr(OFM, "asin", np.arcsin)
+    monkeypatch.setattr(OPF, "solve3", solve)
+
+
+@pytest.mark.skipif(not _NUMPY_AVX512, reason="the golden fits were written by numpy's AVX512F exp")
+@pytest.mark.parametrize("name", PF_CASES)
+def test_fit_schedule_is_the_references(golden, monkeypatch, name):
+    """With numpy's transcendentals and LAPACK's solve in place of its own arithmetic, the oracle reproduces every
+    output of the reference's reconstruction bit for bit: iteration schedule, acceptance rule, damping, step halving,
+    failure rules and subtraction rounds are the reference's; only the arithmetic primitives differ."""
+    _reference_arithmetic(monkeypatch)
+    args = [golden[f"pf_{name}_in_{k}"] for k in PF_ARGS]
+    P = OPF.reconstruct_full(*args)
+    for k, v in P.items():
+        np.testing.assert_array_equal(v, golden[f"pf_{name}_{k}"], err_msg=k)
+
+
+@pytest.mark.skipif(not _NUMPY_AVX512, reason="the golden fits were written by numpy's AVX512F exp")
+def test_reference_fit_moves_with_its_exp(golden, monkeypatch):
+    """The reference's own end point is not reproducible to 1e-9 across hosts: numpy evaluates exp with its AVX512F
+    kernel where the CPU has one and with the C library's exp elsewhere, and that alone moves the fitted PE of the
+    synth12 batch by more than 1e-9 relative. Its pulses stay chi2-equivalent to the recorded ones."""
+    import math
+    libm_exp = np.frompyfunc(math.exp, 1, 1)
+    _reference_arithmetic(monkeypatch, exp=lambda x: np.asarray(libm_exp(x), np.float64))
+    args = [golden[f"pf_synth12_in_{k}"] for k in PF_ARGS]
+    win = []
+    P = OPF.reconstruct_full(*args, windows=win)
+    ref = {k: golden[f"pf_synth12_{k}"] for k in P if k != "wf_near_threshold"}
+    for k in ("wf", "seq", "failed"):
+        np.testing.assert_array_equal(P[k], ref[k])
+    worst, _ = fitcheck.check_fit_values(P, ref, win)
+    assert worst > 1e-9
+
+
+def test_chi2_equivalence_resolves_moves_off_the_flat_direction(golden):
+    """The equivalence is no blanket tolerance: the reference's end points moved by 3e-9 relative in random
+    directions fail it for most pulses, while the oracle's end points pass it for every pulse."""
+    args = [golden[f"pf_double36_in_{k}"] for k in PF_ARGS]
+    win = []
+    OPF.reconstruct_full(*args, windows=win)
+    W = fitcheck.window_map(win)
+    g = lambda k: golden[f"pf_double36_{k}"]
+    rng = np.random.default_rng(4)
+    fails = 0
+    for i in range(len(g("wf"))):
+        t, y, m = W[(int(g("wf")[i]), int(g("seq")[i]))]
+        d = rng.normal(size=3)
+        d *= 3e-9 / np.linalg.norm(d)
+        c0, n0 = fitcheck.chi2_and_noise(t, y, m, g("fit_A")[i], g("fit_mu")[i], g("fit_b")[i])
+        c1, n1 = fitcheck.chi2_and_noise(t, y, m, g("fit_A")[i] * (1 + d[0]), g("fit_mu")[i] * (1 + d[1]),
+                                         g("fit_b")[i] * (1 + d[2]))
+        fails += float(abs(c1 - c0)) > max(n0, n1)
+    assert fails > 0.9 * len(g("wf"))
 
 
 def _finder():
