@@ -61,12 +61,14 @@ def _cpu_shard(args):
     from oracle import chain as OC
     cfg = CF.ChainConfig()
     cond = DT.nominal_conditions()
+    t0 = time.perf_counter()
     out = OC.run(steps, cond, DT.sector_geom(), cfg, seed, TP.load(cfg.waveform))
-    return len(out["clusters"]["cl_event"])
+    return len(out["clusters"]["cl_event"]), time.perf_counter() - t0
 
 
 def cpu_baseline(S, n_events, seed, per_worker, workers):
-    """The oracle chain on `workers` processes over contiguous event shards of the same steps (fork, no GPU)."""
+    """The oracle chain on `workers` processes over contiguous event shards of the same steps (fork, no GPU). The
+    pool is started and every worker has run one event before the clock starts, so start-up is not timed."""
     import multiprocessing as mp
     ev = S["evkey"]
     shards = []
@@ -76,15 +78,19 @@ def cpu_baseline(S, n_events, seed, per_worker, workers):
             break
         lo, hi = np.searchsorted(ev, a), np.searchsorted(ev, b)
         shards.append(({k: v[lo:hi] for k, v in S.items()}, seed))
+    one = np.searchsorted(ev, 1)
+    warm = [({k: v[:one] for k, v in S.items()}, seed)] * len(shards)
     ctx = mp.get_context("fork")
-    t0 = time.perf_counter()
     with ctx.Pool(len(shards)) as pool:
-        pool.map(_cpu_shard, shards)
-    dt = time.perf_counter() - t0
+        pool.map(_cpu_shard, warm, chunksize=1)
+        t0 = time.perf_counter()
+        res = pool.map(_cpu_shard, shards, chunksize=1)
+        dt = time.perf_counter() - t0
     n = len(shards) * per_worker
     return {"value": n / dt, "unit": "events/s", "cores": len(shards), "kind": "port",
-            "sample": f"{n} events ({len(shards)} shards of {per_worker} contiguous events of the same synthetic "
-                      f"batch), oracle chain steps->clusters (FP64), {dt:.1f} s wall"}
+            "sample": f"{n} events ({len(shards)} processes, one shard of {per_worker} contiguous events each, of the "
+                      f"same synthetic batch), oracle chain steps->clusters (FP64), {dt:.1f} s wall after pool "
+                      f"start-up; slowest shard {max(r[1] for r in res):.1f} s"}
 
 
 class Outputs:
@@ -263,14 +269,47 @@ def kernels_fp32(avg, cnt, fires):
 PMC_BYTES_PER_EVENT = {"cosmic": {"kb_sipm": 88487.6, "kb_wave": 37401.1, "kb_fit": 24707.7}}
 
 
-def roofline(kernels, workload=None, events=0):
+def roofline(kernels, workload=None, events=0, per_call=1):
+    """The dominant kernel's algorithmic bytes over its HIP-event time (kernel times and bytes are summed over the
+    `per_call` calls of a pass; per-launch figures divide by them; `events` per call)."""
     dom = max(kernels, key=lambda k: kernels[k][0])
     kms, kbytes = kernels[dom]
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     per_event = PMC_BYTES_PER_EVENT.get(workload, {}).get(dom)
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": per_event * events if per_event else None,
-            "bytes_per_launch": kbytes, "ms_per_launch": kms}
+            "bytes_per_launch": kbytes / per_call, "ms_per_launch": kms / per_call}
+
+
+def workload_leg(workload, n_events, chunk, seed, steps, warmup, batch, autotune):
+    """One extra workload (device-generated, resident in HBM) through the FP32 batched chain in calls of `chunk`
+    events: events/s over `steps` timed passes after the sizing pass, with its own dominant-kernel roofline."""
+    import torch
+    from crvhip import synth_gpu as SG
+    t0 = time.perf_counter()
+    Sd = SG.events(workload, n_events, seed)
+    ds = SG.device_steps(Sd)
+    del Sd
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    v, ms, avg, cnt, ex = measure("fp32", ds, ds.event, n_events, seed + 1, steps, warmup, batch, chunk, autotune)
+    tuned = ex.pop("autotune", None)
+    k = kernels_fp32(avg, cnt, cnt["n_charges"])
+    S_, W_, P_, C_, M_ = (cnt["n_steps"], cnt["n_records"], cnt["n_pulses"], cnt["n_clusters"], cnt["n_members"])
+    ncalls = ex["chunks"]
+    b_event = (STEP_B * S_ + RECORD_B * W_ + PULSE_B * P_ + CLUSTER_B * C_ + MEMBER_B * M_) / max(n_events, 1)
+    out = {"value": v, "unit": "events/s", "events": n_events, "chunk": chunk or n_events, "calls": ncalls,
+           "steps": steps, "ms_per_step": ms, "dtype": "f32", "generator": "device (torch)",
+           "generate_s": round(gen_s, 2), "options": {**batch, **(tuned["chosen"] if tuned else {})},
+           "autotune": tuned, "roofline": roofline(k, workload, chunk or n_events, per_call=ncalls),
+           "chain_roofline": {"bytes_per_event": b_event, "achieved": b_event * v / 1e9, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": b_event * v / 1e9 / HBM_PEAK_GBS},
+           "kernel_ms": {kk: vv[0] for kk, vv in k.items()},
+           "stage_ms": {kk: avg[kk] for kk in ("charges_ms", "digis_ms", "pulses_ms", "clusters_ms", "total_ms")},
+           "per_event": {kk[2:]: cnt[kk] / n_events for kk in cnt if kk.startswith("n_")}}
+    del ds
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -279,7 +318,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--events", type=int, default=1_000_000, help="events of the FP32 batched chain (headline)")
-    ap.add_argument("--fp64-events", type=int, default=100_000, help="events of the FP64 plain chain (0: skip)")
+    ap.add_argument("--fp64-events", type=int, default=1_000_000, help="events of the FP64 plain chain (0: skip)")
+    ap.add_argument("--fp64-steps", type=int, default=3, help="timed passes of the FP64 plain chain")
+    ap.add_argument("--error-events", type=int, default=100_000,
+                    help="events on which the FP32 pulses are paired with the FP64 pulses one by one")
+    ap.add_argument("--uneven-events", type=int, default=10_000_000, help="uneven-mixture leg (0: skip)")
+    ap.add_argument("--uneven-chunk", type=int, default=500_000)
+    ap.add_argument("--overlay-events", type=int, default=200_000, help="high-occupancy overlay leg (0: skip)")
+    ap.add_argument("--overlay-chunk", type=int, default=20_000)
     ap.add_argument("--workload", default="cosmic")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
@@ -295,8 +341,8 @@ def main():
     ap.add_argument("--gen", default="auto", choices=("auto", "host", "device"),
                     help="synthetic input generator: numpy on the host or torch on the device (auto: device for "
                          "uneven / overlay and for more than 2M events)")
-    ap.add_argument("--cpu-per-worker", type=int, default=0, help="oracle events per CPU worker (0: by workload)")
-    ap.add_argument("--cpu-workers", type=int, default=0)
+    ap.add_argument("--cpu-events", type=int, default=12_000, help="events of the oracle CPU baseline (0: skip)")
+    ap.add_argument("--cpu-workers", type=int, default=0, help="oracle processes (0: every core this process may use)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer three times (the reference kernel for HBM counters)")
@@ -321,15 +367,18 @@ def main():
     # inputs: numpy on the host (the oracle's own generator) for the cosmic batch, torch on the device for the large
     # uneven / overlay batches; the CPU baseline runs first, before anything touches the GPU
     gen_device = args.gen == "device" or (args.gen == "auto" and (args.workload != "cosmic" or args.events > 2_000_000))
-    per_worker = args.cpu_per_worker or {"cosmic": 150, "uneven": 40, "overlay": 4}[args.workload]
-    workers = args.cpu_workers or min(len(os.sched_getaffinity(0)), 16)
     S = None if gen_device else generate(args.workload, args.events, args.seed)
     cpu = None
-    if not args.no_cpu:
-        sample = S if S is not None else generate(args.workload, per_worker * workers, args.seed)
-        cpu = cpu_baseline(sample, per_worker * workers, args.seed + 1, per_worker, workers)
-        if S is None:
+    if not args.no_cpu and args.cpu_events:
+        # one process per core of this GPU's share of the host (16 cores per GPU on the measurement boxes)
+        workers = args.cpu_workers or min(len(os.sched_getaffinity(0)), 16)
+        per_worker = max(1, args.cpu_events // workers)
+        n_cpu = per_worker * workers
+        sample = S if S is not None and n_cpu <= args.events else generate(args.workload, n_cpu, args.seed)
+        cpu = cpu_baseline(sample, n_cpu, args.seed + 1, per_worker, workers)
+        if sample is not S:
             cpu["sample"] += " (host-generated events of the same workload; the GPU batch is generated on the device)"
+        progress(f"cpu baseline: {cpu['value']:.0f} events/s on {cpu['cores']} processes")
 
     import torch
     assert torch.cuda.is_available(), "bench needs a GPU"
@@ -346,6 +395,7 @@ def main():
     autotune = args.tile == 0 and not args.no_autotune
     v32, ms32, avg32, cnt32, ex32 = measure("fp32", ds, evkey, args.events, args.seed + 1, args.steps, args.warmup,
                                             batch, args.chunk, autotune)
+    progress(f"fp32 {args.workload} {args.events}: {v32:.0f} events/s")
     tuned = ex32.pop("autotune", None)
     if tuned:
         batch = {**batch, **tuned["chosen"]}
@@ -353,21 +403,33 @@ def main():
     fp64 = None
     if args.fp64_events and args.workload == "cosmic":
         n64 = min(args.fp64_events, args.events)
-        (_, cut), = chunks_of(evkey, args.events, n64)[:1]
-        ev64 = evkey[:cut] if isinstance(evkey, np.ndarray) else evkey[:cut].cpu().numpy().astype(np.int64)
-        ds64 = DeviceSteps(ev64, ds.counter[:cut].cpu().numpy(), ds.edep[:cut].cpu().numpy(),
-                           ds.z[:cut].cpu().numpy(), ds.t[:cut].cpu().numpy())
-        v64, ms64, avg64, cnt64, ex64 = measure("fp64", ds64, ev64, n64, args.seed + 1, args.steps, args.warmup,
-                                                batch, tables=True)
-        # the FP32 chain on exactly these events (same seed, so the same Philox streams): its count errors
-        _, _, _, c32s, ex32s = measure("fp32", ds64, ev64, n64, args.seed + 1, 1, 0, batch, tables=True)
+        if n64 == args.events:
+            ds64, ev64 = ds, evkey
+        else:
+            (_, cut), = chunks_of(evkey, args.events, n64)[:1]
+            ev64 = evkey[:cut] if isinstance(evkey, np.ndarray) else evkey[:cut].cpu().numpy().astype(np.int64)
+            ds64 = DeviceSteps(ev64, ds.counter[:cut].cpu().numpy(), ds.edep[:cut].cpu().numpy(),
+                               ds.z[:cut].cpu().numpy(), ds.t[:cut].cpu().numpy())
+        v64, ms64, avg64, cnt64, ex64 = measure("fp64", ds64, ev64, n64, args.seed + 1, args.fp64_steps, args.warmup,
+                                                batch)
+        progress(f"fp64 {n64}: {v64:.0f} events/s")
         del ds64
+        # FP32 counts against the FP64 chain on the same events (same seed, so the same Philox streams): from the
+        # headline run when it covered the same events, else from an FP32 pass over them
+        c32s = cnt32 if n64 == args.events else measure("fp32", *_sub(ds, evkey, args.events, n64), n64,
+                                                        args.seed + 1, 1, 0, batch)[3]
         fp32_err = {k: (c32s[f"n_{k}"] - cnt64[f"n_{k}"]) / max(cnt64[f"n_{k}"], 1)
                     for k in ("photons", "charges", "pulses", "clusters", "members")}
-        fp32_pulse_err = pulse_errors(ex32s.pop("pulses"), ex64.pop("pulses"))
+        # pulse by pulse on the first error-events events
+        ne = min(args.error_events, n64)
+        dse, eve = _sub(ds, evkey, args.events, ne)
+        _, _, _, _, exe64 = measure("fp64", dse, eve, ne, args.seed + 1, 1, 0, batch, tables=True)
+        _, _, _, _, exe32 = measure("fp32", dse, eve, ne, args.seed + 1, 1, 0, batch, tables=True)
+        del dse
+        fp32_pulse_err = {"events": ne, **pulse_errors(exe32.pop("pulses"), exe64.pop("pulses"))}
         k64 = kernels_fp64(avg64, cnt64)
-        fp64 = {"value": v64, "unit": "events/s", "events": n64, "ms_per_step": ms64, "dtype": "f64",
-                "speedup_fp32_over_fp64": v32 / v64, "roofline": roofline(k64),
+        fp64 = {"value": v64, "unit": "events/s", "events": n64, "steps": args.fp64_steps, "ms_per_step": ms64,
+                "dtype": "f64", "speedup_fp32_over_fp64": v32 / v64, "roofline": roofline(k64),
                 "fp32_rel_count_error": {"events": n64, "note": "photons, fired pixels (integer PE), pulses, "
                                          "clusters, cluster members: (FP32 - FP64) / FP64 on the same events", **fp32_err},
                 "fp32_pulse_error": fp32_pulse_err,
@@ -375,6 +437,19 @@ def main():
                 "stage_ms": {k: avg64[k] for k in ("charges_ms", "digis_ms", "waveforms_ms", "pulses_ms",
                                                    "clusters_ms", "total_ms")},
                 "per_event": {k[2:]: cnt64[k] / n64 for k in cnt64 if k.startswith("n_")}}
+    del ds
+    torch.cuda.empty_cache()
+    legs = {}
+    if args.workload == "cosmic":
+        if args.uneven_events:
+            legs["uneven"] = workload_leg("uneven", args.uneven_events, args.uneven_chunk, args.seed, 1, 1,
+                                          {**batch, "tile": args.tile, "sort_tasks": args.sort}, autotune)
+            progress(f"uneven {args.uneven_events}: {legs['uneven']['value']:.0f} events/s")
+        if args.overlay_events:
+            # the overlay's queue options as the autotuner picked them for this workload (tile 128, octave order)
+            legs["overlay"] = workload_leg("overlay", args.overlay_events, args.overlay_chunk, args.seed, 1, 1,
+                                           {**batch, "tile": 128, "sort_tasks": 2}, False)
+            progress(f"overlay {args.overlay_events}: {legs['overlay']['value']:.0f} events/s")
     S_, W_, P_, C_, M_ = (cnt32["n_steps"], cnt32["n_records"], cnt32["n_pulses"], cnt32["n_clusters"],
                           cnt32["n_members"])
     b_event = (STEP_B * S_ + RECORD_B * W_ + PULSE_B * P_ + CLUSTER_B * C_ + MEMBER_B * M_) / args.events
@@ -394,12 +469,26 @@ def main():
                            "unit": "GB/s", "frac": b_event * v32 / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "fp64_plain": fp64,
+        "workloads": legs,
         "stage_ms": {k: avg32[k] for k in ("charges_ms", "digis_ms", "pulses_ms", "clusters_ms", "total_ms")},
         "kernel_ms": {k: v[0] for k, v in k32.items()},
         "per_event": {k[2:]: cnt32[k] / args.events for k in cnt32 if k.startswith("n_")},
         "batched": ex32,
     }
     print(json.dumps(line))
+
+
+def _sub(ds, evkey, n_ev, n):
+    """Device steps of the first n events of a batch (copies) and their event keys."""
+    from crvhip.chain import DeviceSteps
+    (_, cut), = chunks_of(evkey, n_ev, n)[:1]
+    ev = evkey[:cut] if isinstance(evkey, np.ndarray) else evkey[:cut].cpu().numpy().astype(np.int64)
+    return DeviceSteps(ev, ds.counter[:cut].cpu().numpy(), ds.edep[:cut].cpu().numpy(), ds.z[:cut].cpu().numpy(),
+                       ds.t[:cut].cpu().numpy()), ev
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

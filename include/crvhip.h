@@ -194,6 +194,14 @@ int crv_chain_fetch(crv_ctx *ctx, crv_pulses *pulses, crv_clusters *clusters, cr
 /* Device addresses of the last chain run's waveform records: adc [n * window_samples] u16, label [n] i64. */
 int crv_chain_records(crv_ctx *ctx, const uint16_t **adc, const int64_t **label, int64_t *n);
 
+/* Bytes of device workspace the context holds now. Workspace slots grow to the largest call and are kept for the
+ * next one; no reference counterpart (the reference's JAX twins leave device memory to XLA's allocator). */
+int crv_ctx_workspace(crv_ctx *ctx, uint64_t *bytes);
+/* Return all workspace to the device (after the device has drained), e.g. before the caller allocates large
+ * buffers of its own. Forgets the last chain run's tables: crv_chain_fetch / crv_chain_records and the record
+ * pointers of crv_batch_opts are invalid until the next chain call. No reference counterpart (see above). */
+int crv_ctx_trim(crv_ctx *ctx, uint64_t *freed);
+
 /* Options and diagnostics of the FP32 batched chain. */
 typedef struct crv_batch_opts {
   int32_t grid;        /* persistent workgroups of 64 lanes of the SiPM kernels (0: all that fit on the device) */

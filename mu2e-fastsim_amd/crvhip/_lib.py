@@ -96,7 +96,7 @@ class BatchOpts(C.Structure):
 EXPORTS = ("crv_ctx_create", "crv_ctx_destroy", "crv_set_params", "crv_last_error", "crv_upload_conditions",
            "crv_upload_template", "crv_upload_photon_map", "crv_upload_geometry", "crv_charges_run", "crv_digis_run",
            "crv_reco_waveforms_run", "crv_pulses_run", "crv_clusters_run", "crv_chain_run", "crv_chain_fetch",
-           "crv_chain_records", "crv_chain_run_batched")
+           "crv_chain_records", "crv_chain_run_batched", "crv_ctx_workspace", "crv_ctx_trim")
 
 _lib = None
 
@@ -127,6 +127,8 @@ def load(path=LIB_PATH):
     lib.crv_chain_records.argtypes = [P, C.POINTER(P), C.POINTER(P), C.POINTER(i64)]
     lib.crv_chain_run_batched.argtypes = [P, C.POINTER(Steps), C.POINTER(BatchOpts), C.POINTER(Pulses),
                                           C.POINTER(Clusters), C.POINTER(Timing), P]
+    lib.crv_ctx_workspace.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.crv_ctx_trim.argtypes = [P, C.POINTER(C.c_uint64)]
     for name in EXPORTS:
         getattr(lib, name).restype = C.c_int
     _lib = lib
@@ -148,10 +150,9 @@ def params_from(cfg, seed, evkey_hi=0, precision=64):
         setattr(p, k, float(getattr(o, k)))
     p.photon_window_hi = float(cfg.window[1])
     s = cfg.sipm
-    if s.thermal_rate != 0 or s.trap0_prob != 0 or s.trap1_prob != 0:
-        raise ValueError("CRV response: unsupported SiPM processes (thermal noise, trap afterpulses)")
-    if s.pixel_id not in ("lrint", "truncate"):
-        raise ValueError(f"CRV response: SiPM pixel_id must be one of ('lrint', 'truncate'), got {s.pixel_id!r}")
+    from .contract import validate_sipm, validate_window
+    validate_sipm(s)
+    validate_window(cfg.window)
     p.n_pixels_x, p.n_pixels_y, p.pixel_lrint = s.n_pixels_x, s.n_pixels_y, int(s.pixel_id == "lrint")
     for k in ("overvoltage", "time_constant", "capacitance", "p1", "p2", "cross_talk"):
         setattr(p, k, float(getattr(s, k)))
@@ -201,6 +202,7 @@ class Context:
             self.lib.crv_last_error(None, buf, 512)
             raise CrvError(rc, buf.value.decode())
         self.cfg = cfg
+        self._pmap = None
         self.set_params(cfg, seed=0)
         self.upload_conditions(cond, cfg)
         self.upload_template(tmpl)
@@ -224,6 +226,17 @@ class Context:
             buf = C.create_string_buffer(1024)
             self.lib.crv_last_error(self.h, buf, 1024)
             raise CrvError(rc, buf.value.decode())
+
+    def workspace_bytes(self):
+        b = C.c_uint64(0)
+        self.check(self.lib.crv_ctx_workspace(self.h, C.byref(b)))
+        return int(b.value)
+
+    def trim(self):
+        """Return the context's workspace to the device; returns the bytes freed."""
+        b = C.c_uint64(0)
+        self.check(self.lib.crv_ctx_trim(self.h, C.byref(b)))
+        return int(b.value)
 
     def set_params(self, cfg, seed, evkey_hi=0, precision=64):
         self.params = params_from(cfg, seed, evkey_hi, precision)
@@ -249,8 +262,19 @@ class Context:
         t = self._h(tmpl, np.float64)
         self.check(self.lib.crv_upload_template(self.h, t.ctypes.data_as(P), len(t)))
 
-    def upload_photon_map(self, cfg):
-        pm = self._h(cfg.photon_map, np.float64)
+    def use_photon_map(self, cfg, pmap=None):
+        """Make `pmap` (or cfg.photon_map when None) the context's photon map; uploads only on a change."""
+        pm = cfg.photon_map if pmap is None else pmap
+        if self._pmap is not pm:
+            a = np.asarray(pm, float)
+            if a.ndim != 2 or a.shape[0] < 1 or a.shape[1] < 1 or not np.isfinite(a).all() or (a < 0).any() \
+                    or a.sum() <= 0:
+                raise ValueError("pmap: the photon map must be a 2-d array of finite, non-negative weights")
+            self.upload_photon_map(cfg, pm)
+
+    def upload_photon_map(self, cfg, pmap=None):
+        pm = self._h(cfg.photon_map if pmap is None else pmap, np.float64)
+        self._pmap = cfg.photon_map if pmap is None else pmap
         dead = np.zeros((41, 41), np.uint8)
         for x, y in cfg.sipm.inactive:
             dead[x, y] = 1

@@ -19,6 +19,8 @@ import time as _time
 import numpy as np
 
 from . import _lib as L
+from . import contract as CT
+from . import detector as DT
 from .config import CHAN_MULT, PERIOD, ChainConfig  # noqa: F401  (re-exported: the reference's chain names)
 from .detector import Conditions, effective_status  # noqa: F401
 
@@ -67,27 +69,64 @@ def _dev(a, dtype):
     return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to("cuda")
 
 
-def _empty(n, dtype):
+def trim():
+    """Return the device workspace of every cached context (the tables of their last chain runs included); returns
+    the bytes freed."""
+    return sum(ent[0].trim() for ent in _CTX.values())
+
+
+def _empty(n, dtype, may_trim=True):
+    """A device buffer; when the device is full, the library's cached workspace is returned first and the
+    allocation tried once more (not while a chain run's tables wait in the workspace to be fetched)."""
     torch = _torch()
-    return torch.empty(max(int(n), 1), dtype=dtype, device="cuda")
+    try:
+        return torch.empty(max(int(n), 1), dtype=dtype, device="cuda")
+    except torch.OutOfMemoryError:
+        if not may_trim or not _CTX or trim() == 0:
+            raise
+        return torch.empty(max(int(n), 1), dtype=dtype, device="cuda")
 
 
 def _host(t, n):
     return t[:n].cpu().numpy()
 
 
-def step_table(steps):
-    """(evkey, counter, edep, z, t) of a step table, sorted stably by event key (returns also the order)."""
+def step_table(steps, geom=None):
+    """(evkey, counter, edep, z, t) of a step table, sorted stably by event key (returns also the order). z is the
+    step's position along its counter from the counter centre: the `z` column, or, for the reference's CrvStep columns,
+    the midpoint of (x0, y0, z0)-(x1, y1, z1) placed in its counter by a CRVGeometry `geom` (without one, z0 / z1 are
+    taken as counter-local)."""
     ev = event_key(steps)
     counter = np.asarray(steps["counter"] if "counter" in steps else steps["bar"], np.int64)
     edep = np.asarray(steps["edep"], np.float64)
-    z = np.asarray(steps["z"], np.float64) if "z" in steps else 0.5 * (np.asarray(steps["z0"], float) +
-                                                                       np.asarray(steps["z1"], float))
+    if "z" in steps:
+        z = np.asarray(steps["z"], np.float64)
+    elif DT.is_crv_geometry(geom):
+        if len(counter) and (counter.min() < 0 or counter.max() >= len(geom.sector)):
+            raise ValueError(f"step bar {int(counter.max())} outside the geometry's {len(geom.sector)} bars")
+        col = lambda e: np.column_stack([np.asarray(steps[f"{a}{e}"], float) for a in "xyz"])
+        z = DT.step_longitudinal(geom, counter, col(0), col(1))
+    else:
+        z = 0.5 * (np.asarray(steps["z0"], float) + np.asarray(steps["z1"], float))
     t = np.asarray(steps["t"], np.float64) if "t" in steps else 0.5 * (np.asarray(steps["t0"], float) +
                                                                        np.asarray(steps["t1"], float))
     good = np.isfinite(t)
     order = np.flatnonzero(good)[np.argsort(ev[good], kind="stable")]
     return ev[order], counter[order], edep[order], z[order], t[order], order
+
+
+def _check_response_geometry(geom, cfg):
+    """The closed-form optics read every counter at +-cfg.optics.half_length: a CRVGeometry with other counter lengths
+    is refused (a SectorGeom, or None, is the synthetic detector)."""
+    if geom is None or isinstance(geom, DT.SectorGeom):
+        return
+    if not DT.is_crv_geometry(geom):
+        raise ValueError(f"geom: expected a CRVGeometry, a SectorGeom or None, got {type(geom).__name__}")
+    h = DT.bar_half_lengths(geom)
+    bad = np.flatnonzero(np.abs(h - cfg.optics.half_length) > 1e-6 * cfg.optics.half_length)
+    if len(bad):
+        raise ValueError(f"geom: counter {int(bad[0])} has half length {h[bad[0]]:g} mm; the closed-form optics "
+                         f"(cfg.optics.half_length) read counters of {cfg.optics.half_length:g} mm")
 
 
 class DeviceSteps:
@@ -113,14 +152,26 @@ class DeviceSteps:
 
 
 def charges(steps, parts, geom, cond, cfg, rng, key=None, tabs=None, pmap=None, timers=None, pdg=None, caps=None):
-    """CrvSteps -> SiPM charges: dict chan (label), time, charge, pe, photon (step row), sorted by (label, time)."""
+    """CrvSteps -> SiPM charges: dict chan (label), time, charge, pe, photon (step row), sorted by (label, time).
+
+    The closed-form optics of cfg.optics replace the reference's lookup tables and Cerenkov yield, so `tabs` and
+    `pdg` must be None (`parts` only feeds the reference's pdg lookup and is not read). `pmap` replaces
+    cfg.photon_map for this call. `geom`: None or a SectorGeom (synthetic detector), or a CRVGeometry whose counters
+    all have the optics' length (steps placed by their world coordinates)."""
     timers = timers if timers is not None else {}
+    CT.validate_chain(cfg)
     if caps is not None:
-        raise ValueError("capacities need the batched chain")
-    ev, counter, edep, z, t, order = step_table(steps)
+        raise ValueError("caps: capacities need the batched chain")
+    if tabs is not None:
+        raise ValueError("tabs: lookup-table optics are not supported; the closed-form cfg.optics apply")
+    if pdg is not None:
+        raise ValueError("pdg: the closed-form optics have no particle-dependent (Cerenkov) yield; pass pdg=None")
+    _check_response_geometry(geom, cfg)
+    ev, counter, edep, z, t, order = step_table(steps, geom)
     if len(counter) and int(counter.max()) * 4 + 3 >= CHAN_MULT:
         raise ValueError(f"channel {int(counter.max()) * 4 + 3} >= CHAN_MULT {CHAN_MULT}")
-    ctx = context(cfg, cond, geom)
+    ctx = context(cfg, cond)
+    ctx.use_photon_map(cfg, pmap)
     ds = DeviceSteps(ev, counter, edep, z, t)
     ctx.set_params(cfg, _seed(key, rng), ds.evkey_hi)
     t0 = _time.perf_counter()
@@ -285,11 +336,31 @@ def pulses(W, cond, cfg, geom=None):
     return P
 
 
+def finder_geometry(geom, counters_cfg):
+    """The finder's SectorGeom: a SectorGeom as given, one built from a CRVGeometry and its parsed counters file
+    (reference reco/coinc.py:82-110), or the synthetic detector's for None."""
+    if geom is None:
+        if counters_cfg is not None:
+            raise ValueError("counters_cfg needs a CRVGeometry geom")
+        return DT.sector_geom()
+    if isinstance(geom, DT.SectorGeom):
+        return geom
+    if DT.is_crv_geometry(geom):
+        if counters_cfg is None:
+            raise ValueError("geom: a CRVGeometry needs its counters_cfg (the parsed counters file)")
+        return DT.sector_geom_from(geom, counters_cfg)
+    raise ValueError(f"geom: expected a CRVGeometry, a SectorGeom or None, got {type(geom).__name__}")
+
+
 def clusters(W, cond, geom, counters_cfg, cfg, timers=None, detail=False, rng=None, caps=None):
-    """Reco waveforms -> pulses -> coincidence clusters (reference chain.py:296-362 output, one dict in a list)."""
-    from . import detector as DT
+    """Reco waveforms -> pulses -> coincidence clusters (reference chain.py:296-362 output, one dict in a list).
+    `rng` feeds only the reference's background-pulse model, which this chain has no setting for; `caps` (fixed
+    device shapes of the reference's JAX path) is refused."""
     timers = timers if timers is not None else {}
-    G = geom if isinstance(geom, DT.SectorGeom) else DT.sector_geom()
+    CT.validate_chain(cfg)
+    if caps is not None:
+        raise ValueError("caps: capacities need the batched chain")
+    G = finder_geometry(geom, counters_cfg)
     t0 = _time.perf_counter()
     P = pulses(W, cond, cfg, G)
     timers["pulses_s"] = timers.get("pulses_s", 0.0) + _time.perf_counter() - t0
@@ -338,25 +409,25 @@ def find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G):
     return cluster_table(b, out.n, out.n_mem)
 
 
-def _pulse_bufs(n):
+def _pulse_bufs(n, may_trim=True):
     torch = _torch()
-    b = {k: _empty(n, torch.int64) for k in ("wf", "seq", "ev")}
-    b["chan"] = _empty(n, torch.int32)
+    b = {k: _empty(n, torch.int64, may_trim=may_trim) for k in ("wf", "seq", "ev")}
+    b["chan"] = _empty(n, torch.int32, may_trim=may_trim)
     for k in L.PULSE_F64:
-        b[k] = _empty(n, torch.float64)
+        b[k] = _empty(n, torch.float64, may_trim=may_trim)
     for k in L.PULSE_U8:
-        b[k] = _empty(n, torch.uint8)
+        b[k] = _empty(n, torch.uint8, may_trim=may_trim)
     s = L.Pulses(*[L.ptr(b[k]) for k in ("wf", "seq", "ev", "chan") + L.PULSE_F64 + L.PULSE_U8], n, 0)
     return b, s
 
 
-def _cluster_bufs(nc, nm):
+def _cluster_bufs(nc, nm, may_trim=True):
     torch = _torch()
-    b = {"cl_event": _empty(nc, torch.int64), "cl_type": _empty(nc, torch.int32), "cl_poscalc": _empty(nc, torch.int32),
-         "cl_sidehits": _empty(2 * nc, torch.int32), "cl_mem_off": _empty(nc + 1, torch.int64),
-         "mem_idx": _empty(nm, torch.int64)}
+    e = lambda n, dt: _empty(n, dt, may_trim=may_trim)
+    b = {"cl_event": e(nc, torch.int64), "cl_type": e(nc, torch.int32), "cl_poscalc": e(nc, torch.int32),
+         "cl_sidehits": e(2 * nc, torch.int32), "cl_mem_off": e(nc + 1, torch.int64), "mem_idx": e(nm, torch.int64)}
     for k in L.CLUSTER_F64:
-        b[k] = _empty(3 * nc if k == "cl_pos" else 2 * nc if k in ("cl_sidepe", "cl_sidetime") else nc, torch.float64)
+        b[k] = e(3 * nc if k == "cl_pos" else 2 * nc if k in ("cl_sidepe", "cl_sidetime") else nc, torch.float64)
     s = L.Clusters(L.ptr(b["cl_event"]), L.ptr(b["cl_type"]), L.ptr(b["cl_poscalc"]), L.ptr(b["cl_sidehits"]),
                    *[L.ptr(b[k]) for k in L.CLUSTER_F64], L.ptr(b["cl_mem_off"]), nc, 0, L.ptr(b["mem_idx"]), nm, 0)
     return b, s
@@ -387,8 +458,10 @@ def run_chain(steps, cond, cfg, key=0, geom=None, batch=None, timers=None, recor
     time) order and `wf` the pulse's reco-waveform number; with records=True (FP32 chain) also "records", the ADC
     records of the clusters with digis sorted by (label, FEB tick)."""
     import ctypes
-    from . import detector as DT
-    G = geom if geom is not None else DT.sector_geom()
+    CT.validate_chain(cfg)
+    G = geom if isinstance(geom, DT.SectorGeom) else finder_geometry(None, None) if geom is None else None
+    if G is None:
+        raise ValueError("geom: run_chain takes a SectorGeom or None (build one with finder_geometry)")
     ev, counter, edep, z, t, _ = step_table(steps)
     ds = DeviceSteps(ev, counter, edep, z, t)
     ctx = context(cfg, cond, G)
@@ -409,8 +482,8 @@ def run_chain(steps, cond, cfg, key=0, geom=None, batch=None, timers=None, recor
     else:
         raise ValueError(f"precision must be fp64 or fp32, not {cfg.precision!r}")
     if rc == -3:  # CRV_E_CAPACITY: the sizes are in n; fetch again into larger buffers
-        pb, ps = _pulse_bufs(ps.n)
-        cb, cs = _cluster_bufs(cs.n, cs.n_mem)
+        pb, ps = _pulse_bufs(ps.n, may_trim=False)
+        cb, cs = _cluster_bufs(cs.n, cs.n_mem, may_trim=False)
         rc = ctx.lib.crv_chain_fetch(ctx.h, ctypes.byref(ps), ctypes.byref(cs), None)
     ctx.check(rc)
     n = ps.n

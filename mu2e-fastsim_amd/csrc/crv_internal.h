@@ -94,6 +94,12 @@ struct Ctx {
   size_t ws_fail_bytes = 0;
 };
 
+// Workspace slots: each stage takes fixed slots below WS_STAGE_SLOTS (charges 0-25, digis 30-55, pulses 60-81,
+// reco-waveform grouping 130-138, finder 90-191, reusing 130-138 whose grouping nothing reads by then; tables of the
+// chained FP64 run 300-347; batched chain 500-583). A scan on slot k keeps its temporary storage in WS_SCAN_TMP + k
+// and its widened input in WS_SCAN_WIDE + k, so a scan never lands on a table that a stage still holds.
+constexpr int WS_STAGE_SLOTS = 2000, WS_SCAN_TMP = 2000, WS_SCAN_WIDE = 4000;
+
 // workspace slot of at least `bytes` bytes (grown, never shrunk; when the device has no room, the slots listed in
 // ws_spare are freed once, after the device has drained, and the allocation is tried again)
 void *ws(Ctx *c, int slot, size_t bytes);

@@ -85,18 +85,19 @@ int upload(Ctx *c, const T *host, int64_t n, const T **dst) {
 }  // namespace
 
 int scan_i64(Ctx *c, const int64_t *in, int64_t *out, int64_t n, int slot, hipStream_t s) {
+  if (slot < 0 || slot >= WS_STAGE_SLOTS) return set_error(c, CRV_E_ARG, "scan: workspace slot out of range");
   CRV_HIP(c, hipMemsetAsync(out, 0, 8, s));
   if (n == 0) return CRV_OK;
   size_t tb = 0;
   CRV_HIP(c, hipcub::DeviceScan::InclusiveSum(nullptr, tb, in, out + 1, (int)n, s));
-  void *tmp = ws(c, slot + 200, tb);
+  void *tmp = ws(c, WS_SCAN_TMP + slot, tb);
   if (!tmp) return set_error(c, CRV_E_HIP, "scan workspace allocation failed");
   CRV_HIP(c, hipcub::DeviceScan::InclusiveSum(tmp, tb, in, out + 1, (int)n, s));
   return CRV_OK;
 }
 
 int scan_i32_to_i64(Ctx *c, const int32_t *in, int64_t *out, int64_t n, int slot, hipStream_t s) {
-  int64_t *w = (int64_t *)ws(c, slot + 400, (n + 1) * 8);
+  int64_t *w = (int64_t *)ws(c, WS_SCAN_WIDE + slot, (n + 1) * 8);
   if (!w) return set_error(c, CRV_E_HIP, "scan workspace allocation failed");
   if (n) k_i32_to_i64<<<nblocks(n, 256), 256, 0, s>>>(in, w, n);
   CRV_HIP(c, hipGetLastError());
@@ -662,6 +663,37 @@ int crv_chain_fetch(crv_ctx *ctx, crv_pulses *pulses, crv_clusters *clusters, cr
   e = e == hipSuccess ? cp(clusters->mem_idx, C.mem_idx, C.n_mem * 8) : e;
   if (e != hipSuccess) return set_error(c, CRV_E_HIP, std::string("chain fetch: ") + hipGetErrorString(e));
   CRV_HIP(c, hipStreamSynchronize(s));
+  return CRV_OK;
+}
+
+int crv_ctx_workspace(crv_ctx *ctx, uint64_t *bytes) {
+  if (int rc = guard(ctx)) return rc;
+  if (!bytes) return set_error(&ctx->c, CRV_E_ARG, "NULL argument");
+  uint64_t b = 0;
+  for (size_t i = 0; i < ctx->c.ws.size(); ++i)
+    if (ctx->c.ws[i]) b += ctx->c.ws_size[i];
+  *bytes = b;
+  return CRV_OK;
+}
+
+int crv_ctx_trim(crv_ctx *ctx, uint64_t *freed) {
+  if (int rc = guard(ctx)) return rc;
+  Ctx *c = &ctx->c;
+  CRV_HIP(c, hipDeviceSynchronize());  // no kernel of this context still reads a slot
+  uint64_t b = 0;
+  for (size_t i = 0; i < c->ws.size(); ++i)
+    if (c->ws[i]) {
+      (void)hipFree(c->ws[i]);
+      b += c->ws_size[i];
+      c->ws[i] = nullptr;
+      c->ws_size[i] = 0;
+    }
+  c->ws_spare.clear();
+  c->last_pulses = crv_pulses{};
+  c->last_clusters = crv_clusters{};
+  c->last_digis = crv_digis{};
+  c->b_rec_cap = c->b_run_cap = 0;
+  if (freed) *freed = b;
   return CRV_OK;
 }
 

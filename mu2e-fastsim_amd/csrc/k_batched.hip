@@ -1164,12 +1164,13 @@ __global__ __launch_bounds__(64) void kb_fit(crv_params p, Tables t, const uint6
 // fired-pixel log capacity per lane of each group of 64 SiPMs in queue order, from the largest photon count of the
 // group (3 factor / 4 of it + 16; factor doubles on every re-run); the group takes 64 times that
 __global__ void kb_gcaps(const int64_t *__restrict__ nph, const int32_t *__restrict__ order, int64_t L4,
-                         int64_t *__restrict__ gcap, int64_t *__restrict__ gsize, int factor) {
+                         int64_t *__restrict__ gcap, int64_t *__restrict__ gsize, int factor, int exact_order) {
   const int64_t grp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (grp >= (L4 + 63) / 64) return;
   int64_t m = 0;
-  // sorted longest first (by the count clamped to 32 bits): the group's first SiPM has its largest count
-  const int64_t first = order ? nph[order[grp * 64]] : -1;
+  // sorted longest first by the exact count (clamped to 32 bits): the group's first SiPM has its largest count; the
+  // octave orders keep a bucket's SiPMs in their given order, so there the group's largest count is looked up
+  const int64_t first = order && exact_order ? nph[order[grp * 64]] : -1;
   if (first >= 0 && first < 0xFFFFFFFFll) {
     m = first * 3 * factor / 4 + 16;
   } else {
@@ -1387,7 +1388,8 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   // everything again; a pool overflow grows the pools and runs kb_wave again (and kb_sipm, when passes reuse the log).
   for (int attempt = 0, factor = c->b_log_factor;; ++attempt, factor *= 2) {
     if (attempt > 8) return set_error(c, CRV_E_OVERFLOW, "batched chain: fired-pixel log still too short");
-    if (L4) kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, qorder, L4, gcap, gsize, factor);
+    if (L4)
+      kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, qorder, L4, gcap, gsize, factor, opt->sort_tasks == 1);
     if ((rc = scan_i64(c, gsize, goff, ngrp, 510, s))) return rc;
     if (attempt == 0) CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
     if ((rc = read_i64(c, goff + ngrp, &nlog, s))) return rc;  // (synchronises the stream)
@@ -1395,13 +1397,19 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
       return set_error(c, CRV_E_ARG, "batched chain: channel beyond the uploaded conditions");
     int64_t log_chunks = nlog;  // goff and nlog count 16-byte chunks
     cuts.assign({0, ngrp});
-    const bool pipe = opt->pipeline >= 2 && ngrp >= 2;  // two log buffers, each pass within half the budget
+    bool pipe = opt->pipeline >= 2 && ngrp >= 2;  // two log buffers, each pass within half the budget
     if (pipe || (size_t)nlog * 16 > budget) {
       hoff.resize(ngrp + 1);
       CRV_HIP(c, hipMemcpyAsync(hoff.data(), goff, (ngrp + 1) * 8, hipMemcpyDeviceToHost, s));
       CRV_HIP(c, hipStreamSynchronize(s));
-      int64_t lim = (int64_t)(budget / 16);
-      if (pipe) lim = std::min<int64_t>(lim / 2, (nlog + opt->pipeline - 1) / opt->pipeline);
+      int64_t lim = (int64_t)(budget / 16), gmax = 0;
+      for (int64_t gi = 0; gi < ngrp; ++gi) gmax = std::max<int64_t>(gmax, hoff[gi + 1] - hoff[gi]);
+      if (pipe) {
+        // a pass holds at least the largest group; when two buffers of that size exceed the budget, the passes run
+        // one after the other in one buffer (results are the same either way)
+        if (gmax > lim / 2) pipe = false;
+        else lim = std::min<int64_t>(lim / 2, std::max<int64_t>(gmax, (nlog + opt->pipeline - 1) / opt->pipeline));
+      }
       cuts.assign(1, 0);
       log_chunks = 0;
       for (int64_t g0 = 0; g0 < ngrp;) {

@@ -66,6 +66,7 @@ def find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G):
     idx = np.arange(len(ev)) - np.searchsorted(ev, ev)
     cl = {k: [] for k in ("cl_event", "cl_type", "cl_pe", "cl_start", "cl_end", "cl_slope", "cl_avgtime", "cl_poscalc")}
     mem = {"cl_pos": [], "mem_cl": [], "mem_idx": [], "mem_event": []}
+    side = {"cl_sidehits": [], "cl_sidepe": [], "cl_sidetime": []}
     ue, lo = np.unique(ev, return_index=True)
     hi = np.r_[lo[1:], len(ev)]
     for e, a, b in zip(ue, lo, hi):
@@ -75,6 +76,8 @@ def find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G):
             for k, src in (("cl_type", "sectortype"), ("cl_pe", "pe"), ("cl_start", "start"), ("cl_end", "end"),
                            ("cl_slope", "slope"), ("cl_avgtime", "avgtime"), ("cl_poscalc", "poscalc")):
                 cl[k].append(r[src])
+            for k, src in (("cl_sidehits", "sidehits"), ("cl_sidepe", "sidepe"), ("cl_sidetime", "sidetime")):
+                side[k].append(np.asarray(r[src]))
             n = len(cl["cl_event"]) - 1
             mem["cl_pos"].append(np.asarray(r["pos"], float))
             mem["mem_cl"] += [n] * len(r["pulses"])
@@ -83,6 +86,9 @@ def find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G):
     out = {k: np.asarray(v) for k, v in cl.items()}
     out.update({k: np.asarray(v, np.int64) for k, v in mem.items() if k != "cl_pos"})
     out["cl_pos"] = np.asarray(mem["cl_pos"], float).reshape(-1, 3)
+    out["cl_sidehits"] = np.asarray(side["cl_sidehits"], np.int64).reshape(-1, 2)
+    out["cl_sidepe"] = np.asarray(side["cl_sidepe"], float).reshape(-1, 2)
+    out["cl_sidetime"] = np.asarray(side["cl_sidetime"], float).reshape(-1, 2)
     return out
 
 

@@ -249,3 +249,45 @@ def test_autotuner_picks_a_candidate_and_results_do_not_change(cosmic, tmp_path)
         np.testing.assert_array_equal(r["pulses"][k], v, err_msg=k)
     for k, v in r32["clusters"].items():
         np.testing.assert_array_equal(r["clusters"][k], v, err_msg=k)
+
+
+def test_pipelined_passes_on_small_and_skewed_calls():
+    """More pipelined passes than 64-SiPM groups (a few events), and a longest-first uneven queue whose first group
+    holds more than its share of the log: the passes hold at least the largest group, results are the one-pass run's."""
+    c64, c32 = _cfgs()
+    cond, G = DT.nominal_conditions(), DT.sector_geom()
+    for S, batch in ((SY.cosmic_events(1, seed=4), {}), (SY.cosmic_events(5, seed=4), {}),
+                     (SY.uneven_events(200, seed=5), {"sort_tasks": 2}), (SY.uneven_events(200, seed=5),
+                                                                           {"sort_tasks": 1})):
+        r0 = CH.run_chain(S, cond, c32, key=4, geom=G, batch=batch)
+        r = CH.run_chain(S, cond, c32, key=4, geom=G, batch={**batch, "pipeline": 5})
+        for tab in ("pulses", "clusters"):
+            for k, v in r0[tab].items():
+                np.testing.assert_array_equal(r[tab][k], v, err_msg=f"{batch} {tab} {k}")
+
+
+def test_trim_returns_the_workspace_to_the_caller():
+    """After a call the library keeps its workspace for the next one; a caller allocation that does not fit next to
+    it trims the workspace and succeeds, and the next call gives the same tables."""
+    import torch
+    c64, c32 = _cfgs()
+    cond, G = DT.nominal_conditions(), DT.sector_geom()
+    S = SY.cosmic_events(30_000, seed=6)
+    r0 = CH.run_chain(S, cond, c32, key=5, geom=G)
+    ctx = CH.context(c32, cond, G)
+    ws = ctx.workspace_bytes()
+    assert ws > 256 << 20, ws
+    torch.cuda.empty_cache()
+    free = torch.cuda.mem_get_info()[0]
+    want = free + ws // 2
+    with pytest.raises(torch.OutOfMemoryError):
+        torch.empty(want, dtype=torch.uint8, device="cuda")
+    t = CH._empty(want, torch.uint8)
+    assert ctx.workspace_bytes() == 0
+    del t
+    torch.cuda.empty_cache()
+    r = CH.run_chain(S, cond, c32, key=5, geom=G)
+    for tab in ("pulses", "clusters"):
+        for k, v in r0[tab].items():
+            np.testing.assert_array_equal(r[tab][k], v, err_msg=f"{tab} {k}")
+    assert CH.trim() > 0 and ctx.workspace_bytes() == 0
