@@ -320,6 +320,8 @@ def main():
     ap.add_argument("--events", type=int, default=1_000_000, help="events of the FP32 batched chain (headline)")
     ap.add_argument("--fp64-events", type=int, default=1_000_000, help="events of the FP64 plain chain (0: skip)")
     ap.add_argument("--fp64-steps", type=int, default=3, help="timed passes of the FP64 plain chain")
+    ap.add_argument("--fp64-chunk", type=int, default=250_000,
+                    help="events per FP64 chain call (its charge and digi tables of 1M events exceed the device)")
     ap.add_argument("--error-events", type=int, default=100_000,
                     help="events on which the FP32 pulses are paired with the FP64 pulses one by one")
     ap.add_argument("--uneven-events", type=int, default=10_000_000, help="uneven-mixture leg (0: skip)")
@@ -411,7 +413,7 @@ def main():
             ds64 = DeviceSteps(ev64, ds.counter[:cut].cpu().numpy(), ds.edep[:cut].cpu().numpy(),
                                ds.z[:cut].cpu().numpy(), ds.t[:cut].cpu().numpy())
         v64, ms64, avg64, cnt64, ex64 = measure("fp64", ds64, ev64, n64, args.seed + 1, args.fp64_steps, args.warmup,
-                                                batch)
+                                                batch, args.fp64_chunk)
         progress(f"fp64 {n64}: {v64:.0f} events/s")
         del ds64
         # FP32 counts against the FP64 chain on the same events (same seed, so the same Philox streams): from the
@@ -428,8 +430,10 @@ def main():
         del dse
         fp32_pulse_err = {"events": ne, **pulse_errors(exe32.pop("pulses"), exe64.pop("pulses"))}
         k64 = kernels_fp64(avg64, cnt64)
-        fp64 = {"value": v64, "unit": "events/s", "events": n64, "steps": args.fp64_steps, "ms_per_step": ms64,
-                "dtype": "f64", "speedup_fp32_over_fp64": v32 / v64, "roofline": roofline(k64),
+        fp64 = {"value": v64, "unit": "events/s", "events": n64, "chunk": min(args.fp64_chunk or n64, n64),
+                "calls": ex64["chunks"], "steps": args.fp64_steps, "ms_per_step": ms64,
+                "dtype": "f64", "speedup_fp32_over_fp64": v32 / v64,
+                "roofline": roofline(k64, per_call=ex64["chunks"]),
                 "fp32_rel_count_error": {"events": n64, "note": "photons, fired pixels (integer PE), pulses, "
                                          "clusters, cluster members: (FP32 - FP64) / FP64 on the same events", **fp32_err},
                 "fp32_pulse_error": fp32_pulse_err,
