@@ -160,6 +160,17 @@ int crv_pulses_run(crv_ctx *ctx, const int64_t *ev, const int32_t *chan, const i
                    const int64_t *off, const int16_t *adc, int64_t n_wf, const double *ped, const double *area,
                    const double *height, const double *toff, crv_pulses *out, crv_stream stream);
 
+/* crv_pulses_run without any device allocation: no counterpart in the reference. For callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the caller passes inputs and
+ * outputs (capacity >= 0) as for crv_pulses_run, same results; all scratch lives in one fixed device array per GPU
+ * (about 46 MiB, held from the first context on that GPU until the process ends). Limits per call: 131,072
+ * waveforms and 524,288 samples (CRV_E_ARG beyond). Every call on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (two streams, two host threads) race; the call returns after its stream has finished. */
+int crv_pulses_run_noalloc(crv_ctx *ctx, const int64_t *ev, const int32_t *chan, const int64_t *start,
+                           const int32_t *nsamp, const int64_t *off, const int16_t *adc, int64_t n_wf, const double *ped,
+                           const double *area, const double *height, const double *toff, crv_pulses *out,
+                           crv_stream stream);
+
 /* Coincidence clusters (reference chain.clusters detail output, chain.py:325-362 + reco/coinc.py find_event). */
 typedef struct crv_clusters {
   int64_t *cl_event;

@@ -96,7 +96,8 @@ class BatchOpts(C.Structure):
 EXPORTS = ("crv_ctx_create", "crv_ctx_destroy", "crv_set_params", "crv_last_error", "crv_upload_conditions",
            "crv_upload_template", "crv_upload_photon_map", "crv_upload_geometry", "crv_charges_run", "crv_digis_run",
            "crv_reco_waveforms_run", "crv_pulses_run", "crv_clusters_run", "crv_chain_run", "crv_chain_fetch",
-           "crv_chain_records", "crv_chain_run_batched", "crv_ctx_workspace", "crv_ctx_trim")
+           "crv_chain_records", "crv_chain_run_batched", "crv_ctx_workspace", "crv_ctx_trim",
+           "crv_pulses_run_noalloc")
 
 _lib = None
 
@@ -121,6 +122,7 @@ def load(path=LIB_PATH):
     lib.crv_digis_run.argtypes = [P, P, P, P, i64, C.POINTER(Digis), P]
     lib.crv_reco_waveforms_run.argtypes = [P, P, P, P, i64, C.POINTER(Waveforms), P]
     lib.crv_pulses_run.argtypes = [P] + [P] * 6 + [i64] + [P] * 4 + [C.POINTER(Pulses), P]
+    lib.crv_pulses_run_noalloc.argtypes = [P] + [P] * 6 + [i64] + [P] * 4 + [C.POINTER(Pulses), P]
     lib.crv_clusters_run.argtypes = [P] + [P] * 6 + [i64, C.POINTER(Clusters), P]
     lib.crv_chain_run.argtypes = [P, C.POINTER(Steps), C.POINTER(Pulses), C.POINTER(Clusters), C.POINTER(Timing), P]
     lib.crv_chain_fetch.argtypes = [P, C.POINTER(Pulses), C.POINTER(Clusters), P]

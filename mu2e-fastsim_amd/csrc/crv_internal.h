@@ -92,6 +92,7 @@ struct Ctx {
   std::vector<int> ws_spare;  // slots the running call no longer reads: freed to make room when an allocation fails
   int ws_fail_slot = -1;                  // the workspace request that failed last
   size_t ws_fail_bytes = 0;
+  uint8_t *scratch = nullptr;  // this device's fixed scratch array of the allocation-free pulse entry point
 };
 
 // Workspace slots: each stage takes fixed slots below WS_STAGE_SLOTS (charges 0-25, digis 30-55, pulses 60-81,
@@ -136,6 +137,12 @@ int waveforms_stage(Ctx *c, const int64_t *label, const int64_t *starttdc, const
 int pulses_stage(Ctx *c, const int64_t *ev, const int32_t *chan, const int64_t *start, const int32_t *nsamp,
                  const int64_t *off, const int16_t *adc, int64_t n_wf, const double *ped, const double *area,
                  const double *height, const double *toff, crv_pulses *out, hipStream_t s);
+// the same on the device's fixed scratch array (no allocation; sizes bounded, see k_pulse.hip); noalloc_setup looks
+// up the array's address on the current device (once per device) and keeps it in the context
+int pulses_stage_noalloc(Ctx *c, const int64_t *ev, const int32_t *chan, const int64_t *start, const int32_t *nsamp,
+                         const int64_t *off, const int16_t *adc, int64_t n_wf, const double *ped, const double *area,
+                         const double *height, const double *toff, crv_pulses *out, hipStream_t s);
+int noalloc_setup(Ctx *c);
 int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t *sipm, const double *time,
                    const double *pe, const int64_t *seq, int64_t n, crv_clusters *out, hipStream_t s);
 // the FP32 batched chain; pulses and clusters end in last_pulses / last_clusters

@@ -261,6 +261,11 @@ int crv_ctx_create(int device, crv_ctx **out) {
       (void)crv_ctx_destroy(ctx);
       return set_error(nullptr, CRV_E_HIP, "hipStreamCreate failed");
     }
+  if (noalloc_setup(&ctx->c) != CRV_OK) {
+    const std::string m = ctx->c.err;
+    (void)crv_ctx_destroy(ctx);
+    return set_error(nullptr, CRV_E_HIP, m);
+  }
   *out = ctx;
   return CRV_OK;
 }
@@ -494,6 +499,20 @@ int crv_pulses_run(crv_ctx *ctx, const int64_t *ev, const int32_t *chan, const i
     return set_error(c, CRV_E_ARG, "NULL argument");
   if (!c->have_params || !c->t.status) return set_error(c, CRV_E_STATE, "parameters or conditions missing");
   return pulses_stage(c, ev, chan, start, nsamp, off, adc, n_wf, ped, area, height, toff, out, (hipStream_t)stream);
+}
+
+int crv_pulses_run_noalloc(crv_ctx *ctx, const int64_t *ev, const int32_t *chan, const int64_t *start,
+                           const int32_t *nsamp, const int64_t *off, const int16_t *adc, int64_t n_wf, const double *ped,
+                           const double *area, const double *height, const double *toff, crv_pulses *out,
+                           crv_stream stream) {
+  if (int rc = guard(ctx)) return rc;
+  Ctx *c = &ctx->c;
+  if (!out || (n_wf && (!start || !nsamp || !off || !adc || !ped || !area || !height || !toff)))
+    return set_error(c, CRV_E_ARG, "NULL argument");
+  if (n_wf < 0) return set_error(c, CRV_E_ARG, "pulses (no-alloc): negative waveform count");
+  if (!c->have_params || !c->t.status) return set_error(c, CRV_E_STATE, "parameters or conditions missing");
+  return pulses_stage_noalloc(c, ev, chan, start, nsamp, off, adc, n_wf, ped, area, height, toff, out,
+                              (hipStream_t)stream);
 }
 
 int crv_clusters_run(crv_ctx *ctx, const int64_t *ev, const int32_t *bar, const int32_t *sipm, const double *time,

@@ -10,6 +10,9 @@
 // elimination with partial pivoting; eigenvalues by six cyclic Jacobi sweeps.
 #include <hipcub/hipcub.hpp>
 
+#include <map>
+#include <mutex>
+
 #include "crv_internal.h"
 
 namespace crv {
@@ -401,47 +404,135 @@ __global__ void k_pulses_out(int64_t nwf, const int64_t *__restrict__ ev, const 
 
 }  // namespace
 
-int pulses_stage(Ctx *c, const int64_t *ev, const int32_t *chan, const int64_t *start, const int32_t *nsamp,
-                 const int64_t *off, const int16_t *adc, int64_t n_wf, const double *ped, const double *area,
-                 const double *height, const double *toff, crv_pulses *out, hipStream_t s) {
+namespace {
+
+// Workspace of the allocating entry points: the context's slots (grown on demand).
+struct WsAlloc {
+  Ctx *c;
+  void *operator()(int slot, size_t bytes) { return ws(c, slot, bytes); }
+  int check(int64_t, int64_t) { return CRV_OK; }
+  int fail() { return ws_fail(c); }
+  int scan64(const int64_t *in, int64_t *out, int64_t n, int slot, hipStream_t s) {
+    return scan_i64(c, in, out, n, slot, s);
+  }
+  int scan32(const int32_t *in, int64_t *out, int64_t n, int slot, hipStream_t s) {
+    return scan_i32_to_i64(c, in, out, n, slot, s);
+  }
+  static constexpr bool internal_outputs = true;
+};
+
+// ---- scratch of the allocation-free entry point (crv_pulses_run_noalloc) ----
+// One fixed device array per GPU, carved in the same order by every call; the sizes below bound one call.
+constexpr int64_t NA_MAX_WF = 1 << 17;       // reco waveforms per call
+constexpr int64_t NA_MAX_SAMPLES = 1 << 19;  // samples per call (all waveforms together)
+constexpr int64_t NA_MAX_SLOTS = NA_MAX_SAMPLES / 2 + NA_MAX_WF;  // pulse slots: nsamp / 2 + 1 per waveform
+constexpr size_t NA_SCAN_TMP = 256 << 10;    // bytes for the scan temporaries (checked at set-up)
+constexpr size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t NA_BYTES = al256((NA_MAX_SAMPLES + 1) * 8)          // pedestal-subtracted samples
+                            + 4 * al256((NA_MAX_WF + 1) * 8)          // slot counts, slot offsets, pulse offsets, widened
+                            + al256((NA_MAX_WF + 1) * 4)              // pulses per waveform
+                            + 12 * al256((NA_MAX_SLOTS + 1) * 8)      // slot seq + 11 FP64 columns
+                            + 3 * al256(NA_MAX_SLOTS + 8)             // slot flags
+                            + NA_SCAN_TMP;
+
+static __device__ __align__(256) uint8_t scratch[NA_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int, uint8_t *> g_scratch;  // device -> address of that device's copy of `scratch`
+
+__global__ void k_zero_first(int64_t *__restrict__ out) { *out = 0; }
+
+__global__ void k_widen32(const int32_t *__restrict__ in, int64_t *__restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+struct FixedAlloc {
+  Ctx *c;
+  uint8_t *base;
+  size_t used = 0;
+  void *operator()(int, size_t bytes) {
+    const size_t b = al256(bytes ? bytes : 8);
+    if (used + b > NA_BYTES) return nullptr;
+    void *p = base + used;
+    used += b;
+    return p;
+  }
+  int check(int64_t n_wf, int64_t n_samples) {
+    if (n_wf > NA_MAX_WF || n_samples > NA_MAX_SAMPLES)
+      return set_error(c, CRV_E_ARG, "pulses (no-alloc): " + std::to_string(n_wf) + " waveforms / " +
+                                         std::to_string(n_samples) + " samples exceed the scratch limit of " +
+                                         std::to_string(NA_MAX_WF) + " / " + std::to_string(NA_MAX_SAMPLES));
+    return CRV_OK;
+  }
+  int fail() { return set_error(c, CRV_E_STATE, "pulses (no-alloc): scratch exhausted"); }
+  int scan64(const int64_t *in, int64_t *out, int64_t n, int, hipStream_t s) {
+    k_zero_first<<<1, 1, 0, s>>>(out);
+    CRV_HIP(c, hipGetLastError());
+    if (n == 0) return CRV_OK;
+    size_t tb = 0;
+    CRV_HIP(c, hipcub::DeviceScan::InclusiveSum(nullptr, tb, in, out + 1, (int)n, s));
+    void *tmp = (*this)(0, tb);
+    if (!tmp || tb > NA_SCAN_TMP) return fail();
+    CRV_HIP(c, hipcub::DeviceScan::InclusiveSum(tmp, tb, in, out + 1, (int)n, s));
+    return CRV_OK;
+  }
+  int scan32(const int32_t *in, int64_t *out, int64_t n, int slot, hipStream_t s) {
+    int64_t *w = (int64_t *)(*this)(0, (n + 1) * 8);
+    if (!w) return fail();
+    if (n) k_widen32<<<nblocks(n, 256), 256, 0, s>>>(in, w, n);
+    CRV_HIP(c, hipGetLastError());
+    return scan64(w, out, n, slot, s);
+  }
+  static constexpr bool internal_outputs = false;
+};
+
+template <class Alloc>
+int pulses_impl(Ctx *c, Alloc &A, const int64_t *ev, const int32_t *chan, const int64_t *start, const int32_t *nsamp,
+                const int64_t *off, const int16_t *adc, int64_t n_wf, const double *ped, const double *area,
+                const double *height, const double *toff, crv_pulses *out, hipStream_t s) {
   if (n_wf == 0) {
     out->n = 0;
     return CRV_OK;
   }
-  int64_t nsamp_total = 0;
-  int rc = read_i64(c, off + n_wf, &nsamp_total, s);
+  int rc = A.check(n_wf, 0);
   if (rc) return rc;
-  double *yscr = (double *)ws(c, 60, (nsamp_total + 1) * 8);
-  int64_t *cap = (int64_t *)ws(c, 61, (n_wf + 1) * 8);
-  int64_t *poff = (int64_t *)ws(c, 62, (n_wf + 1) * 8);
-  int32_t *npul = (int32_t *)ws(c, 63, (n_wf + 1) * 4);
-  int64_t *pout = (int64_t *)ws(c, 64, (n_wf + 1) * 8);
-  if (!yscr || !cap || !poff || !npul || !pout) return ws_fail(c);
+  int64_t nsamp_total = 0;
+  if ((rc = read_i64(c, off + n_wf, &nsamp_total, s))) return rc;
+  if ((rc = A.check(n_wf, nsamp_total))) return rc;
+  double *yscr = (double *)A(60, (nsamp_total + 1) * 8);
+  int64_t *cap = (int64_t *)A(61, (n_wf + 1) * 8);
+  int64_t *poff = (int64_t *)A(62, (n_wf + 1) * 8);
+  int32_t *npul = (int32_t *)A(63, (n_wf + 1) * 4);
+  int64_t *pout = (int64_t *)A(64, (n_wf + 1) * 8);
+  if (!yscr || !cap || !poff || !npul || !pout) return A.fail();
   k_pulse_caps<<<nblocks(n_wf, 256), 256, 0, s>>>(n_wf, nsamp, cap);
-  if ((rc = scan_i64(c, cap, poff, n_wf, 65, s))) return rc;
+  CRV_HIP(c, hipGetLastError());
+  if ((rc = A.scan64(cap, poff, n_wf, 65, s))) return rc;
   int64_t ncap = 0;
   if ((rc = read_i64(c, poff + n_wf, &ncap, s))) return rc;
   PulseSlots sl;
-  sl.seq = (int64_t *)ws(c, 66, (ncap + 1) * 8);
+  sl.seq = (int64_t *)A(66, (ncap + 1) * 8);
   double **dd[] = {&sl.pe, &sl.peph, &sl.time, &sl.height, &sl.beta, &sl.chi2, &sl.letime, &sl.ped, &sl.fa, &sl.fmu, &sl.fb};
   for (int i = 0; i < 11; ++i) {
-    *dd[i] = (double *)ws(c, 67 + i, (ncap + 1) * 8);
-    if (!*dd[i]) return ws_fail(c);
+    *dd[i] = (double *)A(67 + i, (ncap + 1) * 8);
+    if (!*dd[i]) return A.fail();
   }
-  sl.failed = (uint8_t *)ws(c, 78, ncap + 8);
-  sl.zerondf = (uint8_t *)ws(c, 79, ncap + 8);
-  sl.pedfromdb = (uint8_t *)ws(c, 80, ncap + 8);
-  if (!sl.seq || !sl.failed || !sl.zerondf || !sl.pedfromdb) return ws_fail(c);
+  sl.failed = (uint8_t *)A(78, ncap + 8);
+  sl.zerondf = (uint8_t *)A(79, ncap + 8);
+  sl.pedfromdb = (uint8_t *)A(80, ncap + 8);
+  if (!sl.seq || !sl.failed || !sl.zerondf || !sl.pedfromdb) return A.fail();
   kmark(c, 2, 0, s);
   k_pulses<<<nblocks(n_wf, 64), 64, 0, s>>>(c->p, c->t, n_wf, chan, start, nsamp, off, adc, ped, area, height, toff,
                                             yscr, poff, sl, npul);
   CRV_HIP(c, hipGetLastError());
   kmark(c, 2, 1, s);
-  if ((rc = scan_i32_to_i64(c, npul, pout, n_wf, 81, s))) return rc;
+  if ((rc = A.scan32(npul, pout, n_wf, 81, s))) return rc;
   int64_t np = 0;
   if ((rc = read_i64(c, pout + n_wf, &np, s))) return rc;
   out->n = np;
   if (out->capacity < 0) {  // internal outputs of the chained run
+    if (!Alloc::internal_outputs) return set_error(c, CRV_E_ARG, "pulses (no-alloc): the caller passes the outputs");
     out->wf = (int64_t *)ws(c, 320, np * 8 + 8);
     out->seq = (int64_t *)ws(c, 321, np * 8 + 8);
     out->ev = (int64_t *)ws(c, 322, np * 8 + 8);
@@ -459,6 +550,41 @@ int pulses_stage(Ctx *c, const int64_t *ev, const int32_t *chan, const int64_t *
   CRV_HIP(c, hipGetLastError());
   CRV_HIP(c, hipStreamSynchronize(s));
   return CRV_OK;
+}
+
+}  // namespace
+
+int pulses_stage(Ctx *c, const int64_t *ev, const int32_t *chan, const int64_t *start, const int32_t *nsamp,
+                 const int64_t *off, const int16_t *adc, int64_t n_wf, const double *ped, const double *area,
+                 const double *height, const double *toff, crv_pulses *out, hipStream_t s) {
+  WsAlloc A{c};
+  return pulses_impl(c, A, ev, chan, start, nsamp, off, adc, n_wf, ped, area, height, toff, out, s);
+}
+
+int noalloc_setup(Ctx *c) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(c->device);
+  if (it == g_scratch.end()) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(c, CRV_E_HIP, std::string("no-alloc scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    size_t tb = 0;  // the scan temporaries of the largest call must fit their share
+    CRV_HIP(c, hipcub::DeviceScan::InclusiveSum(nullptr, tb, (const int64_t *)nullptr, (int64_t *)nullptr,
+                                               (int)NA_MAX_WF, (hipStream_t)0));
+    if (tb > NA_SCAN_TMP) return set_error(c, CRV_E_STATE, "no-alloc scratch: scan temporaries do not fit");
+    it = g_scratch.emplace(c->device, (uint8_t *)p).first;
+  }
+  c->scratch = it->second;
+  return CRV_OK;
+}
+
+int pulses_stage_noalloc(Ctx *c, const int64_t *ev, const int32_t *chan, const int64_t *start, const int32_t *nsamp,
+                         const int64_t *off, const int16_t *adc, int64_t n_wf, const double *ped, const double *area,
+                         const double *height, const double *toff, crv_pulses *out, hipStream_t s) {
+  if (!c->scratch) return set_error(c, CRV_E_STATE, "pulses (no-alloc): scratch not set up on this device");
+  FixedAlloc A{c, c->scratch};
+  return pulses_impl(c, A, ev, chan, start, nsamp, off, adc, n_wf, ped, area, height, toff, out, s);
 }
 
 }  // namespace crv
