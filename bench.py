@@ -135,12 +135,15 @@ def chunks_of(evkey, n_ev, chunk):
 
 
 AUTOTUNE_EVENTS = 200_000  # events of the sample the work-queue autotuner runs its candidates on
+GRAPH_MARGIN = 0.3         # headroom of a captured graph's tables over the first chunk's, per step
 
 
-def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, autotune=False, tables=False):
+def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, autotune=False, tables=False,
+            graph=False):
     """Time `steps` chain passes over the batch (chunk by chunk, steps resident in HBM); returns (events/s, ms/step,
     mean timing fields summed over chunks, last counts summed over chunks, extra). With `tables` (one chunk) extra
-    holds the last pass's pulse columns ev, chan, time, pe on the host."""
+    holds the last pass's pulse columns ev, chan, time, pe on the host. With `graph` (FP32) every chunk replays one
+    captured graph of the chain (crv_graph_launch), sized on the first chunk for the largest chunk's steps."""
     import torch
     from crvhip import _lib as L, config as CF, detector as DT, template as TP
     cfg = CF.ChainConfig(precision=precision)
@@ -168,8 +171,17 @@ def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, aut
         if PROGRESS:
             print(f"[bench] autotune: {chosen}", file=sys.stderr, flush=True)
     opts = L.BatchOpts(**batch)
+    G = None
+    if graph and precision == "fp32":
+        if not opts.tile:
+            opts.tile = 64
+        t0 = time.perf_counter()
+        G = L.Graph(ctx, structs[0], max(st.n for st in structs), opts, GRAPH_MARGIN)
+        graph_s = time.perf_counter() - t0
 
     def call(st, out):
+        if G is not None:
+            return G.launch(st, out.P, out.C, tm, stream)
         if precision == "fp32":
             return ctx.lib.crv_chain_run_batched(ctx.h, ctypes.byref(st), ctypes.byref(opts), ctypes.byref(out.P),
                                                  ctypes.byref(out.C), ctypes.byref(tm), stream)
@@ -211,6 +223,9 @@ def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, aut
     extra["chunks"] = len(structs)
     if tuned:
         extra["autotune"] = tuned
+    if G is not None:
+        extra["graph"] = {**G.stats(), "create_s": round(graph_s, 2), "margin": GRAPH_MARGIN}
+        G.close()
     if tables:
         assert len(structs) == 1, "pulse tables of a one-chunk run only"
         extra["pulses"] = {k: out.keep[0][k][:out.P.n].cpu().numpy() for k in ("ev", "chan", "time", "pe")}
@@ -281,7 +296,7 @@ def roofline(kernels, workload=None, events=0, per_call=1):
             "bytes_per_launch": kbytes / per_call, "ms_per_launch": kms / per_call}
 
 
-def workload_leg(workload, n_events, chunk, seed, steps, warmup, batch, autotune):
+def workload_leg(workload, n_events, chunk, seed, steps, warmup, batch, autotune, graph=False):
     """One extra workload (device-generated, resident in HBM) through the FP32 batched chain in calls of `chunk`
     events: events/s over `steps` timed passes after the sizing pass, with its own dominant-kernel roofline."""
     import torch
@@ -292,7 +307,8 @@ def workload_leg(workload, n_events, chunk, seed, steps, warmup, batch, autotune
     del Sd
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
-    v, ms, avg, cnt, ex = measure("fp32", ds, ds.event, n_events, seed + 1, steps, warmup, batch, chunk, autotune)
+    v, ms, avg, cnt, ex = measure("fp32", ds, ds.event, n_events, seed + 1, steps, warmup, batch, chunk, autotune,
+                                  graph=graph)
     tuned = ex.pop("autotune", None)
     k = kernels_fp32(avg, cnt, cnt["n_charges"])
     S_, W_, P_, C_, M_ = (cnt["n_steps"], cnt["n_records"], cnt["n_pulses"], cnt["n_clusters"], cnt["n_members"])
@@ -301,7 +317,8 @@ def workload_leg(workload, n_events, chunk, seed, steps, warmup, batch, autotune
     out = {"value": v, "unit": "events/s", "events": n_events, "chunk": chunk or n_events, "calls": ncalls,
            "steps": steps, "ms_per_step": ms, "dtype": "f32", "generator": "device (torch)",
            "generate_s": round(gen_s, 2), "options": {**batch, **(tuned["chosen"] if tuned else {})},
-           "autotune": tuned, "roofline": roofline(k, workload, chunk or n_events, per_call=ncalls),
+           "autotune": tuned, "graph": ex.get("graph"),
+           "roofline": roofline(k, workload, chunk or n_events, per_call=ncalls),
            "chain_roofline": {"bytes_per_event": b_event, "achieved": b_event * v / 1e9, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": b_event * v / 1e9 / HBM_PEAK_GBS},
            "kernel_ms": {kk: vv[0] for kk, vv in k.items()},
@@ -326,9 +343,12 @@ def main():
                     help="events on which the FP32 pulses are paired with the FP64 pulses one by one")
     ap.add_argument("--uneven-events", type=int, default=10_000_000, help="uneven-mixture leg (0: skip)")
     ap.add_argument("--uneven-chunk", type=int, default=500_000)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="uneven leg: run every chunk through crv_chain_run_batched instead of one captured graph")
     ap.add_argument("--overlay-events", type=int, default=200_000, help="high-occupancy overlay leg (0: skip)")
     ap.add_argument("--overlay-chunk", type=int, default=20_000)
     ap.add_argument("--workload", default="cosmic")
+    ap.add_argument("--graph", action="store_true", help="headline: replay one captured graph per chunk")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
@@ -396,7 +416,7 @@ def main():
     batch = {"grid": args.grid, "tile": args.tile, "sort_tasks": args.sort, "pipeline": args.pipeline}
     autotune = args.tile == 0 and not args.no_autotune
     v32, ms32, avg32, cnt32, ex32 = measure("fp32", ds, evkey, args.events, args.seed + 1, args.steps, args.warmup,
-                                            batch, args.chunk, autotune)
+                                            batch, args.chunk, autotune, graph=args.graph)
     progress(f"fp32 {args.workload} {args.events}: {v32:.0f} events/s")
     tuned = ex32.pop("autotune", None)
     if tuned:
@@ -447,7 +467,8 @@ def main():
     if args.workload == "cosmic":
         if args.uneven_events:
             legs["uneven"] = workload_leg("uneven", args.uneven_events, args.uneven_chunk, args.seed, 1, 1,
-                                          {**batch, "tile": args.tile, "sort_tasks": args.sort}, autotune)
+                                          {**batch, "tile": args.tile, "sort_tasks": args.sort}, autotune,
+                                          graph=not args.no_graph)
             progress(f"uneven {args.uneven_events}: {legs['uneven']['value']:.0f} events/s")
         if args.overlay_events:
             # the overlay's queue options as the autotuner picked them for this workload (tile 128, octave order)

@@ -245,6 +245,34 @@ typedef struct crv_batch_opts {
 int crv_chain_run_batched(crv_ctx *ctx, const crv_steps *steps, crv_batch_opts *opts, crv_pulses *pulses,
                           crv_clusters *clusters, crv_timing *timing, crv_stream stream);
 
+/* The FP32 batched chain captured into one HIP graph (a persistent, fixed-capacity launch sequence without host
+ * synchronisation): the step grouping, the photon / SiPM / waveform work-queue kernels, the fits, the pulse table and
+ * the coincidence finder of crv_chain_run_batched, for calls of at most max_steps steps. The reference has no
+ * counterpart (its JAX twins dispatch the same chain asynchronously, reco/chain_jax.py:127-210, with fixed shapes,
+ * response/device.py:110-180).
+ *
+ * crv_graph_create runs the chain once uncaptured on `sample` (a call typical of the workload) for the table sizes
+ * per step, sizes every buffer for max_steps steps with (1 + margin) headroom, runs the fixed-capacity sequence
+ * eagerly on the sample (it must equal the uncaptured chain's table sizes), then captures it. opts must give the
+ * work-queue tile and sort order (e.g. from the autotuner); log passes and pipelining do not apply. The graph holds
+ * its own device buffers until crv_graph_destroy. */
+typedef struct crv_graph crv_graph;
+int crv_graph_create(crv_ctx *ctx, const crv_steps *sample, int64_t max_steps, const crv_batch_opts *opts,
+                     double margin, crv_graph **out);
+/* Replay for `steps` (copied into the graph's input buffers on `stream`, then one hipGraphLaunch); outputs as
+ * crv_chain_run_batched's (same tables, CRV_E_CAPACITY convention; crv_graph_fetch copies them again). A call that
+ * exceeds a capacity (more steps than max_steps, or a table larger than planned) runs uncaptured through
+ * crv_chain_run_batched instead, with the same results; crv_graph_stats counts those. The context's parameters must
+ * be those of the capture (CRV_E_STATE otherwise); conditions, template and geometry uploads after the capture are
+ * not seen by it. */
+int crv_graph_launch(crv_graph *g, const crv_steps *steps, crv_pulses *pulses, crv_clusters *clusters,
+                     crv_timing *timing, crv_stream stream);
+int crv_graph_fetch(crv_graph *g, crv_pulses *pulses, crv_clusters *clusters, crv_stream stream);
+/* launches, uncaptured fallbacks, and capacities[7]: steps, log chunks, records, runs, pulses, pulse slots per
+ * waveform, whether the last launch fell back */
+int crv_graph_stats(crv_graph *g, int64_t *launches, int64_t *fallbacks, int64_t *capacities);
+int crv_graph_destroy(crv_graph *g);
+
 #ifdef __cplusplus
 }
 #endif

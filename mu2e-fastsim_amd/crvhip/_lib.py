@@ -97,7 +97,8 @@ EXPORTS = ("crv_ctx_create", "crv_ctx_destroy", "crv_set_params", "crv_last_erro
            "crv_upload_template", "crv_upload_photon_map", "crv_upload_geometry", "crv_charges_run", "crv_digis_run",
            "crv_reco_waveforms_run", "crv_pulses_run", "crv_clusters_run", "crv_chain_run", "crv_chain_fetch",
            "crv_chain_records", "crv_chain_run_batched", "crv_ctx_workspace", "crv_ctx_trim",
-           "crv_pulses_run_noalloc")
+           "crv_pulses_run_noalloc", "crv_graph_create", "crv_graph_launch", "crv_graph_fetch", "crv_graph_stats",
+           "crv_graph_destroy")
 
 _lib = None
 
@@ -129,6 +130,12 @@ def load(path=LIB_PATH):
     lib.crv_chain_records.argtypes = [P, C.POINTER(P), C.POINTER(P), C.POINTER(i64)]
     lib.crv_chain_run_batched.argtypes = [P, C.POINTER(Steps), C.POINTER(BatchOpts), C.POINTER(Pulses),
                                           C.POINTER(Clusters), C.POINTER(Timing), P]
+    lib.crv_graph_create.argtypes = [P, C.POINTER(Steps), i64, C.POINTER(BatchOpts), C.c_double, C.POINTER(P)]
+    lib.crv_graph_launch.argtypes = [P, C.POINTER(Steps), C.POINTER(Pulses), C.POINTER(Clusters), C.POINTER(Timing),
+                                     P]
+    lib.crv_graph_fetch.argtypes = [P, C.POINTER(Pulses), C.POINTER(Clusters), P]
+    lib.crv_graph_stats.argtypes = [P, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
+    lib.crv_graph_destroy.argtypes = [P]
     lib.crv_ctx_workspace.argtypes = [P, C.POINTER(C.c_uint64)]
     lib.crv_ctx_trim.argtypes = [P, C.POINTER(C.c_uint64)]
     for name in EXPORTS:
@@ -301,3 +308,43 @@ class Context:
                  has, self._h(cuts, np.float64)]
         self.check(self.lib.crv_upload_geometry(self.h, nb, *[a.ctypes.data_as(P) for a in arrs], ns,
                                                 *[a.ctypes.data_as(P) for a in sarrs]))
+
+
+class Graph:
+    """A captured FP32 batched chain (crv_graph_create) on a context, for calls of at most max_steps steps."""
+
+    def __init__(self, ctx, sample, max_steps, opts, margin=0.25):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.check(ctx.lib.crv_graph_create(ctx.h, C.byref(sample), int(max_steps), C.byref(opts), float(margin),
+                                           C.byref(self.h)))
+
+    def launch(self, steps, pulses, clusters, timing=None, stream=None):
+        """crv_graph_launch; returns its status (0 or CRV_E_CAPACITY with the sizes in the outputs' n)."""
+        rc = self.ctx.lib.crv_graph_launch(self.h, C.byref(steps), C.byref(pulses), C.byref(clusters),
+                                           C.byref(timing) if timing is not None else None, stream)
+        if rc not in (CRV_OK, CRV_E_CAPACITY):
+            self.ctx.check(rc)
+        return rc
+
+    def fetch(self, pulses, clusters, stream=None):
+        self.ctx.check(self.ctx.lib.crv_graph_fetch(self.h, C.byref(pulses), C.byref(clusters), stream))
+
+    def stats(self):
+        la, fb = i64(0), i64(0)
+        caps = (i64 * 7)()
+        self.ctx.check(self.ctx.lib.crv_graph_stats(self.h, C.byref(la), C.byref(fb), caps))
+        names = ("steps", "log_chunks", "records", "runs", "pulses", "pulse_slots", "last_fell_back")
+        return {"launches": la.value, "fallbacks": fb.value, "capacities": dict(zip(names, list(caps)))}
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.crv_graph_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

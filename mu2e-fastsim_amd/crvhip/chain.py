@@ -486,18 +486,7 @@ def run_chain(steps, cond, cfg, key=0, geom=None, batch=None, timers=None, recor
         cb, cs = _cluster_bufs(cs.n, cs.n_mem, may_trim=False)
         rc = ctx.lib.crv_chain_fetch(ctx.h, ctypes.byref(ps), ctypes.byref(cs), None)
     ctx.check(rc)
-    n = ps.n
-    P = {k: _host(pb[k], n) for k in ("wf", "seq", "ev", "chan") + L.PULSE_F64 + L.PULSE_U8}
-    for k in L.PULSE_U8:
-        P[k] = P[k].astype(bool)
-    P["bar"], P["sipm"] = P["chan"] // 4, P["chan"] % 4
-    timing = {f: getattr(tm, f) for f, _ in L.Timing._fields_}
-    if cfg.precision == "fp32":
-        timing.update(n_records=opts.n_records, n_waveforms=opts.n_waveforms, overlong=opts.overlong,
-                      capped=opts.capped, pulse_slots_used=opts.pulse_slots_used, log_passes=opts.log_passes)
-    if timers is not None:
-        timers.update(timing)
-    out = {"pulses": P, "clusters": cluster_table(cb, cs.n, cs.n_mem), "timing": timing}
+    out = _tables(pb, ps, cb, cs, tm, opts if cfg.precision == "fp32" else None, timers)
     if records and cfg.precision == "fp32":  # the records of the clusters with digis, by (label, tick)
         nr, Wn = opts.n_records, cfg.window_samples
         lab, tick = _fetch(opts.record_label, nr, np.int64), _fetch(opts.record_tick, nr, np.int32)
@@ -506,6 +495,66 @@ def run_chain(steps, cond, cfg, key=0, geom=None, batch=None, timers=None, recor
         o = keep[np.lexsort((tick[keep], lab[keep]))]
         out["records"] = {"label": lab[o], "tick": tick[o], "adc": adc[o]}
     return out
+
+
+def _tables(pb, ps, cb, cs, tm, opts=None, timers=None):
+    n = ps.n
+    P = {k: _host(pb[k], n) for k in ("wf", "seq", "ev", "chan") + L.PULSE_F64 + L.PULSE_U8}
+    for k in L.PULSE_U8:
+        P[k] = P[k].astype(bool)
+    P["bar"], P["sipm"] = P["chan"] // 4, P["chan"] % 4
+    timing = {f: getattr(tm, f) for f, _ in L.Timing._fields_}
+    if opts is not None:
+        timing.update(n_records=opts.n_records, n_waveforms=opts.n_waveforms, overlong=opts.overlong,
+                      capped=opts.capped, pulse_slots_used=opts.pulse_slots_used, log_passes=opts.log_passes)
+    if timers is not None:
+        timers.update(timing)
+    return {"pulses": P, "clusters": cluster_table(cb, cs.n, cs.n_mem), "timing": timing}
+
+
+class ChainGraph:
+    """The FP32 batched chain captured into one HIP graph (crv_graph_*) for calls of up to `max_steps` steps, sized on
+    the `sample` steps. `run(steps)` replays it (calls beyond its capacities run uncaptured, same results)."""
+
+    def __init__(self, sample, cond, cfg, key=0, geom=None, batch=None, max_steps=None, margin=0.25):
+        import dataclasses
+        CT.validate_chain(cfg)
+        cfg = cfg if cfg.precision == "fp32" else dataclasses.replace(cfg, precision="fp32")
+        self.cfg, self.cond = cfg, cond
+        self.G = geom if geom is not None else DT.sector_geom()
+        ev, counter, edep, z, t, _ = step_table(sample)
+        ds = DeviceSteps(ev, counter, edep, z, t)
+        self.ctx = context(cfg, cond, self.G)
+        self.key = _seed(key, None)
+        self.evkey_hi = ds.evkey_hi
+        self.ctx.set_params(cfg, self.key, ds.evkey_hi, 32)
+        opts = L.BatchOpts()
+        for k, v in {"tile": 64, "sort_tasks": 1, **(batch or {})}.items():
+            setattr(opts, k, v)
+        self.graph = L.Graph(self.ctx, ds.struct(), max_steps or max(ds.n, 1), opts, margin)
+
+    def run(self, steps, timers=None):
+        import ctypes
+        ev, counter, edep, z, t, _ = step_table(steps)
+        ds = DeviceSteps(ev, counter, edep, z, t)
+        if ds.evkey_hi != self.evkey_hi:
+            raise ValueError("steps: the graph was captured for another upper half of the event key")
+        self.ctx.set_params(self.cfg, self.key, self.evkey_hi, 32)   # the captured parameters
+        tm = L.Timing()
+        guess = max(1024, 4 * ds.n)
+        pb, ps = _pulse_bufs(guess)
+        cb, cs = _cluster_bufs(guess, guess)
+        rc = self.graph.launch(ds.struct(), ps, cs, tm)
+        if rc == L.CRV_E_CAPACITY:
+            pb, ps = _pulse_bufs(ps.n, may_trim=False)
+            cb, cs = _cluster_bufs(cs.n, cs.n_mem, may_trim=False)
+            self.graph.fetch(ps, cs)
+        out = _tables(pb, ps, cb, cs, tm, None, timers)
+        out["graph"] = self.graph.stats()
+        return out
+
+    def close(self):
+        self.graph.close()
 
 
 def cluster_table(b, nc, nm):

@@ -89,6 +89,8 @@ struct Ctx {
   bool kev_on = false;
   int64_t b_rec_cap = 0, b_run_cap = 0;  // pool sizes of the batched chain (grown on demand)
   int b_log_factor = 1;  // fired-pixel log capacity factor of the batched chain, doubled on overflow and kept
+  int b_pcap = 0;        // pulse slots per waveform the batched chain grew to (kept for later calls)
+  int64_t b_last_log = 0;  // 16-byte chunks of the whole fired-pixel log of the last batched call
   std::vector<int> ws_spare;  // slots the running call no longer reads: freed to make room when an allocation fails
   int ws_fail_slot = -1;                  // the workspace request that failed last
   size_t ws_fail_bytes = 0;
@@ -147,6 +149,43 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
                    const double *pe, const int64_t *seq, int64_t n, crv_clusters *out, hipStream_t s);
 // the FP32 batched chain; pulses and clusters end in last_pulses / last_clusters
 int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timing *timing, hipStream_t s);
+
+// ---- the captured (fixed-capacity) batched chain of crv_graph_* ----
+// Device buffers of a fixed-capacity call: the first (eager) run allocates them in order; while the same sequence is
+// captured into a graph, the same requests receive the same buffers again, so capture allocates nothing.
+struct Arena {
+  std::vector<std::pair<void *, size_t>> blocks;
+  size_t next = 0;
+  bool replay = false;
+  void *get(size_t bytes);
+  void release();
+};
+
+// Capacities of a fixed-capacity batched call and its device-side counts. Every size a kernel or a library call needs
+// is a capacity known at capture; the counts of a call live in device memory, the kernels read them there, and
+// whatever exceeds a capacity sets `skip`, which empties the rest of the call (the host runs it uncaptured instead).
+struct Plan {
+  int64_t n_cap = 0, log_cap = 0, rec_cap = 0, run_cap = 0, np_cap = 0;
+  int pcap = 4, log_factor = 1, tile = 64, sort_tasks = 1, grid_s = 0, grid_w = 0;
+  // the graph's inputs: steps at capacity and their count
+  uint32_t *in_event = nullptr, *in_counter = nullptr;
+  double *in_edep = nullptr, *in_z = nullptr, *in_t = nullptr;
+  int64_t *n_dev = nullptr;
+  int32_t *T_dev = nullptr;  // hit counters of the call
+  int *skip = nullptr;       // [0] skip (a capacity was exceeded), [1] which: 1 log, 2 pools, 3 pulse slots, 4 pulses
+  int64_t *sizes = nullptr;  // [0] pulses, [1] clusters, [2] members, [3] photons, [4] fired pixels, [5] records,
+                             // [6] runs
+  crv_pulses P{};            // outputs at capacity np_cap (clusters and members are bounded by the pulses)
+  crv_clusters C{};
+  hipEvent_t kev[6] = {};    // start / end of kb_sipm, kb_wave and kb_fit (recorded when set)
+};
+int group_steps_fixed(Ctx *c, Plan &pl, Arena &ar, StepGroups &g, hipStream_t s);
+int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s);
+int clusters_fixed(Ctx *c, Plan &pl, Arena &ar, const int64_t *ev, const int32_t *bar, const int32_t *sipm,
+                   const double *time, const double *pe, const int64_t *seq, hipStream_t s);
+// device-side exclusive scan over a fixed item count (n + 1 outputs, first 0), temporaries from the arena
+int scan_fixed(Ctx *c, Arena &ar, const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
+int widen_scan_fixed(Ctx *c, Arena &ar, const int32_t *in, int64_t *out, int64_t n, hipStream_t s);
 
 // device-side exclusive scan of int64 counts (n + 1 outputs, last = total) with hipcub
 int scan_i64(Ctx *c, const int64_t *in, int64_t *out, int64_t n, int slot, hipStream_t s);

@@ -12,6 +12,24 @@ struct crv_ctx {
   crv::Ctx c;
 };
 
+// a captured FP32 batched chain (crv_graph_*): the plan's buffers, the graph and its launch statistics
+struct crv_graph {
+  crv_ctx *ctx = nullptr;
+  crv::Plan pl;
+  crv::Arena fixed;   // inputs, counts and outputs of the plan
+  crv::Arena run;     // everything the captured sequence uses
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cs = nullptr;  // the capture stream
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  crv_params params{};        // the parameters captured into the kernels
+  crv_batch_opts opts{};      // queue options of the plan (for the uncaptured runs)
+  int64_t launches = 0, fallbacks = 0;
+  int last_fell_back = 0;
+  int64_t sizes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t n_host = 0;  // the step count a launch copies to the device (kept alive until the copy has run)
+};
+
 namespace crv {
 
 static std::string g_create_error;
@@ -94,6 +112,53 @@ int scan_i64(Ctx *c, const int64_t *in, int64_t *out, int64_t n, int slot, hipSt
   if (!tmp) return set_error(c, CRV_E_HIP, "scan workspace allocation failed");
   CRV_HIP(c, hipcub::DeviceScan::InclusiveSum(tmp, tb, in, out + 1, (int)n, s));
   return CRV_OK;
+}
+
+void *Arena::get(size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (bytes == 0) bytes = 256;
+  if (replay) {  // the same request order as the eager run: hand out its buffers again
+    if (next >= blocks.size() || blocks[next].second < bytes) return nullptr;
+    return blocks[next++].first;
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  blocks.emplace_back(p, bytes);
+  return p;
+}
+
+void Arena::release() {
+  for (auto &b : blocks) (void)hipFree(b.first);
+  blocks.clear();
+  next = 0;
+  replay = false;
+}
+
+namespace {
+__global__ void k_first_zero(int64_t *__restrict__ out) { *out = 0; }
+}  // namespace
+
+int scan_fixed(Ctx *c, Arena &ar, const int64_t *in, int64_t *out, int64_t n, hipStream_t s) {
+  k_first_zero<<<1, 1, 0, s>>>(out);
+  CRV_HIP(c, hipGetLastError());
+  if (n == 0) return CRV_OK;
+  size_t tb = 0;
+  CRV_HIP(c, hipcub::DeviceScan::InclusiveSum(nullptr, tb, in, out + 1, (int)n, s));
+  void *tmp = ar.get(tb);
+  if (!tmp) return set_error(c, CRV_E_HIP, "graph: scan buffer allocation failed");
+  CRV_HIP(c, hipcub::DeviceScan::InclusiveSum(tmp, tb, in, out + 1, (int)n, s));
+  return CRV_OK;
+}
+
+int widen_scan_fixed(Ctx *c, Arena &ar, const int32_t *in, int64_t *out, int64_t n, hipStream_t s) {
+  int64_t *w = (int64_t *)ar.get((n + 1) * 8);
+  if (!w) return set_error(c, CRV_E_HIP, "graph: scan buffer allocation failed");
+  if (n) k_i32_to_i64<<<nblocks(n, 256), 256, 0, s>>>(in, w, n);
+  CRV_HIP(c, hipGetLastError());
+  return scan_fixed(c, ar, w, out, n, s);
 }
 
 int scan_i32_to_i64(Ctx *c, const int32_t *in, int64_t *out, int64_t n, int slot, hipStream_t s) {
@@ -234,6 +299,48 @@ __global__ void k_bar_sipm(int64_t n, const int32_t *__restrict__ chan, int32_t 
   if (i >= n) return;
   bar[i] = chan[i] / 4;
   sipm[i] = chan[i] % 4;
+}
+
+// copy a pulse and cluster table (device) into the caller's buffers; CRV_E_CAPACITY with the sizes when they are short
+int copy_tables(Ctx *c, const crv_pulses &P, const crv_clusters &C, crv_pulses *pulses, crv_clusters *clusters,
+                hipStream_t s) {
+  pulses->n = P.n;
+  clusters->n = C.n;
+  clusters->n_mem = C.n_mem;
+  if (P.n > pulses->capacity || C.n > clusters->capacity || C.n_mem > clusters->mem_capacity)
+    return set_error(c, CRV_E_CAPACITY, "chain: pulse or cluster capacity too small (sizes in n; call crv_chain_fetch)");
+  auto cp = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
+    if (!dst || !bytes) return hipSuccess;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
+  };
+  const int64_t np = P.n, nc = C.n;
+  hipError_t e = hipSuccess;
+  void *pd[] = {pulses->wf, pulses->seq, pulses->ev};
+  const void *ps[] = {P.wf, P.seq, P.ev};
+  for (int i = 0; i < 3; ++i) e = e == hipSuccess ? cp(pd[i], ps[i], np * 8) : e;
+  e = e == hipSuccess ? cp(pulses->chan, P.chan, np * 4) : e;
+  double *pdd[] = {pulses->pe, pulses->peph, pulses->time, pulses->height, pulses->beta, pulses->chi2, pulses->letime,
+                   pulses->ped, pulses->fit_a, pulses->fit_mu, pulses->fit_b};
+  const double *psd[] = {P.pe, P.peph, P.time, P.height, P.beta, P.chi2, P.letime, P.ped, P.fit_a, P.fit_mu, P.fit_b};
+  for (int i = 0; i < 11; ++i) e = e == hipSuccess ? cp(pdd[i], psd[i], np * 8) : e;
+  e = e == hipSuccess ? cp(pulses->failed, P.failed, np) : e;
+  e = e == hipSuccess ? cp(pulses->zerondf, P.zerondf, np) : e;
+  e = e == hipSuccess ? cp(pulses->pedfromdb, P.pedfromdb, np) : e;
+  e = e == hipSuccess ? cp(clusters->cl_event, C.cl_event, nc * 8) : e;
+  e = e == hipSuccess ? cp(clusters->cl_type, C.cl_type, nc * 4) : e;
+  e = e == hipSuccess ? cp(clusters->cl_poscalc, C.cl_poscalc, nc * 4) : e;
+  e = e == hipSuccess ? cp(clusters->cl_sidehits, C.cl_sidehits, nc * 8) : e;
+  double *cdd[] = {clusters->cl_pe, clusters->cl_start, clusters->cl_end, clusters->cl_slope, clusters->cl_avgtime};
+  const double *csd[] = {C.cl_pe, C.cl_start, C.cl_end, C.cl_slope, C.cl_avgtime};
+  for (int i = 0; i < 5; ++i) e = e == hipSuccess ? cp(cdd[i], csd[i], nc * 8) : e;
+  e = e == hipSuccess ? cp(clusters->cl_pos, C.cl_pos, nc * 24) : e;
+  e = e == hipSuccess ? cp(clusters->cl_sidepe, C.cl_sidepe, nc * 16) : e;
+  e = e == hipSuccess ? cp(clusters->cl_sidetime, C.cl_sidetime, nc * 16) : e;
+  e = e == hipSuccess ? cp(clusters->cl_mem_off, C.cl_mem_off, (nc + 1) * 8) : e;
+  e = e == hipSuccess ? cp(clusters->mem_idx, C.mem_idx, C.n_mem * 8) : e;
+  if (e != hipSuccess) return set_error(c, CRV_E_HIP, std::string("chain fetch: ") + hipGetErrorString(e));
+  CRV_HIP(c, hipStreamSynchronize(s));
+  return CRV_OK;
 }
 
 }  // namespace
@@ -643,46 +750,7 @@ int crv_chain_records(crv_ctx *ctx, const uint16_t **adc, const int64_t **label,
 int crv_chain_fetch(crv_ctx *ctx, crv_pulses *pulses, crv_clusters *clusters, crv_stream stream) {
   if (int rc = guard(ctx)) return rc;
   Ctx *c = &ctx->c;
-  hipStream_t s = (hipStream_t)stream;
-  const crv_pulses &P = c->last_pulses;
-  const crv_clusters &C = c->last_clusters;
-  pulses->n = P.n;
-  clusters->n = C.n;
-  clusters->n_mem = C.n_mem;
-  if (P.n > pulses->capacity || C.n > clusters->capacity || C.n_mem > clusters->mem_capacity)
-    return set_error(c, CRV_E_CAPACITY, "chain: pulse or cluster capacity too small (sizes in n; call crv_chain_fetch)");
-  auto cp = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
-    if (!dst || !bytes) return hipSuccess;
-    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
-  };
-  const int64_t np = P.n, nc = C.n;
-  hipError_t e = hipSuccess;
-  void *pd[] = {pulses->wf, pulses->seq, pulses->ev};
-  const void *ps[] = {P.wf, P.seq, P.ev};
-  for (int i = 0; i < 3; ++i) e = e == hipSuccess ? cp(pd[i], ps[i], np * 8) : e;
-  e = e == hipSuccess ? cp(pulses->chan, P.chan, np * 4) : e;
-  double *pdd[] = {pulses->pe, pulses->peph, pulses->time, pulses->height, pulses->beta, pulses->chi2, pulses->letime,
-                   pulses->ped, pulses->fit_a, pulses->fit_mu, pulses->fit_b};
-  const double *psd[] = {P.pe, P.peph, P.time, P.height, P.beta, P.chi2, P.letime, P.ped, P.fit_a, P.fit_mu, P.fit_b};
-  for (int i = 0; i < 11; ++i) e = e == hipSuccess ? cp(pdd[i], psd[i], np * 8) : e;
-  e = e == hipSuccess ? cp(pulses->failed, P.failed, np) : e;
-  e = e == hipSuccess ? cp(pulses->zerondf, P.zerondf, np) : e;
-  e = e == hipSuccess ? cp(pulses->pedfromdb, P.pedfromdb, np) : e;
-  e = e == hipSuccess ? cp(clusters->cl_event, C.cl_event, nc * 8) : e;
-  e = e == hipSuccess ? cp(clusters->cl_type, C.cl_type, nc * 4) : e;
-  e = e == hipSuccess ? cp(clusters->cl_poscalc, C.cl_poscalc, nc * 4) : e;
-  e = e == hipSuccess ? cp(clusters->cl_sidehits, C.cl_sidehits, nc * 8) : e;
-  double *cdd[] = {clusters->cl_pe, clusters->cl_start, clusters->cl_end, clusters->cl_slope, clusters->cl_avgtime};
-  const double *csd[] = {C.cl_pe, C.cl_start, C.cl_end, C.cl_slope, C.cl_avgtime};
-  for (int i = 0; i < 5; ++i) e = e == hipSuccess ? cp(cdd[i], csd[i], nc * 8) : e;
-  e = e == hipSuccess ? cp(clusters->cl_pos, C.cl_pos, nc * 24) : e;
-  e = e == hipSuccess ? cp(clusters->cl_sidepe, C.cl_sidepe, nc * 16) : e;
-  e = e == hipSuccess ? cp(clusters->cl_sidetime, C.cl_sidetime, nc * 16) : e;
-  e = e == hipSuccess ? cp(clusters->cl_mem_off, C.cl_mem_off, (nc + 1) * 8) : e;
-  e = e == hipSuccess ? cp(clusters->mem_idx, C.mem_idx, C.n_mem * 8) : e;
-  if (e != hipSuccess) return set_error(c, CRV_E_HIP, std::string("chain fetch: ") + hipGetErrorString(e));
-  CRV_HIP(c, hipStreamSynchronize(s));
-  return CRV_OK;
+  return copy_tables(c, c->last_pulses, c->last_clusters, pulses, clusters, (hipStream_t)stream);
 }
 
 int crv_ctx_workspace(crv_ctx *ctx, uint64_t *bytes) {
@@ -712,7 +780,266 @@ int crv_ctx_trim(crv_ctx *ctx, uint64_t *freed) {
   c->last_clusters = crv_clusters{};
   c->last_digis = crv_digis{};
   c->b_rec_cap = c->b_run_cap = 0;
+  c->b_pcap = 0;
   if (freed) *freed = b;
+  return CRV_OK;
+}
+
+
+// ---- captured batched chain ----
+
+int crv_graph_create(crv_ctx *ctx, const crv_steps *sample, int64_t max_steps, const crv_batch_opts *opts,
+                     double margin, crv_graph **out) {
+  if (int rc = guard(ctx)) return rc;
+  Ctx *c = &ctx->c;
+  if (!sample || !opts || !out) return set_error(c, CRV_E_ARG, "NULL argument");
+  *out = nullptr;
+  if (int rc = ready(c, true)) return rc;
+  if (max_steps < sample->n || max_steps <= 0 || max_steps > INT32_MAX / 4)
+    return set_error(c, CRV_E_ARG, "graph: max_steps must hold the sample and stay below 2^29");
+  if (!(margin >= 0.0) || margin > 10.0) return set_error(c, CRV_E_ARG, "graph: margin must be in [0, 10]");
+  if (opts->tile <= 0 || opts->tile % 64 || opts->sort_tasks < 0 || opts->sort_tasks > 3)
+    return set_error(c, CRV_E_ARG, "graph: opts need a tile (multiple of 64) and a sort order 0-3");
+  // sizing: the uncaptured chain on the sample gives the workload's table sizes per step
+  crv_batch_opts so = *opts;
+  so.log_budget = 0;
+  so.pipeline = 0;
+  crv_timing tm{};
+  c->kev_on = false;
+  int rc = batched_stage(c, sample, &so, &tm, nullptr);
+  if (rc) return rc;
+  const double scale = (double)max_steps / (double)std::max<int64_t>(sample->n, 1) * (1.0 + margin);
+  auto cap = [&](int64_t v) { return (int64_t)std::ceil((double)v * scale) + 1024; };
+  crv_graph *G = new crv_graph();
+  G->ctx = ctx;
+  G->opts = so;
+  Plan &pl = G->pl;
+  pl.n_cap = max_steps;
+  pl.log_cap = cap(c->b_last_log);
+  pl.rec_cap = cap(tm.n_records);
+  pl.run_cap = cap(tm.n_waveforms);
+  pl.np_cap = cap(tm.n_pulses);
+  pl.pcap = std::max(so.pulse_slots_used, 4);
+  pl.log_factor = c->b_log_factor;
+  pl.tile = so.tile;
+  pl.sort_tasks = so.sort_tasks;
+  pl.grid_s = pl.grid_w = so.grid;
+  auto fail = [&](int code, const std::string &m) {
+    (void)crv_graph_destroy(G);
+    return set_error(c, code, m);
+  };
+  Arena &F = G->fixed;
+  const int64_t n = pl.n_cap, np = pl.np_cap;
+  pl.in_event = (uint32_t *)F.get(n * 4);
+  pl.in_counter = (uint32_t *)F.get(n * 4);
+  pl.in_edep = (double *)F.get(n * 8);
+  pl.in_z = (double *)F.get(n * 8);
+  pl.in_t = (double *)F.get(n * 8);
+  pl.n_dev = (int64_t *)F.get(8);
+  pl.T_dev = (int32_t *)F.get(8);
+  pl.skip = (int *)F.get(16);
+  pl.sizes = (int64_t *)F.get(64);
+  crv_pulses &P = pl.P;
+  int64_t **ip[] = {&P.wf, &P.seq, &P.ev};
+  for (auto q : ip) *q = (int64_t *)F.get(np * 8 + 8);
+  P.chan = (int32_t *)F.get(np * 4 + 8);
+  double **dp[] = {&P.pe, &P.peph, &P.time, &P.height, &P.beta, &P.chi2, &P.letime, &P.ped, &P.fit_a, &P.fit_mu,
+                   &P.fit_b};
+  for (auto q : dp) *q = (double *)F.get(np * 8 + 8);
+  uint8_t **bp[] = {&P.failed, &P.zerondf, &P.pedfromdb};
+  for (auto q : bp) *q = (uint8_t *)F.get(np + 8);
+  crv_clusters &C = pl.C;
+  C.cl_event = (int64_t *)F.get(np * 8 + 8);
+  C.cl_type = (int32_t *)F.get(np * 4 + 8);
+  C.cl_poscalc = (int32_t *)F.get(np * 4 + 8);
+  C.cl_sidehits = (int32_t *)F.get(np * 8 + 8);
+  double **cp[] = {&C.cl_pe, &C.cl_start, &C.cl_end, &C.cl_slope, &C.cl_avgtime};
+  for (auto q : cp) *q = (double *)F.get(np * 8 + 8);
+  C.cl_pos = (double *)F.get(np * 24 + 8);
+  C.cl_sidepe = (double *)F.get(np * 16 + 8);
+  C.cl_sidetime = (double *)F.get(np * 16 + 8);
+  C.cl_mem_off = (int64_t *)F.get((np + 2) * 8);
+  C.mem_idx = (int64_t *)F.get(np * 8 + 8);
+  bool ok = pl.in_event && pl.in_counter && pl.in_edep && pl.in_z && pl.in_t && pl.n_dev && pl.T_dev && pl.skip &&
+            pl.sizes && P.chan && C.cl_event && C.cl_type && C.cl_poscalc && C.cl_sidehits && C.cl_pos && C.cl_sidepe &&
+            C.cl_sidetime && C.cl_mem_off && C.mem_idx;
+  for (auto q : ip) ok = ok && *q;
+  for (auto q : dp) ok = ok && *q;
+  for (auto q : bp) ok = ok && *q;
+  for (auto q : cp) ok = ok && *q;
+  if (!ok) return fail(CRV_E_HIP, "graph: plan buffer allocation failed");
+  if (hipStreamCreateWithFlags(&G->cs, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&G->e0) != hipSuccess || hipEventCreate(&G->e1) != hipSuccess)
+    return fail(CRV_E_HIP, "graph: stream or event creation failed");
+  for (auto &e : pl.kev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(CRV_E_HIP, "graph: event creation failed");
+  // the eager run on the sample allocates the sequence's buffers and checks it against the uncaptured chain
+  auto load = [&](const crv_steps *st, hipStream_t s) -> hipError_t {
+    const int64_t m = st->n;
+    G->n_host = m;
+    hipError_t e = hipMemcpyAsync(pl.n_dev, &G->n_host, 8, hipMemcpyHostToDevice, s);
+    if (m) {
+      if (e == hipSuccess && st->event != pl.in_event)
+        e = hipMemcpyAsync(pl.in_event, st->event, m * 4, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess && st->counter != pl.in_counter)
+        e = hipMemcpyAsync(pl.in_counter, st->counter, m * 4, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess && st->edep != pl.in_edep)
+        e = hipMemcpyAsync(pl.in_edep, st->edep, m * 8, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess && st->z != pl.in_z) e = hipMemcpyAsync(pl.in_z, st->z, m * 8, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess && st->t != pl.in_t) e = hipMemcpyAsync(pl.in_t, st->t, m * 8, hipMemcpyDeviceToDevice, s);
+    }
+    return e;
+  };
+  if (load(sample, G->cs) != hipSuccess) return fail(CRV_E_HIP, "graph: input copy failed");
+  if ((rc = batched_fixed(c, pl, G->run, G->cs))) {
+    (void)hipStreamSynchronize(G->cs);
+    const std::string m = c->err;
+    return fail(rc, m);
+  }
+  int skip[2] = {0, 0};
+  if (hipMemcpyAsync(skip, pl.skip, 8, hipMemcpyDeviceToHost, G->cs) != hipSuccess ||
+      hipMemcpyAsync(G->sizes, pl.sizes, 64, hipMemcpyDeviceToHost, G->cs) != hipSuccess ||
+      hipStreamSynchronize(G->cs) != hipSuccess)
+    return fail(CRV_E_HIP, "graph: eager run failed");
+  if (skip[0]) return fail(CRV_E_STATE, "graph: the sample exceeds the plan's capacities (reason " +
+                                          std::to_string(skip[1]) + ")");
+  if (G->sizes[0] != tm.n_pulses || G->sizes[1] != tm.n_clusters || G->sizes[2] != tm.n_members)
+    return fail(CRV_E_STATE, "graph: the fixed-capacity sequence disagrees with the batched chain on the sample");
+  // capture the same sequence: it receives the eager run's buffers again, in the same order
+  G->run.replay = true;
+  G->run.next = 0;
+  if (hipStreamBeginCapture(G->cs, hipStreamCaptureModeRelaxed) != hipSuccess)
+    return fail(CRV_E_HIP, "graph: hipStreamBeginCapture failed");
+  rc = batched_fixed(c, pl, G->run, G->cs);
+  hipGraph_t gr = nullptr;
+  const hipError_t ec = hipStreamEndCapture(G->cs, &gr);
+  G->graph = gr;
+  if (rc) {
+    const std::string m = c->err;
+    return fail(rc, "graph: capture: " + m);
+  }
+  if (ec != hipSuccess || !gr) return fail(CRV_E_HIP, std::string("graph: capture failed: ") + hipGetErrorString(ec));
+  if (G->run.next != G->run.blocks.size()) return fail(CRV_E_STATE, "graph: the captured sequence's buffers differ");
+  if (hipGraphInstantiate(&G->exec, gr, nullptr, nullptr, 0) != hipSuccess)
+    return fail(CRV_E_HIP, "graph: hipGraphInstantiate failed");
+  G->params = c->p;
+  *out = G;
+  return CRV_OK;
+}
+
+int crv_graph_launch(crv_graph *G, const crv_steps *steps, crv_pulses *pulses, crv_clusters *clusters,
+                     crv_timing *timing, crv_stream stream) {
+  if (!G) return CRV_E_ARG;
+  crv_ctx *ctx = G->ctx;
+  if (int rc = guard(ctx)) return rc;
+  Ctx *c = &ctx->c;
+  if (!steps || !pulses || !clusters) return set_error(c, CRV_E_ARG, "NULL argument");
+  if (std::memcmp(&G->params, &c->p, sizeof(crv_params)) != 0)
+    return set_error(c, CRV_E_STATE, "graph: parameters changed since the capture (create the graph again)");
+  hipStream_t s = (hipStream_t)stream;
+  ++G->launches;
+  G->last_fell_back = 0;
+  auto fallback = [&]() {
+    ++G->fallbacks;
+    G->last_fell_back = 1;
+    crv_batch_opts o = G->opts;
+    return crv_chain_run_batched(ctx, steps, &o, pulses, clusters, timing, stream);
+  };
+  if (steps->n > G->pl.n_cap) return fallback();
+  Plan &pl = G->pl;
+  const int64_t m = steps->n;
+  G->n_host = m;
+  CRV_HIP(c, hipMemcpyAsync(pl.n_dev, &G->n_host, 8, hipMemcpyHostToDevice, s));
+  if (m) {
+    if (steps->event != pl.in_event)
+      CRV_HIP(c, hipMemcpyAsync(pl.in_event, steps->event, m * 4, hipMemcpyDeviceToDevice, s));
+    if (steps->counter != pl.in_counter)
+      CRV_HIP(c, hipMemcpyAsync(pl.in_counter, steps->counter, m * 4, hipMemcpyDeviceToDevice, s));
+    if (steps->edep != pl.in_edep) CRV_HIP(c, hipMemcpyAsync(pl.in_edep, steps->edep, m * 8, hipMemcpyDeviceToDevice, s));
+    if (steps->z != pl.in_z) CRV_HIP(c, hipMemcpyAsync(pl.in_z, steps->z, m * 8, hipMemcpyDeviceToDevice, s));
+    if (steps->t != pl.in_t) CRV_HIP(c, hipMemcpyAsync(pl.in_t, steps->t, m * 8, hipMemcpyDeviceToDevice, s));
+  }
+  CRV_HIP(c, hipEventRecord(G->e0, s));
+  CRV_HIP(c, hipGraphLaunch(G->exec, s));
+  CRV_HIP(c, hipEventRecord(G->e1, s));
+  int skip[2] = {0, 0}, hf[F_NFLAGS];
+  CRV_HIP(c, hipMemcpyAsync(skip, pl.skip, 8, hipMemcpyDeviceToHost, s));
+  CRV_HIP(c, hipMemcpyAsync(G->sizes, pl.sizes, 64, hipMemcpyDeviceToHost, s));
+  CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
+  CRV_HIP(c, hipStreamSynchronize(s));
+  if (hf[F_UNSORTED]) return set_error(c, CRV_E_ARG, "steps must be sorted by event");
+  if (hf[F_BADCHAN]) return set_error(c, CRV_E_ARG, "batched chain: channel beyond the uploaded conditions");
+  if (hf[F_COUNTER])
+    return set_error(c, CRV_E_OVERFLOW, "batched chain: counter id >= 8192 or more than 65535 photoelectrons from one "
+                                        "step");
+  if (hf[F_CASCADE])
+    return set_error(c, CRV_E_OVERFLOW,
+                     "batched chain: cross-talk cascade wider than 16 pixels or deeper than 31 generations");
+  if (skip[0]) return fallback();  // a capacity was exceeded: this call runs uncaptured
+  if (timing) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, G->e0, G->e1);
+    *timing = crv_timing{};
+    timing->total_ms = ms;
+    float km[3] = {0.f, 0.f, 0.f};  // the finder's kernels run on three streams: no single span is measured
+    for (int k = 0; k < 3; ++k) (void)hipEventElapsedTime(&km[k], pl.kev[2 * k], pl.kev[2 * k + 1]);
+    timing->k_sipm_ms = km[0];
+    timing->k_digi_ms = km[1];
+    timing->k_pulse_ms = km[2];
+    timing->n_steps = m;
+    timing->n_photons = G->sizes[3];
+    timing->n_charges = G->sizes[4];
+    timing->n_records = G->sizes[5];
+    timing->n_waveforms = G->sizes[6];
+    timing->n_pulses = G->sizes[0];
+    timing->n_clusters = G->sizes[1];
+    timing->n_members = G->sizes[2];
+  }
+  return crv_graph_fetch(G, pulses, clusters, stream);
+}
+
+int crv_graph_fetch(crv_graph *G, crv_pulses *pulses, crv_clusters *clusters, crv_stream stream) {
+  if (!G) return CRV_E_ARG;
+  if (int rc = guard(G->ctx)) return rc;
+  Ctx *c = &G->ctx->c;
+  if (!pulses || !clusters) return set_error(c, CRV_E_ARG, "NULL argument");
+  if (G->last_fell_back)
+    return copy_tables(c, c->last_pulses, c->last_clusters, pulses, clusters, (hipStream_t)stream);
+  crv_pulses P = G->pl.P;
+  crv_clusters C = G->pl.C;
+  P.n = G->sizes[0];
+  C.n = G->sizes[1];
+  C.n_mem = G->sizes[2];
+  return copy_tables(c, P, C, pulses, clusters, (hipStream_t)stream);
+}
+
+int crv_graph_stats(crv_graph *G, int64_t *launches, int64_t *fallbacks, int64_t *capacities) {
+  if (!G) return CRV_E_ARG;
+  if (launches) *launches = G->launches;
+  if (fallbacks) *fallbacks = G->fallbacks;
+  if (capacities) {
+    const Plan &pl = G->pl;
+    const int64_t v[7] = {pl.n_cap, pl.log_cap, pl.rec_cap, pl.run_cap, pl.np_cap, pl.pcap, G->last_fell_back};
+    for (int i = 0; i < 7; ++i) capacities[i] = v[i];
+  }
+  return CRV_OK;
+}
+
+int crv_graph_destroy(crv_graph *G) {
+  if (!G) return CRV_OK;
+  if (G->ctx) (void)hipSetDevice(G->ctx->c.device);
+  if (G->cs) (void)hipStreamSynchronize(G->cs);
+  (void)hipDeviceSynchronize();
+  if (G->exec) (void)hipGraphExecDestroy(G->exec);
+  if (G->graph) (void)hipGraphDestroy(G->graph);
+  if (G->e0) (void)hipEventDestroy(G->e0);
+  if (G->e1) (void)hipEventDestroy(G->e1);
+  for (auto &e : G->pl.kev)
+    if (e) (void)hipEventDestroy(e);
+  if (G->cs) (void)hipStreamDestroy(G->cs);
+  G->run.release();
+  G->fixed.release();
+  delete G;
   return CRV_OK;
 }
 

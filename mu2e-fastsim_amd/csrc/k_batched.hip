@@ -323,7 +323,16 @@ struct Queue {
   int64_t n;  // one past the last queue position
   const int32_t *order;
   int64_t start;  // first queue position (a multiple of 64)
+  const int32_t *tdev = nullptr;  // fixed-capacity calls: the hit counters, n limited to 4 of them per counter
+  const int *skip = nullptr;      // fixed-capacity calls: nonzero when the call exceeded a capacity
 };
+
+// the queue of this launch: fixed-capacity calls read their extent from device memory; false: nothing to do
+__device__ __forceinline__ bool resolve(Queue &q) {
+  if (q.skip && *q.skip) return false;
+  if (q.tdev) q.n = min(q.n, 4 * (int64_t)(*q.tdev));
+  return true;
+}
 
 __device__ __forceinline__ bool grab(const Queue &qu, int *tile_base, int64_t &base) {
   if (threadIdx.x == 0) *tile_base = atomicAdd(qu.ctr, qu.tile);
@@ -371,6 +380,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
                                               Out o, Queue qu, int32_t *__restrict__ nfired, int *__restrict__ flags) {
   __shared__ uint32_t bm[CRV_DEAD_WORDS * 64];
   __shared__ int tile_base;
+  if (!resolve(qu)) return;
   const int lane = threadIdx.x;
   const float Vov = (float)p.overvoltage, p1 = (float)p.p1;
   const float inv_tau = 1.f / (float)p.time_constant, inv_p2 = 1.f / (float)p.p2;
@@ -910,6 +920,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void kb
                                               int *__restrict__ flags) {
   __shared__ int tile_base;
   __shared__ float tmpl_s[TMPL_LDS];  // the single-PE template, staged once per workgroup
+  if (!resolve(qu)) return;
   const int lane = threadIdx.x;
   const double gap = p.single_pe_max_time + 4.0 * PERIODD;
   const DrawR dr{(uint32_t)p.seed, p.evkey_hi};
@@ -1154,8 +1165,11 @@ __device__ __forceinline__ void fit_run(const crv_params &p, const Tables &t, co
 
 __global__ __launch_bounds__(64) void kb_fit(crv_params p, Tables t, const uint64_t *__restrict__ task_key,
                                              const int64_t *__restrict__ roff, Out o, int64_t nruns,
-                                             int *__restrict__ flags) {
+                                             int *__restrict__ flags, const int32_t *__restrict__ ndev = nullptr,
+                                             const int *__restrict__ skip = nullptr) {
   __shared__ float win[FWIN * 64];
+  if (skip && *skip) return;
+  if (ndev) nruns = min(nruns, (int64_t)*ndev);  // fixed-capacity calls: the runs of the call
   const int64_t lanes = (int64_t)gridDim.x * blockDim.x, gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int64_t r = gid; r < nruns; r += lanes)
     fit_run(p, t, task_key, roff, o, r, o.yscr + gid, lanes, win + threadIdx.x, flags);
@@ -1164,9 +1178,16 @@ __global__ __launch_bounds__(64) void kb_fit(crv_params p, Tables t, const uint6
 // fired-pixel log capacity per lane of each group of 64 SiPMs in queue order, from the largest photon count of the
 // group (3 factor / 4 of it + 16; factor doubles on every re-run); the group takes 64 times that
 __global__ void kb_gcaps(const int64_t *__restrict__ nph, const int32_t *__restrict__ order, int64_t L4,
-                         int64_t *__restrict__ gcap, int64_t *__restrict__ gsize, int factor, int exact_order) {
+                         int64_t *__restrict__ gcap, int64_t *__restrict__ gsize, int factor, int exact_order,
+                         const int32_t *__restrict__ Tdev = nullptr) {
   const int64_t grp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (grp >= (L4 + 63) / 64) return;
+  if (Tdev && grp * 64 >= 4 * (int64_t)*Tdev) {  // fixed-capacity calls: groups past the call hold nothing
+    gcap[grp] = 0;
+    gsize[grp] = 0;
+    return;
+  }
+  if (Tdev) L4 = min(L4, 4 * (int64_t)*Tdev);
   int64_t m = 0;
   // sorted longest first by the exact count (clamped to 32 bits): the group's first SiPM has its largest count; the
   // octave orders keep a bucket's SiPMs in their given order, so there the group's largest count is looked up
@@ -1197,8 +1218,11 @@ __global__ void kb_widen(const int32_t *__restrict__ in, int64_t *__restrict__ o
 // the pulse table in run order (= (event, channel, time) order), one lane per run
 __global__ void kb_pulses_out(crv_params p, Tables t, int64_t nruns, const uint64_t *__restrict__ task_key,
                               const int64_t *__restrict__ poff, Out o, crv_pulses P, int32_t *__restrict__ pbar,
-                              int32_t *__restrict__ psipm) {
+                              int32_t *__restrict__ psipm, const int32_t *__restrict__ ndev = nullptr,
+                              const int *__restrict__ skip = nullptr) {
   const int64_t fr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (skip && *skip) return;
+  if (ndev) nruns = min(nruns, (int64_t)*ndev);
   if (fr >= nruns) return;
   const int64_t dst = poff[fr], n = poff[fr + 1] - dst;
   if (n == 0) return;
@@ -1245,6 +1269,63 @@ int resident_blocks(Ctx *c, const void *kernel) {
 }
 
 }  // namespace
+
+// ---- fixed-capacity calls (captured into a graph): padding and capacity checks ----
+
+// SiPM positions past the call's 4 T: a sort key after every real one (their stable order keeps them last), no
+// photons, no fired pixels, no runs
+__global__ void kb_pad_sipms(int64_t L4cap, const int32_t *__restrict__ Tdev, uint32_t pad_key,
+                             uint32_t *__restrict__ skey, int32_t *__restrict__ sidx, int64_t *__restrict__ sipm_nph,
+                             int32_t *__restrict__ nfired, int32_t *__restrict__ nrun) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= L4cap || g < 4 * (int64_t)*Tdev) return;
+  skey[g] = pad_key;
+  sidx[g] = (int32_t)g;
+  sipm_nph[g] = 0;
+  nfired[g] = 0;
+  nrun[g] = 0;
+}
+
+// capacity checks: `skip` set (with the first reason) when the call's log, pools, pulse slots or pulses exceed them
+__global__ void kb_check_log(const int64_t *__restrict__ total, int64_t cap, int *__restrict__ skip) {
+  if (*total > cap && atomicCAS(skip, 0, 1) == 0) skip[1] = 1;
+}
+__global__ void kb_check_wave(const int *__restrict__ flags, const int32_t *__restrict__ counts, int64_t rec_cap,
+                              int64_t run_cap, int *__restrict__ skip) {
+  // counts[0] records, counts[1] runs; a log overflow of kb_sipm shows in F_OVERFLOW
+  if (flags[F_OVERFLOW] && atomicCAS(skip, 0, 1) == 0) skip[1] = 1;
+  if ((counts[0] > rec_cap || counts[1] > run_cap) && atomicCAS(skip, 0, 1) == 0) skip[1] = 2;
+}
+__global__ void kb_check_fit(const int *__restrict__ flags, const int64_t *__restrict__ np, int64_t np_cap,
+                             int *__restrict__ skip) {
+  if (flags[F_PSLOT] && atomicCAS(skip, 0, 1) == 0) skip[1] = 3;
+  if (*np > np_cap && atomicCAS(skip, 0, 1) == 0) skip[1] = 4;
+}
+
+__global__ void kb_zero32(int32_t *__restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// pulse rows past the call's pulses (all of them when the call was skipped): an event key after every real one, so
+// the finder's grouping sees them as one trailing event of no pulses
+__global__ void kb_pad_pulses(int64_t np_cap, const int64_t *__restrict__ np, const int *__restrict__ skip,
+                              int64_t *__restrict__ ev) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np_cap) return;
+  if (*skip || i >= *np) ev[i] = INT64_MAX;
+}
+
+// the call's sizes for the host: pulses, photons, fired pixels, records, runs (clusters and members are written by
+// the finder's part)
+__global__ void kb_sizes(const int64_t *__restrict__ np, const int64_t *__restrict__ sums,
+                         const int32_t *__restrict__ counts, const int *__restrict__ skip, int64_t *__restrict__ sz) {
+  sz[0] = *skip ? 0 : *np;
+  sz[3] = sums[1];
+  sz[4] = sums[0];
+  sz[5] = counts[0];
+  sz[6] = counts[1];
+}
 
 // ---- host side ----
 int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timing *timing, hipStream_t s) {
@@ -1424,6 +1505,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
       }
     }
     npass = (int)cuts.size() - 1;
+    c->b_last_log = nlog;
     o.goff = goff;
     o.gcap = gcap;
     o.flog = (float4 *)ws(c, 538, log_chunks * 16 + 16);
@@ -1565,7 +1647,9 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   o.npr = (int32_t *)ws(c, 582, nruns * 4 + 8);
   o.rmap = (int32_t *)ws(c, 583, nruns * 4 + 8);
   if (!o.yscr || !o.npr || !o.rmap) return ws_fail(c);
-  o.pcap = opt->pulse_slots > 0 ? opt->pulse_slots : PSLOTS;
+  // the caller's slot count, else the count the context's earlier calls grew to (kept like the log factor, so a
+  // workload whose waveforms hold more pulses fits its slots from the second call on)
+  o.pcap = opt->pulse_slots > 0 ? opt->pulse_slots : std::max(PSLOTS, c->b_pcap);
   for (int attempt = 0;; ++attempt) {
     const int64_t ns = nruns * o.pcap + 8;
     o.slot = (Slot *)ws(c, 520, ns * sizeof(Slot));
@@ -1581,6 +1665,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     if (attempt >= 5) return set_error(c, CRV_E_OVERFLOW, "batched chain: pulse slots still too few");
     o.pcap *= 2;
   }
+  if (opt->pulse_slots <= 0) c->b_pcap = o.pcap;
   CRV_HIP(c, hipEventRecord(ev[2], s));
   // pulse table
   int64_t *npr64 = (int64_t *)ws(c, 541, (nruns + 1) * 8), *poff = (int64_t *)ws(c, 542, (nruns + 2) * 8);
@@ -1654,6 +1739,146 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     timing->n_members = C.n_mem;
   }
   return CRV_OK;
+}
+
+
+// The batched chain as one fixed-capacity sequence without host synchronisation (crv_graph_*): single pass, pools,
+// slots, log and pulse table at the plan's capacities, every count read on the device. Results equal batched_stage's
+// whenever no capacity is exceeded; otherwise pl.skip is set and the rest of the call does nothing.
+int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
+  const crv_params &p = c->p;
+  const int W = p.window_samples;
+  if (W > 32 * DWORDS || p.samples_zs < 1 || W <= p.samples_zs)
+    return set_error(c, CRV_E_ARG, "batched chain: window_samples must be in (samples_zs, 192]");
+  if (!c->t.tmpl_f) return set_error(c, CRV_E_STATE, "template not uploaded");
+  if (c->t.ntmpl > TMPL_LDS || c->d.nreach > RINGB)
+    return set_error(c, CRV_E_ARG, "batched chain: single-PE template too long");
+  if (pl.tile % 64 || pl.tile <= 0) return set_error(c, CRV_E_ARG, "batched chain: tile must be a multiple of 64");
+  if (pl.grid_s <= 0) pl.grid_s = resident_blocks(c, (const void *)kb_sipm);
+  if (pl.grid_w <= 0) pl.grid_w = resident_blocks(c, (const void *)kb_wave);
+  StepGroups g;
+  int rc = group_steps_fixed(c, pl, ar, g, s);
+  if (rc) return rc;
+  const int64_t n = pl.n_cap, L4 = 4 * n, ngrp = (L4 + 63) / 64;
+  auto A = [&](size_t b) { return ar.get(b); };
+  int32_t *nph = (int32_t *)A(n * 16 + 16);
+  int64_t *sipm_nph = (int64_t *)A((L4 + 1) * 8);
+  int64_t *gcap = (int64_t *)A((ngrp + 1) * 8), *gsize = (int64_t *)A((ngrp + 1) * 8);
+  int64_t *goff = (int64_t *)A((ngrp + 2) * 8);
+  uint32_t *skey = (uint32_t *)A(L4 * 4 + 8), *skey2 = (uint32_t *)A(L4 * 4 + 8);
+  int32_t *sidx = (int32_t *)A(L4 * 4 + 8), *order = (int32_t *)A(L4 * 4 + 8);
+  int32_t *nfired = (int32_t *)A(L4 * 4 + 8);
+  int *counters = (int *)A(64);
+  Out o{};
+  o.xstride = (int64_t)pl.grid_s * 64;
+  o.xcur = (uint16_t *)A((size_t)o.xstride * CMAXB * 2);
+  o.xcand = (uint16_t *)A((size_t)o.xstride * CMAXB * 2);
+  o.xmult = (uint8_t *)A((size_t)o.xstride * CMAXB);
+  o.lastt = (float *)A((size_t)o.xstride * CRV_NPIX * CRV_NPIX * 4);
+  o.gS = (float *)A(n * 16 + 16);
+  o.gHead = (double *)A(n * 32 + 16);
+  o.gTb = (double *)A(n * 32 + 16);
+  o.gRJ = (uint64_t *)A(n * 32 + 16);
+  o.nrun = (int32_t *)A(L4 * 4 + 8);
+  o.flog = (float4 *)A(pl.log_cap * 16 + 16);
+  o.tbase = (double *)A(L4 * 8 + 8);
+  o.rec_cap = pl.rec_cap;
+  o.run_cap = pl.run_cap;
+  o.rec_adc = (uint16_t *)A(o.rec_cap * W * 2);
+  o.rec_label = (int64_t *)A(o.rec_cap * 8);
+  o.rec_tick = (int32_t *)A(o.rec_cap * 4);
+  o.run_rec = (int32_t *)A(o.run_cap * 4);
+  o.run_i0 = (int32_t *)A(o.run_cap * 4);
+  o.run_n = (int32_t *)A(o.run_cap * 4);
+  o.run_k = (int32_t *)A(o.run_cap * 4);
+  o.run_g = (int64_t *)A(o.run_cap * 8);
+  o.run_tdc = (int64_t *)A(o.run_cap * 8);
+  int64_t *sums = (int64_t *)A(16), *fired64 = (int64_t *)A(L4 * 8 + 8);
+  int64_t *nrun64 = (int64_t *)A((L4 + 1) * 8), *roff = (int64_t *)A((L4 + 2) * 8);
+  const int fit_blocks = (int)std::min<int64_t>(nblocks(o.run_cap, 64), 8192);
+  o.yscr = (float *)A((size_t)fit_blocks * 64 * W * 4 + 8);
+  o.npr = (int32_t *)A(o.run_cap * 4 + 8);
+  o.rmap = (int32_t *)A(o.run_cap * 4 + 8);
+  o.pcap = pl.pcap;
+  o.slot = (Slot *)A((o.run_cap * o.pcap + 8) * sizeof(Slot));
+  int64_t *npr64 = (int64_t *)A((o.run_cap + 1) * 8), *poff = (int64_t *)A((o.run_cap + 2) * 8);
+  int32_t *pbar = (int32_t *)A(pl.np_cap * 4 + 8), *psipm = (int32_t *)A(pl.np_cap * 4 + 8);
+  void *ptrs[] = {nph, sipm_nph, gcap, gsize, goff, skey, skey2, sidx, order, nfired, counters, o.xcur, o.xcand,
+                  o.xmult, o.lastt, o.gS, o.gHead, o.gTb, o.gRJ, o.nrun, o.flog, o.tbase, o.rec_adc, o.rec_label,
+                  o.rec_tick, o.run_rec, o.run_i0, o.run_n, o.run_k, o.run_g, o.run_tdc, sums, fired64, nrun64, roff,
+                  o.yscr, o.npr, o.rmap, o.slot, npr64, poff, pbar, psipm};
+  for (void *q : ptrs)
+    if (!q) return set_error(c, CRV_E_HIP, "graph: buffer allocation failed");
+  o.rec_count = counters + 2;
+  o.run_count = counters + 3;
+  o.goff = goff;
+  o.gcap = gcap;
+  o.log0 = 0;
+  kb_zero<<<1, 64, 0, s>>>(counters, 16);
+  kb_zero<<<1, 64, 0, s>>>(pl.skip, 2);
+  const BStep st{g.edep, g.z, g.t};
+  kb_count<<<nblocks(L4, 256), 256, 0, s>>>(p, c->t, c->d, st, pl.T_dev, g.task_key, g.task_off, g.perm, g.evfirst,
+                                            nph, sipm_nph, skey, sidx, pl.sort_tasks, c->flags);
+  const int sbits = sort_key_bits(pl.sort_tasks);
+  const uint32_t pad_key = sbits >= 32 ? 0xFFFFFFFFu : (1u << sbits) - 1u;
+  kb_pad_sipms<<<nblocks(L4, 256), 256, 0, s>>>(L4, pl.T_dev, pad_key, skey, sidx, sipm_nph, nfired, o.nrun);
+  CRV_HIP(c, hipGetLastError());
+  const bool sorted = pl.sort_tasks != 0;
+  if (sorted) {
+    size_t tb = 0;
+    CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, skey, skey2, sidx, order, (int)L4, 0, sbits, s));
+    void *tmp = A(tb);
+    if (!tmp) return set_error(c, CRV_E_HIP, "graph: sort buffer allocation failed");
+    CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, skey, skey2, sidx, order, (int)L4, 0, sbits, s));
+  }
+  const int32_t *qorder = sorted ? order : nullptr;
+  kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, qorder, L4, gcap, gsize, pl.log_factor,
+                                              pl.sort_tasks == 1, pl.T_dev);
+  CRV_HIP(c, hipGetLastError());
+  if ((rc = scan_fixed(c, ar, gsize, goff, ngrp, s))) return rc;
+  kb_check_log<<<1, 1, 0, s>>>(goff + ngrp, pl.log_cap, pl.skip);
+  auto mark = [&](int k) -> hipError_t { return pl.kev[k] ? hipEventRecord(pl.kev[k], s) : hipSuccess; };
+  CRV_HIP(c, mark(0));
+  kb_sipm<<<pl.grid_s, 64, 0, s>>>(p, c->t, c->d, st, g.task_key, g.task_off, g.perm, g.evfirst, nph, o,
+                                   Queue{counters, pl.tile, L4, qorder, 0, pl.T_dev, pl.skip}, nfired, c->flags);
+  CRV_HIP(c, hipGetLastError());
+  CRV_HIP(c, mark(1));
+  CRV_HIP(c, mark(2));
+  kb_wave<<<pl.grid_w, 64, 0, s>>>(p, c->t, c->d, g.task_key, nfired, o,
+                                   Queue{counters + 1, pl.tile, L4, qorder, 0, pl.T_dev, pl.skip}, c->flags);
+  CRV_HIP(c, hipGetLastError());
+  CRV_HIP(c, mark(3));
+  kb_check_wave<<<1, 1, 0, s>>>(c->flags, counters + 2, o.rec_cap, o.run_cap, pl.skip);
+  // photons drawn and pixels fired
+  kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(nfired, fired64, L4);
+  {
+    size_t tb = 0;
+    CRV_HIP(c, hipcub::DeviceReduce::Sum(nullptr, tb, fired64, sums, (int)L4, s));
+    void *tmp = A(tb);
+    if (!tmp) return set_error(c, CRV_E_HIP, "graph: reduce buffer allocation failed");
+    CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, fired64, sums, (int)L4, s));
+    CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, sipm_nph, sums + 1, (int)L4, s));
+  }
+  kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(o.nrun, nrun64, L4);
+  if ((rc = scan_fixed(c, ar, nrun64, roff, L4, s))) return rc;
+  kb_zero32<<<nblocks(o.run_cap + 1, 256), 256, 0, s>>>(o.npr, o.run_cap + 1);
+  kb_zero<<<1, 64, 0, s>>>(c->flags + F_PSLOT, 1);
+  CRV_HIP(c, mark(4));
+  kb_fit<<<fit_blocks, 64, 0, s>>>(p, c->t, g.task_key, roff, o, o.run_cap, c->flags, o.run_count, pl.skip);
+  CRV_HIP(c, hipGetLastError());
+  CRV_HIP(c, mark(5));
+  kb_widen<<<nblocks(o.run_cap, 256), 256, 0, s>>>(o.npr, npr64, o.run_cap);
+  if ((rc = scan_fixed(c, ar, npr64, poff, o.run_cap, s))) return rc;
+  kb_check_fit<<<1, 1, 0, s>>>(c->flags, poff + o.run_cap, pl.np_cap, pl.skip);
+  crv_pulses P = pl.P;
+  P.capacity = pl.np_cap;
+  kb_pulses_out<<<nblocks(o.run_cap, 256), 256, 0, s>>>(p, c->t, o.run_cap, g.task_key, poff, o, P, pbar, psipm,
+                                                         o.run_count, pl.skip);
+  CRV_HIP(c, hipGetLastError());
+  kb_pad_pulses<<<nblocks(pl.np_cap, 256), 256, 0, s>>>(pl.np_cap, poff + o.run_cap, pl.skip, P.ev);
+  kb_sizes<<<1, 1, 0, s>>>(poff + o.run_cap, sums, counters + 2, pl.skip, pl.sizes);
+  CRV_HIP(c, hipGetLastError());
+  return clusters_fixed(c, pl, ar, P.ev, pbar, psipm, P.time, P.pe, P.seq, s);
 }
 
 }  // namespace crv

@@ -457,10 +457,12 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
                                                    const double *__restrict__ ppe, Ev e, int32_t *__restrict__ ncl,
                                                    int32_t *__restrict__ nmem, ClSlots slots,
                                                    int32_t *__restrict__ cl_moff, int32_t *__restrict__ mem,
-                                                   int64_t min_n, int *__restrict__ flags) {
+                                                   int64_t min_n, int *__restrict__ flags,
+                                                   const int32_t *__restrict__ nev_dev = nullptr) {
   constexpr int WNW = WMAX / 64;
   __shared__ WS<WMAX> S;
   const int lane = threadIdx.x;
+  if (nev_dev) nev = min(nev, (int64_t)*nev_dev);  // fixed-capacity calls: the events of the call
   for (int64_t g = blockIdx.x; g < nev; g += gridDim.x) {
     const int64_t a = eoff[g], n64 = eoff[g + 1] - a;
     if (n64 > WMAX || n64 < min_n) continue;  // another size class takes it
@@ -1976,9 +1978,10 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
 // the block finder's events: each event of at least min_n pulses appended (in no particular order) with its pulse
 // count, at most 65535, as the queue's sort key
 __global__ void k_block_keys(int64_t nev, const int64_t *__restrict__ eoff, int64_t min_n, uint32_t *__restrict__ key,
-                             int32_t *__restrict__ idx, int *__restrict__ count) {
+                             int32_t *__restrict__ idx, int *__restrict__ count,
+                             const int32_t *__restrict__ nev_dev = nullptr) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nev) return;
+  if (g >= nev || (nev_dev && g >= *nev_dev)) return;
   const int64_t n = eoff[g + 1] - eoff[g];
   if (n < min_n) return;
   const int k = atomicAdd(count, 1);
@@ -1991,8 +1994,13 @@ __global__ void k_coinc_out(int64_t nev, const int64_t *__restrict__ eoff, const
                             const int64_t *__restrict__ cl_ev, const int32_t *__restrict__ cl_type,
                             const int32_t *__restrict__ cl_poscalc, const int32_t *__restrict__ cl_sidehits,
                             const double *__restrict__ cl_d, const int32_t *__restrict__ cl_moff,
-                            const int32_t *__restrict__ mem, crv_clusters o) {
+                            const int32_t *__restrict__ mem, crv_clusters o,
+                            const int32_t *__restrict__ nev_dev = nullptr) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nev_dev) {  // fixed-capacity calls: the events of the call, and the member offsets' end when there are none
+    nev = min(nev, (int64_t)*nev_dev);
+    if (nev == 0 && g == 0) o.cl_mem_off[0] = 0;
+  }
   if (g >= nev) return;
   int64_t a = eoff[g], c0 = clo[g], m0 = mo[g];
   int nc = ncl[g];
@@ -2174,6 +2182,133 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
                                                 mem, *out);
   CRV_HIP(c, hipGetLastError());
   CRV_HIP(c, hipStreamSynchronize(s));
+  return CRV_OK;
+}
+
+
+namespace {
+
+// events of a fixed-capacity call: the runs of real event keys (the padding rows, key INT64_MAX, form one more run);
+// the pulse counts past them are cleared, so the event offsets of the padding events all equal the pulse total
+__global__ void k_fix_events(const int32_t *__restrict__ nruns, const int64_t *__restrict__ np, int64_t n_cap,
+                             int32_t *__restrict__ cnt, int32_t *__restrict__ nev_dev) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = *np;
+  const int32_t nev = n == 0 ? 0 : *nruns - (n < n_cap ? 1 : 0);
+  if (i == 0) *nev_dev = nev;
+  if (i >= nev && i <= n_cap) cnt[i] = 0;
+}
+
+__global__ void k_zero_i32(int32_t *__restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_cluster_sizes(const int64_t *__restrict__ clo, const int64_t *__restrict__ mo, int64_t nev_cap,
+                                const int *__restrict__ skip, int64_t *__restrict__ sz) {
+  sz[1] = *skip ? 0 : clo[nev_cap];
+  sz[2] = *skip ? 0 : mo[nev_cap];
+}
+
+}  // namespace
+
+// clusters_stage over a fixed capacity of n_cap pulse rows, rows past the call's pulses carrying the event key
+// INT64_MAX: every event of the capacity runs through the size-class finders (the padding events are empty and find
+// nothing), the block finder takes its events from a queue sorted over the whole capacity (non-queue entries have
+// key 0 and end it); outputs at the plan's capacity.
+int clusters_fixed(Ctx *c, Plan &pl, Arena &ar, const int64_t *ev, const int32_t *bar, const int32_t *sipm,
+                   const double *time, const double *pe, const int64_t *seq, hipStream_t s) {
+  if (!c->t.cuts) return set_error(c, CRV_E_STATE, "clusters: geometry not uploaded");
+  if (c->t.nbars + 1 > BH || 8 * c->t.ncit + 1 > BH)
+    return set_error(c, CRV_E_ARG, "clusters: geometry too large for the block finder's bar / column buckets");
+  const int64_t n = pl.np_cap, nev = n + 1;  // event capacity: every pulse its own event, and the padding event
+  auto A = [&](size_t b) { return ar.get(b); };
+  int64_t *evk = (int64_t *)A(nev * 8);
+  int32_t *cnt = (int32_t *)A((nev + 1) * 4);
+  int32_t *nruns = (int32_t *)A(8), *nev_dev = (int32_t *)A(8);
+  int64_t *eoff = (int64_t *)A((nev + 1) * 8);
+  const int64_t ns = n + nev + 2;
+  int32_t *ncl = (int32_t *)A((nev + 1) * 4), *nmem = (int32_t *)A((nev + 1) * 4);
+  int64_t *cl_ev = (int64_t *)A((n + 1) * 8);
+  int32_t *cl_type = (int32_t *)A((n + 1) * 4), *cl_pc = (int32_t *)A((n + 1) * 4);
+  int32_t *cl_sh = (int32_t *)A((n + 1) * 8);
+  double *cl_d = (double *)A((n + 1) * 12 * 8);
+  int32_t *cl_moff = (int32_t *)A(ns * 4), *mem = (int32_t *)A((n + 1) * 4);
+  int64_t *clo = (int64_t *)A((nev + 1) * 8), *mo = (int64_t *)A((nev + 1) * 8);
+  BS b{};
+  double **bd[] = {&b.t, &b.pe, &b.x, &b.y, &b.maxd, &b.wm, &b.cmaxd, &b.wq, &b.cdt, &b.px, &b.pt, &b.pmd};
+  for (auto q : bd) *q = (double *)A(ns * 8);
+  int32_t **bi[] = {&b.bar, &b.sec, &b.cit, &b.hl, &b.filt, &b.items, &b.off, &b.coinc, &b.byl, &b.cl2, &b.off2,
+                    &b.bsort, &b.tsort, &b.label, &b.pass, &b.chk, &b.sk, &b.sc, &b.clmc, &b.tmp, &b.clid, &b.tbin};
+  for (auto q : bi) *q = (int32_t *)A(ns * 4);
+  uint8_t **bu[] = {&b.layer, &b.side, &b.sipm, &b.seq0, &b.kept, &b.fpass, &b.memf, &b.pside};
+  for (auto q : bu) *q = (uint8_t *)A(ns + 8);
+  uint32_t *qk = (uint32_t *)A(nev * 4 + 4), *qk2 = (uint32_t *)A(nev * 4 + 4);
+  int32_t *qi = (int32_t *)A(nev * 4 + 4), *qo = (int32_t *)A(nev * 4 + 4);
+  int *qctr = (int *)A(64);
+  bool ok = evk && cnt && nruns && nev_dev && eoff && ncl && nmem && cl_ev && cl_type && cl_pc && cl_sh && cl_d &&
+            cl_moff && mem && clo && mo && qk && qk2 && qi && qo && qctr;
+  for (auto q : bd) ok = ok && *q;
+  for (auto q : bi) ok = ok && *q;
+  for (auto q : bu) ok = ok && *q;
+  if (!ok) return set_error(c, CRV_E_HIP, "graph: finder buffer allocation failed");
+  size_t tb = 0;
+  CRV_HIP(c, hipcub::DeviceRunLengthEncode::Encode(nullptr, tb, ev, evk, cnt, nruns, (int)n, s));
+  void *tmp = A(tb);
+  if (!tmp) return set_error(c, CRV_E_HIP, "graph: run-length buffer allocation failed");
+  CRV_HIP(c, hipcub::DeviceRunLengthEncode::Encode(tmp, tb, ev, evk, cnt, nruns, (int)n, s));
+  k_fix_events<<<nblocks(nev + 1, 256), 256, 0, s>>>(nruns, pl.sizes, n, cnt, nev_dev);
+  CRV_HIP(c, hipGetLastError());
+  int rc = widen_scan_fixed(c, ar, cnt, eoff, nev, s);
+  if (rc) return rc;
+  Ev e{bar, sipm, seq};
+  const ClSlots slots{cl_ev, cl_type, cl_pc, cl_sh, cl_d};
+  const int grid = (int)std::min<int64_t>(nev, 16384);
+  k_zero_i32<<<1, 64, 0, s>>>(qctr, 16);
+  k_zero_i32<<<nblocks(nev + 1, 256), 256, 0, s>>>((int32_t *)qk, nev + 1);
+  k_zero_i32<<<nblocks(nev + 1, 256), 256, 0, s>>>(ncl, nev + 1);   // events past the call's find nothing
+  k_zero_i32<<<nblocks(nev + 1, 256), 256, 0, s>>>(nmem, nev + 1);
+  k_block_keys<<<nblocks(nev, 256), 256, 0, s>>>(nev, eoff, (int64_t)257, qk, qi, qctr + 1, nev_dev);
+  CRV_HIP(c, hipGetLastError());
+  size_t qtb = 0;
+  CRV_HIP(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, qtb, qk, qk2, qi, qo, (int)nev, 0, 16, c->aux[1]));
+  void *qtmp = A(qtb);
+  if (!qtmp) return set_error(c, CRV_E_HIP, "graph: queue sort buffer allocation failed");
+  CRV_HIP(c, hipEventRecord(c->fork[0], s));
+  CRV_HIP(c, hipStreamWaitEvent(c->aux[0], c->fork[0], 0));
+  CRV_HIP(c, hipStreamWaitEvent(c->aux[1], c->fork[0], 0));
+  k_coinc_wave<64><<<grid, 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff, mem,
+                                       (int64_t)0, c->flags, nev_dev);
+  const hipError_t e0 = hipGetLastError();
+  k_coinc_wave<128><<<grid, 64, 0, c->aux[0]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff,
+                                                mem, (int64_t)65, c->flags, nev_dev);
+  const hipError_t e1 = hipGetLastError();
+  k_coinc_wave<256><<<std::min(grid, 4096), 64, 0, c->aux[1]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem,
+                                                                slots, cl_moff, mem, (int64_t)129, c->flags, nev_dev);
+  const hipError_t e2 = hipGetLastError();
+  hipError_t e3 = hipcub::DeviceRadixSort::SortPairsDescending(qtmp, qtb, qk, qk2, qi, qo, (int)nev, 0, 16, c->aux[1]);
+  if (e3 == hipSuccess) {
+    k_coinc_block<<<2048, BT, 0, c->aux[1]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, b, ncl, nmem, slots, cl_moff,
+                                             mem, (int64_t)257, qo, qk2, qctr, c->flags);
+    e3 = hipGetLastError();
+  }
+  CRV_HIP(c, hipEventRecord(c->fork[1], c->aux[0]));
+  CRV_HIP(c, hipEventRecord(c->fork[2], c->aux[1]));
+  CRV_HIP(c, hipStreamWaitEvent(s, c->fork[1], 0));
+  CRV_HIP(c, hipStreamWaitEvent(s, c->fork[2], 0));
+  CRV_HIP(c, e0);
+  CRV_HIP(c, e1);
+  CRV_HIP(c, e2);
+  CRV_HIP(c, e3);
+  if ((rc = widen_scan_fixed(c, ar, ncl, clo, nev, s))) return rc;
+  if ((rc = widen_scan_fixed(c, ar, nmem, mo, nev, s))) return rc;
+  k_cluster_sizes<<<1, 1, 0, s>>>(clo, mo, nev, pl.skip, pl.sizes);
+  crv_clusters C = pl.C;
+  C.capacity = n;
+  C.mem_capacity = n;
+  k_coinc_out<<<nblocks(nev, 256), 256, 0, s>>>(nev, eoff, clo, mo, ncl, cl_ev, cl_type, cl_pc, cl_sh, cl_d, cl_moff,
+                                                mem, C, nev_dev);
+  CRV_HIP(c, hipGetLastError());
   return CRV_OK;
 }
 

@@ -371,6 +371,95 @@ int group_steps(Ctx *c, const crv_steps *st, StepGroups &g, hipStream_t s) {
   return CRV_OK;
 }
 
+namespace {
+
+// k_step_keys over the capacity of a fixed-capacity call: steps past the call's count get a key above every real one
+// (bit 45), so they sort last and form one run of their own
+__global__ void k_step_keys_fixed(const uint32_t *__restrict__ ev, const uint32_t *__restrict__ counter,
+                                  const int64_t *__restrict__ n_dev, int64_t n_cap, uint64_t *__restrict__ key,
+                                  int32_t *__restrict__ idx, int32_t *__restrict__ evfirst, int *__restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_cap) return;
+  const int64_t n = *n_dev;
+  idx[i] = (int32_t)i;
+  if (i >= n) {
+    key[i] = 1ull << 45;
+    return;
+  }
+  const uint32_t e = ev[i];
+  int64_t lo = 0, hi = i;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (ev[mid] < e) lo = mid + 1; else hi = mid;
+  }
+  evfirst[i] = (int32_t)lo;
+  if (i > 0 && ev[i - 1] > e) atomicAdd(&flags[F_UNSORTED], 1);
+  const uint32_t c = counter[i];
+  if (c >= 8192u) atomicAdd(&flags[F_COUNTER], 1);
+  key[i] = ((uint64_t)e << 13) | (uint64_t)(c & 8191u);
+}
+
+// hit counters = runs of real keys (the padding forms one more run when the call is below capacity); the run counts
+// past them are cleared so that the task offsets end at the last real step
+__global__ void k_fix_tasks(const int32_t *__restrict__ nruns, const int64_t *__restrict__ n_dev, int64_t n_cap,
+                            int32_t *__restrict__ T_dev, int32_t *__restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = *n_dev;
+  const int32_t T = n == 0 ? 0 : *nruns - (n < n_cap ? 1 : 0);
+  if (i == 0) *T_dev = T;
+  if (i >= T && i <= n_cap) cnt[i] = 0;
+}
+
+__global__ void k_zero_flags(int *__restrict__ f, int n) {
+  const int i = threadIdx.x;
+  if (i < n) f[i] = 0;
+}
+
+}  // namespace
+
+int group_steps_fixed(Ctx *c, Plan &pl, Arena &ar, StepGroups &g, hipStream_t s) {
+  const int64_t n = pl.n_cap;
+  g.n_steps = n;
+  g.ev = pl.in_event;
+  g.counter = pl.in_counter;
+  g.edep = pl.in_edep;
+  g.z = pl.in_z;
+  g.t = pl.in_t;
+  g.T = n;  // capacity; the call's count is in T_dev
+  uint64_t *key = (uint64_t *)ar.get(n * 8), *key_s = (uint64_t *)ar.get(n * 8);
+  int32_t *idx = (int32_t *)ar.get(n * 4);
+  g.perm = (int32_t *)ar.get(n * 4);
+  g.evfirst = (int32_t *)ar.get(n * 4);
+  g.task_key = (uint64_t *)ar.get(n * 8);
+  int32_t *cnt = (int32_t *)ar.get((n + 1) * 4);
+  g.task_off = (int32_t *)ar.get((n + 1) * 4);
+  int32_t *nruns = (int32_t *)ar.get(8);
+  if (!key || !key_s || !idx || !g.perm || !g.evfirst || !g.task_key || !cnt || !g.task_off || !nruns)
+    return set_error(c, CRV_E_HIP, "graph: step grouping allocation failed");
+  k_zero_flags<<<1, 64, 0, s>>>(c->flags, F_NFLAGS);
+  k_step_keys_fixed<<<nblocks(n, 256), 256, 0, s>>>(pl.in_event, pl.in_counter, pl.n_dev, n, key, idx, g.evfirst,
+                                                   c->flags);
+  CRV_HIP(c, hipGetLastError());
+  size_t tb = 0;
+  CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key_s, idx, g.perm, (int)n, 0, 46, s));
+  void *tmp = ar.get(tb);
+  if (!tmp) return set_error(c, CRV_E_HIP, "graph: sort buffer allocation failed");
+  CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, key_s, idx, g.perm, (int)n, 0, 46, s));
+  size_t tb2 = 0;
+  CRV_HIP(c, hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, key_s, g.task_key, cnt, nruns, (int)n, s));
+  void *tmp2 = ar.get(tb2);
+  if (!tmp2) return set_error(c, CRV_E_HIP, "graph: run-length buffer allocation failed");
+  CRV_HIP(c, hipcub::DeviceRunLengthEncode::Encode(tmp2, tb2, key_s, g.task_key, cnt, nruns, (int)n, s));
+  k_fix_tasks<<<nblocks(n + 1, 256), 256, 0, s>>>(nruns, pl.n_dev, n, pl.T_dev, cnt);
+  CRV_HIP(c, hipGetLastError());
+  size_t tb3 = 0;
+  CRV_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, cnt, g.task_off, (int)(n + 1), s));
+  void *tmp3 = ar.get(tb3);
+  if (!tmp3) return set_error(c, CRV_E_HIP, "graph: scan buffer allocation failed");
+  CRV_HIP(c, hipcub::DeviceScan::ExclusiveSum(tmp3, tb3, cnt, g.task_off, (int)(n + 1), s));
+  return CRV_OK;
+}
+
 int charges_stage(Ctx *c, StepGroups &g, crv_charges *out, hipStream_t s) {
   const int64_t T = g.T, L = T * 4;
   out->n_photons = 0;
