@@ -87,10 +87,11 @@ struct Ctx {
   std::vector<hipEvent_t> pev;               // per-pass events of the pipelined batched chain (grown on demand)
   hipEvent_t fork[3];                       // fork / join events of the side streams
   bool kev_on = false;
-  int64_t b_rec_cap = 0, b_run_cap = 0;  // pool sizes of the batched chain (grown on demand)
+  int64_t b_rec_cap = 0, b_run_cap = 0, b_desc_cap = 0;  // pool sizes of the batched chain (grown on demand)
   int b_log_factor = 1;  // fired-pixel log capacity factor of the batched chain, doubled on overflow and kept
   int b_pcap = 0;        // pulse slots per waveform the batched chain grew to (kept for later calls)
   int64_t b_last_log = 0;  // 16-byte chunks of the whole fired-pixel log of the last batched call
+  int64_t b_last_desc = 0; // kept charge clusters of the last batched call
   std::vector<int> ws_spare;  // slots the running call no longer reads: freed to make room when an allocation fails
   int ws_fail_slot = -1;                  // the workspace request that failed last
   size_t ws_fail_bytes = 0;
@@ -165,8 +166,8 @@ struct Arena {
 // is a capacity known at capture; the counts of a call live in device memory, the kernels read them there, and
 // whatever exceeds a capacity sets `skip`, which empties the rest of the call (the host runs it uncaptured instead).
 struct Plan {
-  int64_t n_cap = 0, log_cap = 0, rec_cap = 0, run_cap = 0, np_cap = 0;
-  int pcap = 4, log_factor = 1, tile = 64, sort_tasks = 1, grid_s = 0, grid_w = 0;
+  int64_t n_cap = 0, log_cap = 0, rec_cap = 0, run_cap = 0, np_cap = 0, desc_cap = 0;
+  int pcap = 4, log_factor = 1, tile = 64, sort_tasks = 1, grid_s = 0, grid_r = 0;
   // the graph's inputs: steps at capacity and their count
   uint32_t *in_event = nullptr, *in_counter = nullptr;
   double *in_edep = nullptr, *in_z = nullptr, *in_t = nullptr;

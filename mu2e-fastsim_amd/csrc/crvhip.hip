@@ -779,7 +779,7 @@ int crv_ctx_trim(crv_ctx *ctx, uint64_t *freed) {
   c->last_pulses = crv_pulses{};
   c->last_clusters = crv_clusters{};
   c->last_digis = crv_digis{};
-  c->b_rec_cap = c->b_run_cap = 0;
+  c->b_rec_cap = c->b_run_cap = c->b_desc_cap = 0;
   c->b_pcap = 0;
   if (freed) *freed = b;
   return CRV_OK;
@@ -819,11 +819,12 @@ int crv_graph_create(crv_ctx *ctx, const crv_steps *sample, int64_t max_steps, c
   pl.rec_cap = cap(tm.n_records);
   pl.run_cap = cap(tm.n_waveforms);
   pl.np_cap = cap(tm.n_pulses);
+  pl.desc_cap = cap(c->b_last_desc);
   pl.pcap = std::max(so.pulse_slots_used, 4);
   pl.log_factor = c->b_log_factor;
   pl.tile = so.tile;
   pl.sort_tasks = so.sort_tasks;
-  pl.grid_s = pl.grid_w = so.grid;
+  pl.grid_s = pl.grid_r = so.grid;
   auto fail = [&](int code, const std::string &m) {
     (void)crv_graph_destroy(G);
     return set_error(c, code, m);

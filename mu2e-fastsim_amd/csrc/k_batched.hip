@@ -1,11 +1,13 @@
-// The FP32 batched chain: steps -> pulses in three persistent / data-parallel kernels, then the coincidence finder.
+// The FP32 batched chain: steps -> pulses in four persistent / data-parallel kernels, then the coincidence finder.
 //
 //  kb_sipm  one lane per SiPM: the photon streams of the counter's steps (ordered exponential delays) -> pixel,
 //           recovery, firing, cross-talk cascade (reference response/sipm.py:141-198) -> the SiPM's fired-pixel log
-//           (time, overvoltage); no photon or charge table.
-//  kb_wave  one lane per SiPM: its fired pixels in time order -> charge clusters -> a 16-sample register window of
-//           template sums, noise, ADC words and zero suppression (reference reco/waveform.py:96-172) -> the ADC record
-//           of each cluster with digis (record pool) and its reco waveforms, the runs of continuing digis (run pool).
+//           (time, overvoltage), and its charge clusters as they are logged (gaps of more than the template's
+//           length plus four periods) -> a descriptor per cluster that can reach the zero-suppression threshold;
+//           no photon or charge table.
+//  kb_wrec  one lane per cluster descriptor: a 16-sample register window of template sums, noise, ADC words and zero
+//           suppression (reference reco/waveform.py:96-172) -> the cluster's ADC record (record pool) and its reco
+//           waveforms, the runs of continuing digis (run pool).
 //  kb_fit   one lane per reco waveform: peak finding and the Gumbel fit (reference reco/pulsefit.py:236-316).
 // The SiPM kernels take tiles of SiPMs from an atomic work queue (persistent grid), optionally longest first (photon
 // count), so that the lanes of a wave carry similar work. A run map (SiPM, run number) -> pool slot keeps every
@@ -280,6 +282,11 @@ struct alignas(16) Slot {
   uint32_t sf;  // seq | failed << 16 | zerondf << 17 | pedfromdb << 18
 };
 
+struct ClDesc {  // one charge cluster of a SiPM whose waveform is kept: its log entries [q0, q1)
+  int64_t lb;     // the SiPM's log base (goff of its group + its lane)
+  int32_t g, q0, q1, pad_;
+};
+
 struct Out {
   // fired-pixel log: entries (time - tbase of the SiPM, overvoltage) in chunks of two (16 bytes); per group of 64
   // SiPMs in queue order, chunk c of the group's lane l at goff + 64 c + l (the lanes of a wavefront write and read
@@ -302,6 +309,7 @@ struct Out {
   int64_t *rec_label;
   int32_t *rec_tick;
   int32_t *rec_count;
+  int32_t *rec_used;  // records that hold a waveform with digis (the others stay reserved with label -1)
   int64_t rec_cap;
   // run pool: the reco waveforms (record, first sample, samples, SiPM, run number in the SiPM, start TDC)
   int32_t *run_rec, *run_i0, *run_n, *run_k;
@@ -309,6 +317,15 @@ struct Out {
   int32_t *run_count;
   int64_t run_cap;
   int32_t *nrun;  // per SiPM
+  // charge clusters whose waveform is kept: per SiPM its first descriptor and their count; the descriptor pool (the
+  // work units of kb_wrec) and the descriptors reserved so far; per descriptor its reco waveforms and their offset
+  // within the SiPM; per run its descriptor
+  int32_t *cd0, *ncls;
+  struct ClDesc *cdesc;
+  int32_t *cdesc_count;
+  int64_t cdesc_cap;
+  int32_t *ncr, *crl;
+  int32_t *run_c;
   // pulse slots per pool run (one 48-byte record each), and per final run the pulse count and the pool run
   int pcap;
   Slot *slot;
@@ -371,8 +388,61 @@ __device__ __forceinline__ void log_fire(const Out &o, int64_t lb, int64_t &nlog
   ++nlog;
 }
 
+// the kept charge clusters of one SiPM's fired-pixel log (entries [0, nf) at lb): clusters split at gaps of more than
+// `gap`, kept when nq * vmax >= vmin (nq: the cluster's charges); the first CL_REG of them in (qs, qe), the ones from
+// `emit_from` on written to dst; returns the number kept
+constexpr int CL_REG = 4;
+__device__ __forceinline__ int scan_clusters(const Out &o, int64_t lb, int nf, double tb0, double gap, float vmax,
+                                             float vmin, int32_t qs[CL_REG], int32_t qe[CL_REG], int emit_from,
+                                             ClDesc *dst, int32_t g) {
+  int q = 0, q_open = 0, nq = 0, m = 0;
+  bool open = false;
+  double t_last = 0.0;
+  float4 ck = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (nf > 0) ck = o.flog[lb];
+  auto emit = [&](int q1) {
+    if ((float)nq * vmax >= vmin) {
+#pragma unroll
+      for (int k = 0; k < CL_REG; ++k)
+        if (k == m) {
+          qs[k] = q_open;
+          qe[k] = q1;
+        }
+      if (dst && m >= emit_from) dst[m] = ClDesc{lb, g, q_open, q1, 0};
+      ++m;
+    }
+  };
+  while (q < nf) {
+    const float tqf = (q & 1) ? ck.z : ck.x;
+    const double tq = tb0 + (double)tqf;
+    int n = 1;
+    for (;;) {
+      const int e = q + n;
+      if (e >= nf) break;
+      if ((e & 1) == 0) ck = o.flog[lb + (int64_t)(e >> 1) * 64];
+      if (((e & 1) ? ck.z : ck.x) != tqf) break;
+      ++n;
+    }
+    if (open && tq - t_last > gap) {
+      emit(q);
+      open = false;
+    }
+    if (!open) {
+      open = true;
+      q_open = q;
+      nq = 0;
+    }
+    nq += n;
+    t_last = tq;
+    q += n;
+  }
+  if (open) emit(nf);
+  return m;
+}
+
 // kb_sipm, one lane per SiPM: photons of the counter's steps in time order -> pixel, recovery, firing, cross-talk
-// cascade -> the SiPM's fired-pixel log (time, overvoltage, pixel), in firing order.
+// cascade -> the SiPM's fired-pixel log (time, overvoltage, pixel), in firing order, and a descriptor for each of its
+// kept charge clusters (cd0 / ncls: the SiPM's first descriptor and their count).
 __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv, BStep st,
                                               const uint64_t *__restrict__ task_key,
                                               const int32_t *__restrict__ task_off, const int32_t *__restrict__ perm,
@@ -393,330 +463,392 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
   uint8_t *xmult = o.xmult + xb;
   const int64_t xs = o.xstride;
   float *lastt = o.lastt + xb;  // read only where the pixel's bitmap bit is set: no clearing between SiPMs
+  // charge clusters: the split rule of the waveform window (a gap of more than single_pe_max_time + 4 periods after
+  // the last logged time closes a cluster), kept when its charges reach the zero-suppression threshold
+  const double gap = p.single_pe_max_time + 4.0 * PERIODD;
+  const float vmax = (float)t.vmax, vmin = (float)p.min_voltage;
   int64_t base = 0;
   while (grab(qu, &tile_base, base)) {
-    for (int64_t gq = base + lane; gq < base + qu.tile && gq < qu.n; gq += 64) {
-      const int64_t g = qu.order ? (int64_t)qu.order[gq] : gq;
-      const int64_t tk = g >> 2;
-      const int s = (int)(g & 3);
-      const uint64_t key = task_key[tk];
-      const uint32_t ev = (uint32_t)(key >> 13);
-      const int ch = (int)(key & 8191u) * 4 + s;
-      if (ch >= t.n_channels || (t.status[ch] & 0b101) != 0) {
-        nfired[g] = 0;
-        continue;
-      }
-      const int a = task_off[tk], b = task_off[tk + 1];
-      const int nstr = b - a;
-      // the bitmap starts as the dead-pixel mask: a clear bit (the usual case) is a live pixel that has not fired,
-      // so only a set bit needs the dead table (dead: the photon is lost) or the pixel's last firing
-      for (int w = 0; w < CRV_DEAD_WORDS; ++w) bm[w * 64 + lane] = t.dead[w];
-      const int64_t lb = o.goff[gq >> 6] - o.log0 + (gq & 63), lcap = o.gcap[gq >> 6];
-      const double tbase = st.t[perm[a]];  // log times are FP32 offsets from it (they span microseconds)
-      o.tbase[g] = tbase;
+    for (int64_t gq0 = base; gq0 < base + qu.tile && gq0 < qu.n; gq0 += 64) {
+      const int64_t gq = gq0 + lane;
+      const int64_t g = gq < qu.n ? (qu.order ? (int64_t)qu.order[gq] : gq) : -1;
+      const int64_t lb = o.goff[gq0 >> 6] - o.log0 + lane;
+      double tbase = 0.0;
       int64_t nlog = 0;
-      float pend_t = 0.f, pend_v = 0.f;  // the log's pending even entry
-      // photon streams: in registers for a counter with one step, in global memory otherwise
-      float S1 = 0.f;
-      double head1 = 0.0, tb1 = 0.0;
-      int r1 = 0, N1 = 0;
-      uint32_t j1 = 0;
-      U4 dq1{0u, 0u, 0u, 0u};  // the delay draw holding the stream's next four exponentials
-      for (int k = a; k < b; ++k) {
-        const int64_t gi = (int64_t)k * 4 + s;
-        const int N = nph[gi];
-        if (nstr > 1 && N <= 0) {
-          o.gRJ[gi] = 0;
-          o.gHead[gi] = INFINITY;  // an empty or exhausted stream's head is +inf
-        }
-        if (N <= 0) continue;
-        const int32_t i = perm[k];
-        const uint32_t j = (uint32_t)(i - evfirst[i]);
-        const double z = st.z[i];
-        const double d = (s & 1) == 0 ? z + p.half_length : p.half_length - z;
-        const double tb = st.t[i] + d * p.inv_speed;
-        const U4 d0 = dr(KIND_DELAY, ev, (uint32_t)ch, j, 0u);
-        const float S = -__logf(u32f(d0.x)) / (float)N;
-        if (nstr == 1) {
-          dq1 = d0;
-          N1 = N;
-          j1 = j;
-          tb1 = tb;
-          S1 = S;
-          head1 = tb + (double)(tauf * S);
-        } else {
-          o.gS[gi] = S;
-          o.gTb[gi] = tb;
-          o.gHead[gi] = tb + (double)(tauf * S);
-          o.gRJ[gi] = (uint64_t)j << 32;
-        }
-      }
-      bool bad = false;
-      for (;;) {
-        // next photon: the earliest stream head (lowest step first on ties)
-        int kb = -1;
-        double tph = 0.0;
-        if (nstr == 1) {
-          if (r1 < N1) {
-            kb = a;
-            tph = head1;
-          }
-        } else {  // one load per stream, four streams' loads in flight at a time
-          tph = INFINITY;
-          int k = a;
-          for (; k + 8 <= b; k += 8) {
-            double h[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) h[q] = o.gHead[(int64_t)(k + q) * 4 + s];
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-              if (h[q] < tph) {
-                kb = k + q;
-                tph = h[q];
-              }
-          }
-          for (; k + 4 <= b; k += 4) {
-            const double h0 = o.gHead[(int64_t)k * 4 + s], h1 = o.gHead[(int64_t)(k + 1) * 4 + s];
-            const double h2 = o.gHead[(int64_t)(k + 2) * 4 + s], h3 = o.gHead[(int64_t)(k + 3) * 4 + s];
-            if (h0 < tph) {
-              kb = k;
-              tph = h0;
-            }
-            if (h1 < tph) {
-              kb = k + 1;
-              tph = h1;
-            }
-            if (h2 < tph) {
-              kb = k + 2;
-              tph = h2;
-            }
-            if (h3 < tph) {
-              kb = k + 3;
-              tph = h3;
-            }
-          }
-          for (; k < b; ++k) {
-            const double h = o.gHead[(int64_t)k * 4 + s];
-            if (h < tph) {
-              kb = k;
-              tph = h;
-            }
-          }
-        }
-        if (kb < 0 || tph > p.photon_window_hi) break;  // later photons of every stream are later still
-        const float toff = (float)(tph - tbase);
-        // advance the stream
-        uint32_t j;
-        int r;
-        if (nstr == 1) {
-          j = j1;
-          r = r1;
-          if (r1 + 1 < N1) {
-            const int rn = r1 + 1;
-            if ((rn & 3) == 0) dq1 = dr(KIND_DELAY, ev, (uint32_t)ch, j1, (uint32_t)(rn >> 2));
-            const uint32_t x = (rn & 3) == 0 ? dq1.x : (rn & 3) == 1 ? dq1.y : (rn & 3) == 2 ? dq1.z : dq1.w;
-            S1 += (-__logf(u32f(x))) / (float)(N1 - rn);
-            head1 = tb1 + (double)(tauf * S1);
-          }
-          r1 += 1;
-        } else {
-          const int64_t gi = (int64_t)kb * 4 + s;
-          const uint64_t rj = o.gRJ[gi];  // one load for the step ordinal and the rank
-          j = (uint32_t)(rj >> 32);
-          r = (int)(uint32_t)rj;
-          const int N = nph[gi];
-          if (r + 1 < N) {
-            const int rn = r + 1;
-            const U4 uu = dr(KIND_DELAY, ev, (uint32_t)ch, j, (uint32_t)(rn >> 2));
-            const uint32_t x = (rn & 3) == 0 ? uu.x : (rn & 3) == 1 ? uu.y : (rn & 3) == 2 ? uu.z : uu.w;
-            const float S = o.gS[gi] + (-__logf(u32f(x))) / (float)(N - rn);
-            o.gS[gi] = S;
-            o.gHead[gi] = o.gTb[gi] + (double)(tauf * S);
-          } else {
-            o.gHead[gi] = INFINITY;
-          }
-          o.gRJ[gi] = rj + 1u;
-        }
-        // pixel of the photon (photon-map CDF), firing with the recovered overvoltage
-        const U4 u = dr(KIND_PIXEL, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, 0u, 0u));
-        const double xc = unif(u.x) * t.cdf_total;
-        int bin;
-        if (t.map_uniform) {
-          bin = min((int)xc, t.ncdf - 1);
-        } else {
-          int lo = 0, hi = t.ncdf;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (t.cdf[mid] <= xc) lo = mid + 1;
-            else hi = mid;
-          }
-          bin = min(lo, t.ncdf - 1);
-        }
-        int px = (int)(((float)bin + 0.5f) * inv_ny);  // bin < 1600: the float quotient's floor is exact
-        int py = bin - px * t.map_ny;
-        if (p.pixel_lrint) {
-          px += u32f(u.y) >= 0.5f ? 1 : 0;
-          py += u32f(u.z) >= 0.5f ? 1 : 0;
-        }
-        const int pix = px * CRV_NPIX + py;
-        const bool hit_set = bget(bm, lane, pix);
-        if (hit_set && dead_px(t, pix)) continue;
-        float v = Vov;
-        if (hit_set) v = Vov * (1.f - __expf(-(toff - lastt[pix * xs]) * inv_tau));
-        const float pf = v == Vov ? pf_full : p1 * (1.f - __expf(-v * inv_p2));
-        if (!(u32f(u.w) < pf)) continue;
-        if (nlog >= lcap) {
-          bad = true;
+      int m = 0;  // kept clusters; the first CL_REG of them in (qs, qe)
+      int32_t qs[CL_REG], qe[CL_REG];
+      do {  // the lane's SiPM (none past the end of the queue)
+        if (g < 0) break;
+        const int64_t tk = g >> 2;
+        const int s = (int)(g & 3);
+        const uint64_t key = task_key[tk];
+        const uint32_t ev = (uint32_t)(key >> 13);
+        const int ch = (int)(key & 8191u) * 4 + s;
+        if (ch >= t.n_channels || (t.status[ch] & 0b101) != 0) {
+          nfired[g] = 0;
           break;
         }
-        log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, pix, toff, v);
-        bset(bm, lane, pix);
-        // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order. While a
-        // generation's pixels stay inside the 7 x 7 window around the photon's pixel, the generation, its candidates
-        // and their multiplicities are bit masks over the window in registers (bit (dx + 3) * 7 + dy + 3; ascending
-        // bit = ascending pixel); a generation with a pixel on the window's border continues in the per-lane lists
-        // in global scratch below. Both visit the same draws in the same order.
-        const int cx = pix / CRV_NPIX, cy = pix % CRV_NPIX;
-        uint64_t cur = 1ull << 24;  // generation 0: the photon's pixel at the window's centre
-        int gnum = 0;
-        bool window = true;
-        while (!bad) {
-          if (cur & WIN_BORDER) {
-            window = false;  // neighbours could leave the window
-            break;
+        const int a = task_off[tk], b = task_off[tk + 1];
+        const int nstr = b - a;
+        // the bitmap starts as the dead-pixel mask: a clear bit (the usual case) is a live pixel that has not fired,
+        // so only a set bit needs the dead table (dead: the photon is lost) or the pixel's last firing
+        for (int w = 0; w < CRV_DEAD_WORDS; ++w) bm[w * 64 + lane] = t.dead[w];
+        const int64_t lcap = o.gcap[gq0 >> 6];
+        tbase = st.t[perm[a]];  // log times are FP32 offsets from it (they span microseconds)
+        o.tbase[g] = tbase;
+        int q_open = 0;      // first entry of the open cluster
+        double c_last = 0.0; // time of the last logged entry
+        auto close_cluster = [&](int q1) {
+          if ((float)(q1 - q_open) * vmax >= vmin) {
+  #pragma unroll
+            for (int k = 0; k < CL_REG; ++k)
+              if (k == m) {
+                qs[k] = q_open;
+                qe[k] = q1;
+              }
+            ++m;
           }
-          uint64_t m0 = 0, m1 = 0, m2 = 0;  // candidate multiplicity, bit planes of a 3-bit count per position
-          for (uint64_t rem = cur; rem; rem &= rem - 1) {
-            const int bq = __ffsll((long long)rem) - 1;
-            const int fx = cx + bq / 7 - 3, fy = cy + bq % 7 - 3;
-            const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j,
-                             pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)(fx * CRV_NPIX + fy)));
-            uint64_t add = 0;
-            if (fx > 0 && u32f(ux.x) < qct) add |= 1ull << (bq - 7);
-            if (fx + 1 < p.n_pixels_x && u32f(ux.y) < qct) add |= 1ull << (bq + 7);
-            if (fy > 0 && u32f(ux.z) < qct) add |= 1ull << (bq - 1);
-            if (fy + 1 < p.n_pixels_y && u32f(ux.w) < qct) add |= 1ull << (bq + 1);
-            const uint64_t c0 = m0 & add;
-            m0 ^= add;
-            const uint64_t c1 = m1 & c0;
-            m1 ^= c0;
-            m2 ^= c1;
+        };
+        float pend_t = 0.f, pend_v = 0.f;  // the log's pending even entry
+        // photon streams: in registers for a counter with one step, in global memory otherwise
+        float S1 = 0.f;
+        double head1 = 0.0, tb1 = 0.0;
+        int r1 = 0, N1 = 0;
+        uint32_t j1 = 0;
+        U4 dq1{0u, 0u, 0u, 0u};  // the delay draw holding the stream's next four exponentials
+        for (int k = a; k < b; ++k) {
+          const int64_t gi = (int64_t)k * 4 + s;
+          const int N = nph[gi];
+          if (nstr > 1 && N <= 0) {
+            o.gRJ[gi] = 0;
+            o.gHead[gi] = INFINITY;  // an empty or exhausted stream's head is +inf
           }
-          const uint64_t cand = m0 | m1 | m2;
-          if (!cand) break;
-          if (__popcll(cand) > CMAXB || gnum + 1 > MAXGEN) {
-            atomicAdd(&flags[F_CASCADE], 1);
-            bad = true;
-            break;
+          if (N <= 0) continue;
+          const int32_t i = perm[k];
+          const uint32_t j = (uint32_t)(i - evfirst[i]);
+          const double z = st.z[i];
+          const double d = (s & 1) == 0 ? z + p.half_length : p.half_length - z;
+          const double tb = st.t[i] + d * p.inv_speed;
+          const U4 d0 = dr(KIND_DELAY, ev, (uint32_t)ch, j, 0u);
+          const float S = -__logf(u32f(d0.x)) / (float)N;
+          if (nstr == 1) {
+            dq1 = d0;
+            N1 = N;
+            j1 = j;
+            tb1 = tb;
+            S1 = S;
+            head1 = tb + (double)(tauf * S);
+          } else {
+            o.gS[gi] = S;
+            o.gTb[gi] = tb;
+            o.gHead[gi] = tb + (double)(tauf * S);
+            o.gRJ[gi] = (uint64_t)j << 32;
           }
-          ++gnum;
-          uint64_t nxt = 0;
-          for (uint64_t rem = cand; rem; rem &= rem - 1) {
-            const int bq = __ffsll((long long)rem) - 1;
-            const int c = (cx + bq / 7 - 3) * CRV_NPIX + cy + bq % 7 - 3;
-            const int m = (int)((m0 >> bq) & 1u) + 2 * (int)((m1 >> bq) & 1u) + 4 * (int)((m2 >> bq) & 1u);
-            const bool c_set = bget(bm, lane, c);
-            if (c_set && dead_px(t, c)) continue;
-            float vc = Vov;
-            if (c_set) vc = Vov * (1.f - __expf(-(toff - lastt[c * xs]) * inv_tau));
-            const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
-            float qm = q1;
-            for (int mm = 2; mm <= m; ++mm) qm *= q1;
-            const float uc =
-                u32f(dr(KIND_XTFIRE, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)c)).x);
-            if (!(uc < 1.f - qm)) continue;
-            if (nlog >= lcap) {
-              bad = true;
-              break;
+        }
+        bool bad = false;
+        for (;;) {
+          // next photon: the earliest stream head (lowest step first on ties)
+          int kb = -1;
+          double tph = 0.0;
+          if (nstr == 1) {
+            if (r1 < N1) {
+              kb = a;
+              tph = head1;
             }
-            log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, c, toff, vc);
-            bset(bm, lane, c);
-            nxt |= 1ull << bq;
-          }
-          cur = nxt;
-          if (!cur) break;
-        }
-        if (bad) break;
-        if (window) continue;
-        int ncur = 0;
-        for (uint64_t rem = cur; rem; rem &= rem - 1, ++ncur) {
-          const int bq = __ffsll((long long)rem) - 1;
-          xcur[ncur * xs] = (uint16_t)((cx + bq / 7 - 3) * CRV_NPIX + cy + bq % 7 - 3);
-        }
-        while (ncur > 0 && !bad) {
-          int ncand = 0;
-          for (int ai = 0; ai < ncur; ++ai) {
-            const int f = (int)xcur[ai * xs];
-            const int fx = f / CRV_NPIX, fy = f % CRV_NPIX;
-            const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)f));
-            const uint32_t uw[4] = {ux.x, ux.y, ux.z, ux.w};
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-              const int nx = nb == 0 ? fx - 1 : nb == 1 ? fx + 1 : fx;
-              const int ny = nb == 2 ? fy - 1 : nb == 3 ? fy + 1 : fy;
-              const bool ok =
-                  nb == 0 ? fx > 0 : nb == 1 ? fx + 1 < p.n_pixels_x : nb == 2 ? fy > 0 : fy + 1 < p.n_pixels_y;
-              if (!ok || !(u32f(uw[nb]) < qct)) continue;
-              const int c = nx * CRV_NPIX + ny;
-              int at = 0;
-              while (at < ncand && xcand[at * xs] < c) ++at;
-              if (at < ncand && xcand[at * xs] == c) {
-                xmult[at * xs] += 1;
-              } else if (ncand < CMAXB) {
-                for (int q = ncand; q > at; --q) {
-                  xcand[q * xs] = xcand[(q - 1) * xs];
-                  xmult[q * xs] = xmult[(q - 1) * xs];
+          } else {  // one load per stream, four streams' loads in flight at a time
+            tph = INFINITY;
+            int k = a;
+            for (; k + 8 <= b; k += 8) {
+              double h[8];
+  #pragma unroll
+              for (int q = 0; q < 8; ++q) h[q] = o.gHead[(int64_t)(k + q) * 4 + s];
+  #pragma unroll
+              for (int q = 0; q < 8; ++q)
+                if (h[q] < tph) {
+                  kb = k + q;
+                  tph = h[q];
                 }
-                xcand[at * xs] = (uint16_t)c;
-                xmult[at * xs] = 1;
-                ++ncand;
-              } else {
-                atomicAdd(&flags[F_CASCADE], 1);
-                bad = true;
+            }
+            for (; k + 4 <= b; k += 4) {
+              const double h0 = o.gHead[(int64_t)k * 4 + s], h1 = o.gHead[(int64_t)(k + 1) * 4 + s];
+              const double h2 = o.gHead[(int64_t)(k + 2) * 4 + s], h3 = o.gHead[(int64_t)(k + 3) * 4 + s];
+              if (h0 < tph) {
+                kb = k;
+                tph = h0;
+              }
+              if (h1 < tph) {
+                kb = k + 1;
+                tph = h1;
+              }
+              if (h2 < tph) {
+                kb = k + 2;
+                tph = h2;
+              }
+              if (h3 < tph) {
+                kb = k + 3;
+                tph = h3;
+              }
+            }
+            for (; k < b; ++k) {
+              const double h = o.gHead[(int64_t)k * 4 + s];
+              if (h < tph) {
+                kb = k;
+                tph = h;
               }
             }
           }
-          if (ncand == 0 || bad) break;
-          if (++gnum > MAXGEN) {
-            atomicAdd(&flags[F_CASCADE], 1);
+          if (kb < 0 || tph > p.photon_window_hi) break;  // later photons of every stream are later still
+          const float toff = (float)(tph - tbase);
+          // advance the stream
+          uint32_t j;
+          int r;
+          if (nstr == 1) {
+            j = j1;
+            r = r1;
+            if (r1 + 1 < N1) {
+              const int rn = r1 + 1;
+              if ((rn & 3) == 0) dq1 = dr(KIND_DELAY, ev, (uint32_t)ch, j1, (uint32_t)(rn >> 2));
+              const uint32_t x = (rn & 3) == 0 ? dq1.x : (rn & 3) == 1 ? dq1.y : (rn & 3) == 2 ? dq1.z : dq1.w;
+              S1 += (-__logf(u32f(x))) / (float)(N1 - rn);
+              head1 = tb1 + (double)(tauf * S1);
+            }
+            r1 += 1;
+          } else {
+            const int64_t gi = (int64_t)kb * 4 + s;
+            const uint64_t rj = o.gRJ[gi];  // one load for the step ordinal and the rank
+            j = (uint32_t)(rj >> 32);
+            r = (int)(uint32_t)rj;
+            const int N = nph[gi];
+            if (r + 1 < N) {
+              const int rn = r + 1;
+              const U4 uu = dr(KIND_DELAY, ev, (uint32_t)ch, j, (uint32_t)(rn >> 2));
+              const uint32_t x = (rn & 3) == 0 ? uu.x : (rn & 3) == 1 ? uu.y : (rn & 3) == 2 ? uu.z : uu.w;
+              const float S = o.gS[gi] + (-__logf(u32f(x))) / (float)(N - rn);
+              o.gS[gi] = S;
+              o.gHead[gi] = o.gTb[gi] + (double)(tauf * S);
+            } else {
+              o.gHead[gi] = INFINITY;
+            }
+            o.gRJ[gi] = rj + 1u;
+          }
+          // pixel of the photon (photon-map CDF), firing with the recovered overvoltage
+          const U4 u = dr(KIND_PIXEL, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, 0u, 0u));
+          const double xc = unif(u.x) * t.cdf_total;
+          int bin;
+          if (t.map_uniform) {
+            bin = min((int)xc, t.ncdf - 1);
+          } else {
+            int lo = 0, hi = t.ncdf;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (t.cdf[mid] <= xc) lo = mid + 1;
+              else hi = mid;
+            }
+            bin = min(lo, t.ncdf - 1);
+          }
+          int px = (int)(((float)bin + 0.5f) * inv_ny);  // bin < 1600: the float quotient's floor is exact
+          int py = bin - px * t.map_ny;
+          if (p.pixel_lrint) {
+            px += u32f(u.y) >= 0.5f ? 1 : 0;
+            py += u32f(u.z) >= 0.5f ? 1 : 0;
+          }
+          const int pix = px * CRV_NPIX + py;
+          const bool hit_set = bget(bm, lane, pix);
+          if (hit_set && dead_px(t, pix)) continue;
+          float v = Vov;
+          if (hit_set) v = Vov * (1.f - __expf(-(toff - lastt[pix * xs]) * inv_tau));
+          const float pf = v == Vov ? pf_full : p1 * (1.f - __expf(-v * inv_p2));
+          if (!(u32f(u.w) < pf)) continue;
+          if (nlog >= lcap) {
             bad = true;
             break;
           }
-          ncur = 0;
-          for (int ai = 0; ai < ncand; ++ai) {
-            const int c = xcand[ai * xs];
-            const int m = xmult[ai * xs];
-            const bool c_set = bget(bm, lane, c);
-            if (c_set && dead_px(t, c)) continue;
-            float vc = Vov;
-            if (c_set) vc = Vov * (1.f - __expf(-(toff - lastt[c * xs]) * inv_tau));
-            const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
-            float qm = q1;
-            for (int mm = 2; mm <= m; ++mm) qm *= q1;
-            const float uc =
-                u32f(dr(KIND_XTFIRE, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)c)).x);
-            if (!(uc < 1.f - qm)) continue;
-            if (nlog >= lcap) {
+          {  // the photon's firings (its pixel and cross-talk) share its time
+            const double tq = tbase + (double)toff;
+            if (nlog > 0 && tq - c_last > gap) {
+              close_cluster((int)nlog);
+              q_open = (int)nlog;
+            }
+            c_last = tq;
+          }
+          log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, pix, toff, v);
+          bset(bm, lane, pix);
+          // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order. While a
+          // generation's pixels stay inside the 7 x 7 window around the photon's pixel, the generation, its candidates
+          // and their multiplicities are bit masks over the window in registers (bit (dx + 3) * 7 + dy + 3; ascending
+          // bit = ascending pixel); a generation with a pixel on the window's border continues in the per-lane lists
+          // in global scratch below. Both visit the same draws in the same order.
+          const int cx = pix / CRV_NPIX, cy = pix % CRV_NPIX;
+          uint64_t cur = 1ull << 24;  // generation 0: the photon's pixel at the window's centre
+          int gnum = 0;
+          bool window = true;
+          while (!bad) {
+            if (cur & WIN_BORDER) {
+              window = false;  // neighbours could leave the window
+              break;
+            }
+            uint64_t m0 = 0, m1 = 0, m2 = 0;  // candidate multiplicity, bit planes of a 3-bit count per position
+            for (uint64_t rem = cur; rem; rem &= rem - 1) {
+              const int bq = __ffsll((long long)rem) - 1;
+              const int fx = cx + bq / 7 - 3, fy = cy + bq % 7 - 3;
+              const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j,
+                               pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)(fx * CRV_NPIX + fy)));
+              uint64_t add = 0;
+              if (fx > 0 && u32f(ux.x) < qct) add |= 1ull << (bq - 7);
+              if (fx + 1 < p.n_pixels_x && u32f(ux.y) < qct) add |= 1ull << (bq + 7);
+              if (fy > 0 && u32f(ux.z) < qct) add |= 1ull << (bq - 1);
+              if (fy + 1 < p.n_pixels_y && u32f(ux.w) < qct) add |= 1ull << (bq + 1);
+              const uint64_t c0 = m0 & add;
+              m0 ^= add;
+              const uint64_t c1 = m1 & c0;
+              m1 ^= c0;
+              m2 ^= c1;
+            }
+            const uint64_t cand = m0 | m1 | m2;
+            if (!cand) break;
+            if (__popcll(cand) > CMAXB || gnum + 1 > MAXGEN) {
+              atomicAdd(&flags[F_CASCADE], 1);
               bad = true;
               break;
             }
-            log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, c, toff, vc);
-            bset(bm, lane, c);
-            xcur[ncur * xs] = (uint16_t)c;
-            ++ncur;
+            ++gnum;
+            uint64_t nxt = 0;
+            for (uint64_t rem = cand; rem; rem &= rem - 1) {
+              const int bq = __ffsll((long long)rem) - 1;
+              const int c = (cx + bq / 7 - 3) * CRV_NPIX + cy + bq % 7 - 3;
+              const int m = (int)((m0 >> bq) & 1u) + 2 * (int)((m1 >> bq) & 1u) + 4 * (int)((m2 >> bq) & 1u);
+              const bool c_set = bget(bm, lane, c);
+              if (c_set && dead_px(t, c)) continue;
+              float vc = Vov;
+              if (c_set) vc = Vov * (1.f - __expf(-(toff - lastt[c * xs]) * inv_tau));
+              const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
+              float qm = q1;
+              for (int mm = 2; mm <= m; ++mm) qm *= q1;
+              const float uc =
+                  u32f(dr(KIND_XTFIRE, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)c)).x);
+              if (!(uc < 1.f - qm)) continue;
+              if (nlog >= lcap) {
+                bad = true;
+                break;
+              }
+              log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, c, toff, vc);
+              bset(bm, lane, c);
+              nxt |= 1ull << bq;
+            }
+            cur = nxt;
+            if (!cur) break;
           }
+          if (bad) break;
+          if (window) continue;
+          int ncur = 0;
+          for (uint64_t rem = cur; rem; rem &= rem - 1, ++ncur) {
+            const int bq = __ffsll((long long)rem) - 1;
+            xcur[ncur * xs] = (uint16_t)((cx + bq / 7 - 3) * CRV_NPIX + cy + bq % 7 - 3);
+          }
+          while (ncur > 0 && !bad) {
+            int ncand = 0;
+            for (int ai = 0; ai < ncur; ++ai) {
+              const int f = (int)xcur[ai * xs];
+              const int fx = f / CRV_NPIX, fy = f % CRV_NPIX;
+              const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)f));
+              const uint32_t uw[4] = {ux.x, ux.y, ux.z, ux.w};
+  #pragma unroll
+              for (int nb = 0; nb < 4; ++nb) {
+                const int nx = nb == 0 ? fx - 1 : nb == 1 ? fx + 1 : fx;
+                const int ny = nb == 2 ? fy - 1 : nb == 3 ? fy + 1 : fy;
+                const bool ok =
+                    nb == 0 ? fx > 0 : nb == 1 ? fx + 1 < p.n_pixels_x : nb == 2 ? fy > 0 : fy + 1 < p.n_pixels_y;
+                if (!ok || !(u32f(uw[nb]) < qct)) continue;
+                const int c = nx * CRV_NPIX + ny;
+                int at = 0;
+                while (at < ncand && xcand[at * xs] < c) ++at;
+                if (at < ncand && xcand[at * xs] == c) {
+                  xmult[at * xs] += 1;
+                } else if (ncand < CMAXB) {
+                  for (int q = ncand; q > at; --q) {
+                    xcand[q * xs] = xcand[(q - 1) * xs];
+                    xmult[q * xs] = xmult[(q - 1) * xs];
+                  }
+                  xcand[at * xs] = (uint16_t)c;
+                  xmult[at * xs] = 1;
+                  ++ncand;
+                } else {
+                  atomicAdd(&flags[F_CASCADE], 1);
+                  bad = true;
+                }
+              }
+            }
+            if (ncand == 0 || bad) break;
+            if (++gnum > MAXGEN) {
+              atomicAdd(&flags[F_CASCADE], 1);
+              bad = true;
+              break;
+            }
+            ncur = 0;
+            for (int ai = 0; ai < ncand; ++ai) {
+              const int c = xcand[ai * xs];
+              const int m = xmult[ai * xs];
+              const bool c_set = bget(bm, lane, c);
+              if (c_set && dead_px(t, c)) continue;
+              float vc = Vov;
+              if (c_set) vc = Vov * (1.f - __expf(-(toff - lastt[c * xs]) * inv_tau));
+              const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
+              float qm = q1;
+              for (int mm = 2; mm <= m; ++mm) qm *= q1;
+              const float uc =
+                  u32f(dr(KIND_XTFIRE, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)c)).x);
+              if (!(uc < 1.f - qm)) continue;
+              if (nlog >= lcap) {
+                bad = true;
+                break;
+              }
+              log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, c, toff, vc);
+              bset(bm, lane, c);
+              xcur[ncur * xs] = (uint16_t)c;
+              ++ncur;
+            }
+          }
+          if (bad) break;
         }
-        if (bad) break;
+        if (nlog & 1) o.flog[lb + (nlog >> 1) * 64] = make_float4(pend_t, pend_v, 0.f, 0.f);
+        if (bad) {
+          atomicAdd(&flags[F_OVERFLOW], 1);  // the batch is run again with larger logs
+          m = 0;
+        } else if (nlog > 0) {
+          close_cluster((int)nlog);
+        }
+        nfired[g] = (int32_t)nlog;
+      } while (false);
+      // the descriptors of the wavefront's 64 SiPMs, reserved together in lane order (kb_wrec's lanes then read
+      // neighbouring log chunks)
+      int incl = m;  // inclusive prefix over the lanes
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
       }
-      if (nlog & 1) o.flog[lb + (nlog >> 1) * 64] = make_float4(pend_t, pend_v, 0.f, 0.f);
-      if (bad) atomicAdd(&flags[F_OVERFLOW], 1);  // the batch is run again with larger logs
-      nfired[g] = (int32_t)nlog;
+      const int tot = __shfl(incl, 63);
+      int wb = 0;
+      if (lane == 0 && tot) wb = atomicAdd(o.cdesc_count, tot);
+      wb = __shfl(wb, 0);
+      const int32_t b0 = wb + incl - m;
+      if (g >= 0) {
+        if ((int64_t)b0 + m <= o.cdesc_cap) {  // else the pool is full: the call runs again with a larger one
+#pragma unroll
+          for (int k = 0; k < CL_REG; ++k)
+            if (k < m) o.cdesc[b0 + k] = ClDesc{lb, (int32_t)g, qs[k], qe[k], 0};
+          if (m > CL_REG)  // the rest from the log just written
+            scan_clusters(o, lb, (int)nlog, tbase, gap, vmax, vmin, qs, qe, CL_REG, o.cdesc + b0, (int32_t)g);
+        }
+        o.cd0[g] = b0;
+        o.ncls[g] = m;
+      }
     }
   }
 }
 
 
-// ---- waveform machine (kb_wave) ----
+// ---- waveform machine (kb_wrec) ----
 struct Wave {
   float win[RINGB];        // open samples E .. E + RINGB - 1
   U4 nz;                   // noise draw of the sample pair holding E - 1
@@ -736,7 +868,8 @@ struct Lane {
   int64_t evkey;
   double td;  // TDC0 of the SiPM's FEB
   int64_t task;
-  int nrun;   // reco waveforms of the SiPM so far
+  int nrun;   // reco waveforms of the cluster so far
+  int32_t cl; // the cluster's descriptor (kb_wrec)
   int W;
   const float *tmpl;
 };
@@ -890,6 +1023,7 @@ __device__ __forceinline__ void wave_close(const crv_params &p, const Tables &t,
   for (int J = w.E; J < W; ++J) put_adc(o, L, w, J, (uint32_t)pedw);
   o.rec_label[w.rec] = L.evkey * CRV_CHAN_MULT + L.ch;
   o.rec_tick[w.rec] = w.kf;
+  atomicAdd(o.rec_used, 1);
   // reco waveforms: runs of continuing digis (start TDC = window tick + first sample), in start order
   const int64_t tdc_base = (int64_t)llrint((w.t_start - L.td) / PERIODD);
   int pos = 0;
@@ -905,6 +1039,7 @@ __device__ __forceinline__ void wave_close(const crv_params &p, const Tables &t,
       o.run_i0[r] = i0;
       o.run_n[r] = last + Sz - i0;  // <= W: every digi starts below W - Sz
       o.run_k[r] = L.nrun;
+      o.run_c[r] = L.cl;
       o.run_g[r] = L.task;
       o.run_tdc[r] = tdc_base + i0;
     }
@@ -913,72 +1048,89 @@ __device__ __forceinline__ void wave_close(const crv_params &p, const Tables &t,
   w.rec = -1;
 }
 
-// kb_wave, one lane per SiPM: its fired pixels in time order -> charge clusters -> waveform window (template sums,
-// noise, ADC, zero suppression) -> ADC records and reco waveforms of the clusters with digis.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void kb_wave(crv_params p, Tables t, Derived dv, const uint64_t *__restrict__ task_key,
-                                              const int32_t *__restrict__ nfired, Out o, Queue qu,
-                                              int *__restrict__ flags) {
-  __shared__ int tile_base;
-  __shared__ float tmpl_s[TMPL_LDS];  // the single-PE template, staged once per workgroup
-  if (!resolve(qu)) return;
+// ---- the waveform stage: one lane per kept charge cluster (kb_wrec) ----
+
+// the end of a pass's descriptors (the reservations of its kb_sipm): drange[1] of the pass, drange[0] of the next
+__global__ void kb_dmark(const int32_t *__restrict__ cdesc_count, int32_t *__restrict__ dend) { *dend = *cdesc_count; }
+
+// one lane per kept cluster of the pass, descriptors [drange[0], drange[1]) (grid-stride): the waveform window over
+// the cluster's charges -> its ADC record and reco waveforms (run number within the cluster, and its descriptor)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void kb_wrec(
+    crv_params p, Tables t, Derived dv, const uint64_t *__restrict__ task_key, Out o,
+    const int32_t *__restrict__ drange, const int *__restrict__ skip, int *__restrict__ flags) {
+  __shared__ float tmpl_s[TMPL_LDS];
+  if (skip && *skip) return;
+  // a full descriptor pool (some SiPMs reserved descriptors they could not write): the call runs again with a
+  // larger pool, so none of this pass's descriptors is read
+  const int64_t d0 = drange[0], d1 = drange[1];
+  if (d1 > o.cdesc_cap) return;
   const int lane = threadIdx.x;
-  const double gap = p.single_pe_max_time + 4.0 * PERIODD;
   const DrawR dr{(uint32_t)p.seed, p.evkey_hi};
   for (int i = lane; i < t.ntmpl; i += 64) tmpl_s[i] = t.tmpl_f[i];
   __syncthreads();
+  const int64_t lanes = (int64_t)gridDim.x * 64;
   Lane L;
   L.W = p.window_samples;
   L.tmpl = tmpl_s;
-  int64_t base = 0;
-  while (grab(qu, &tile_base, base)) {
-    for (int64_t gq = base + lane; gq < base + qu.tile && gq < qu.n; gq += 64) {
-      const int64_t g = qu.order ? (int64_t)qu.order[gq] : gq;
-      const uint64_t key = task_key[g >> 2];
-      L.ev = (uint32_t)(key >> 13);
-      L.evkey = ((int64_t)p.evkey_hi << 32) | (int64_t)L.ev;
-      L.ch = (int)(key & 8191u) * 4 + (int)(g & 3);
-      L.task = g;
-      L.nrun = 0;
-      const int nf = nfired[g];
-      const int64_t lb = o.goff[gq >> 6] - o.log0 + (gq & 63);
-      const double tb0 = nf > 0 ? o.tbase[g] : 0.0;
-      L.td = 0.0;
-      if (nf > 0) {
-        const uint32_t fk = p.feb_key_roc ? (uint32_t)(t.roc[L.ch] * 1000 + t.feb[L.ch]) : (uint32_t)t.feb[L.ch];
-        L.td = (double)u32f(dr(KIND_TDC0, L.ev, fk, 0u, 0u).x) * p.feb_phase_width;
-      }
-      Wave wv;
-      wv.open = false;
-      wv.overlong = 0;
-      wv.rec = -1;
-      int q = 0;
-      float4 ck = make_float4(0.f, 0.f, 0.f, 0.f);  // the log chunk holding entry q
-      if (nf > 0) ck = o.flog[lb];
-      while (q < nf) {
-        // the charges of one time (a fired pixel and its cross-talk)
-        const float tqf = (q & 1) ? ck.z : ck.x;
-        const double tq = tb0 + (double)tqf;
-        float vsum = (q & 1) ? ck.w : ck.y;
-        int n = 1;
-        for (;;) {
-          const int e = q + n;
-          if (e >= nf) break;
-          if ((e & 1) == 0) ck = o.flog[lb + (int64_t)(e >> 1) * 64];
-          if (((e & 1) ? ck.z : ck.x) != tqf) break;
-          vsum += (e & 1) ? ck.w : ck.y;
-          ++n;
-        }
-        q += n;
-        if (wv.open && tq - wv.t_last > gap) wave_close(p, t, L, o, wv, flags);
-        if (!wv.open) wave_open(L, o, wv, tq);
-        wave_add(p, t, L, o, wv, tq, vsum, n, dv.nreach);
-      }
-      if (wv.open) wave_close(p, t, L, o, wv, flags);
-      if (wv.rec >= 0) o.rec_label[wv.rec] = -1;  // reserved, never used
-      if (wv.overlong) atomicAdd(&flags[F_OVERLONG], 1);
-      o.nrun[g] = L.nrun;
+  for (int64_t d = d0 + (int64_t)blockIdx.x * 64 + lane; d < d1; d += lanes) {
+    const ClDesc cd = o.cdesc[d];
+    if (cd.q1 <= cd.q0) {
+      o.ncr[d] = 0;
+      continue;
     }
+    const int64_t g = cd.g;
+    const uint64_t key = task_key[g >> 2];
+    L.ev = (uint32_t)(key >> 13);
+    L.evkey = ((int64_t)p.evkey_hi << 32) | (int64_t)L.ev;
+    L.ch = (int)(key & 8191u) * 4 + (int)(g & 3);
+    L.task = g;
+    L.nrun = 0;
+    L.cl = (int32_t)d;
+    const uint32_t fk = p.feb_key_roc ? (uint32_t)(t.roc[L.ch] * 1000 + t.feb[L.ch]) : (uint32_t)t.feb[L.ch];
+    L.td = (double)u32f(dr(KIND_TDC0, L.ev, fk, 0u, 0u).x) * p.feb_phase_width;
+    const double tb0 = o.tbase[g];
+    const int64_t lb = cd.lb;
+    Wave wv;
+    wv.open = false;
+    wv.overlong = 0;
+    wv.rec = -1;
+    int q = cd.q0;
+    float4 ck = o.flog[lb + (int64_t)(q >> 1) * 64];
+    while (q < cd.q1) {
+      const float tqf = (q & 1) ? ck.z : ck.x;
+      const double tq = tb0 + (double)tqf;
+      float vsum = (q & 1) ? ck.w : ck.y;
+      int n = 1;
+      for (;;) {
+        const int e = q + n;
+        if (e >= cd.q1) break;
+        if ((e & 1) == 0) ck = o.flog[lb + (int64_t)(e >> 1) * 64];
+        if (((e & 1) ? ck.z : ck.x) != tqf) break;
+        vsum += (e & 1) ? ck.w : ck.y;
+        ++n;
+      }
+      q += n;
+      if (!wv.open) wave_open(L, o, wv, tq);
+      wave_add(p, t, L, o, wv, tq, vsum, n, dv.nreach);
+    }
+    wave_close(p, t, L, o, wv, flags);
+    if (wv.rec >= 0) o.rec_label[wv.rec] = -1;  // reserved, never used (the cluster has no digis)
+    if (wv.overlong) atomicAdd(&flags[F_OVERLONG], 1);
+    o.ncr[d] = L.nrun;
   }
+}
+
+// per SiPM: the offset of each cluster's reco waveforms within the SiPM (clusters in time order), and the SiPM's total
+__global__ void kb_run_order(int64_t L4, const int32_t *__restrict__ Tdev, Out o, const int *__restrict__ skip) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= L4 || (Tdev && g >= 4 * (int64_t)*Tdev) || (skip && *skip)) return;
+  const int32_t b0 = o.ncls[g] ? o.cd0[g] : 0, m = o.ncls[g];
+  int32_t acc = 0;
+  for (int32_t k = 0; k < m; ++k) {
+    o.crl[b0 + k] = acc;
+    acc += o.ncr[b0 + k];
+  }
+  o.nrun[g] = acc;
 }
 
 // kb_fit, one lane per reco waveform of the run pool: pedestal, peaks, Gumbel fits, pulses -> the run's slots; the
@@ -1158,7 +1310,7 @@ __device__ __forceinline__ void fit_run(const crv_params &p, const Tables &t, co
     startb = fe;
   }
   if (np > o.pcap) atomicAdd(&flags[F_PSLOT], 1);
-  const int64_t fr = roff[g] + o.run_k[r];
+  const int64_t fr = roff[g] + o.crl[o.run_c[r]] + o.run_k[r];
   o.npr[fr] = min(np, o.pcap);
   o.rmap[fr] = (int32_t)r;
 }
@@ -1291,10 +1443,10 @@ __global__ void kb_check_log(const int64_t *__restrict__ total, int64_t cap, int
   if (*total > cap && atomicCAS(skip, 0, 1) == 0) skip[1] = 1;
 }
 __global__ void kb_check_wave(const int *__restrict__ flags, const int32_t *__restrict__ counts, int64_t rec_cap,
-                              int64_t run_cap, int *__restrict__ skip) {
-  // counts[0] records, counts[1] runs; a log overflow of kb_sipm shows in F_OVERFLOW
+                              int64_t run_cap, int64_t desc_cap, int *__restrict__ skip) {
+  // counts[0] records, counts[1] runs, counts[2] cluster descriptors; a log overflow of kb_sipm shows in F_OVERFLOW
   if (flags[F_OVERFLOW] && atomicCAS(skip, 0, 1) == 0) skip[1] = 1;
-  if ((counts[0] > rec_cap || counts[1] > run_cap) && atomicCAS(skip, 0, 1) == 0) skip[1] = 2;
+  if ((counts[0] > rec_cap || counts[1] > run_cap || counts[2] > desc_cap) && atomicCAS(skip, 0, 1) == 0) skip[1] = 2;
 }
 __global__ void kb_check_fit(const int *__restrict__ flags, const int64_t *__restrict__ np, int64_t np_cap,
                              int *__restrict__ skip) {
@@ -1323,7 +1475,7 @@ __global__ void kb_sizes(const int64_t *__restrict__ np, const int64_t *__restri
   sz[0] = *skip ? 0 : *np;
   sz[3] = sums[1];
   sz[4] = sums[0];
-  sz[5] = counts[0];
+  sz[5] = counts[4];  // records with digis
   sz[6] = counts[1];
 }
 
@@ -1344,7 +1496,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   if (opt->sort_tasks < 0 || opt->sort_tasks > 3)
     return set_error(c, CRV_E_ARG, "batched chain: sort_tasks must be 0, 1, 2 or 3");
   const int grid_s = opt->grid > 0 ? opt->grid : resident_blocks(c, (const void *)kb_sipm);
-  const int grid_w = opt->grid > 0 ? opt->grid : resident_blocks(c, (const void *)kb_wave);
+  const int grid_r = opt->grid > 0 ? opt->grid : resident_blocks(c, (const void *)kb_wrec);
   hipEvent_t *ev = c->ev, *kev = c->kev;
   CRV_HIP(c, hipEventRecord(ev[0], s));
   StepGroups g;
@@ -1361,7 +1513,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   uint32_t *skey = (uint32_t *)ws(c, 505, L4 * 4 + 8), *skey2 = (uint32_t *)ws(c, 506, L4 * 4 + 8);
   int32_t *sidx = (int32_t *)ws(c, 507, L4 * 4 + 8), *order = (int32_t *)ws(c, 508, L4 * 4 + 8);
   int32_t *nfired = (int32_t *)ws(c, 512, L4 * 4 + 8);
-  int *counters = (int *)ws(c, 540, 64);  // queues of kb_sipm and kb_wave, record / run / digi counts
+  int *counters = (int *)ws(c, 540, 64);  // kb_sipm's queue, then record / run / descriptor / used-record counts
   Out o{};
   o.xstride = (int64_t)grid_s * 64;
   o.xcur = (uint16_t *)ws(c, 543, (size_t)o.xstride * CMAXB * 2);
@@ -1373,11 +1525,16 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   o.gTb = (double *)ws(c, 536, n * 32 + 16);
   o.gRJ = (uint64_t *)ws(c, 537, n * 32 + 16);
   o.nrun = (int32_t *)ws(c, 513, L4 * 4 + 8);
+  o.cd0 = (int32_t *)ws(c, 521, L4 * 4 + 8);
+  o.ncls = (int32_t *)ws(c, 522, L4 * 4 + 8);
+  if (!o.cd0 || !o.ncls) return ws_fail(c);
   if (!Tdev || !nph || !sipm_nph || !gcap || !gsize || !goff || !skey || !skey2 || !sidx || !order || !nfired || !counters ||
       !o.xcur || !o.xcand || !o.xmult || !o.lastt || !o.gS || !o.gHead || !o.gTb || !o.gRJ || !o.nrun)
     return ws_fail(c);
   o.rec_count = counters + 2;
   o.run_count = counters + 3;
+  o.cdesc_count = counters + 4;
+  o.rec_used = counters + 6;
   const int32_t T32 = (int32_t)T;
   CRV_HIP(c, hipMemcpyAsync(Tdev, &T32, 4, hipMemcpyHostToDevice, s));
   kb_zero<<<1, 64, 0, s>>>(c->flags, F_NFLAGS);
@@ -1397,7 +1554,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   }
   CRV_HIP(c, hipEventRecord(ev[1], s));
   int hf[F_NFLAGS];
-  int32_t cnt[2] = {0, 0};
+  int32_t cnt[3] = {0, 0, 0};  // records, runs, cluster descriptors
   float ms_sipm = 0.f, ms_wave = 0.f;  // summed over the passes of the last attempt
   const int32_t *qorder = sorted ? order : nullptr;
   auto kernel_ms = [&](int k) {
@@ -1406,7 +1563,9 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     return e;
   };
   // kb_sipm over queue positions [t0, t1): 0, 1 when a fired-pixel log overflowed, or an error code
-  auto run_sipm = [&](int64_t t0, int64_t t1) -> int {
+  // (and the end of its descriptors in dpass[pi + 1])
+  int32_t *dpass = nullptr;  // per pass its first descriptor, then the end of the last pass's
+  auto run_sipm = [&](int64_t t0, int64_t t1, int pi) -> int {
     kb_zero<<<1, 64, 0, s>>>(counters, 1);
     kb_zero<<<1, 64, 0, s>>>(c->flags + F_OVERFLOW, 1);
     CRV_HIP(c, hipEventRecord(kev[0], s));
@@ -1414,6 +1573,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
                                   Queue{counters, tile, t1, qorder, t0}, nfired, c->flags);
     CRV_HIP(c, hipGetLastError());
     CRV_HIP(c, hipEventRecord(kev[1], s));
+    kb_dmark<<<1, 1, 0, s>>>(o.cdesc_count, dpass + pi + 1);
     CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
     CRV_HIP(c, hipStreamSynchronize(s));
     ms_sipm += kernel_ms(0);
@@ -1423,11 +1583,9 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     if (hf[F_COUNTER]) return set_error(c, CRV_E_OVERFLOW, "batched chain: more than 65535 photoelectrons from one step");
     return hf[F_OVERFLOW] ? 1 : 0;
   };
-  auto run_wave = [&](int64_t t0, int64_t t1) -> int {
-    kb_zero<<<1, 64, 0, s>>>(counters + 1, 1);
+  auto run_wave = [&](int pi) -> int {
     CRV_HIP(c, hipEventRecord(kev[2], s));
-    kb_wave<<<grid_w, 64, 0, s>>>(p, c->t, c->d, g.task_key, nfired, o, Queue{counters + 1, tile, t1, qorder, t0},
-                                  c->flags);
+    kb_wrec<<<grid_r, 64, 0, s>>>(p, c->t, c->d, g.task_key, o, dpass + pi, nullptr, c->flags);
     CRV_HIP(c, hipGetLastError());
     CRV_HIP(c, hipEventRecord(kev[3], s));
     CRV_HIP(c, hipStreamSynchronize(s));
@@ -1447,14 +1605,19 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     o.run_k = (int32_t *)ws(c, 576, o.run_cap * 4);
     o.run_g = (int64_t *)ws(c, 577, o.run_cap * 8);
     o.run_tdc = (int64_t *)ws(c, 578, o.run_cap * 8);
+    o.run_c = (int32_t *)ws(c, 523, o.run_cap * 4);
+    o.cdesc_cap = std::max<int64_t>(c->b_desc_cap, L4 + 1024);
+    o.cdesc = (ClDesc *)ws(c, 524, o.cdesc_cap * sizeof(ClDesc));
+    o.ncr = (int32_t *)ws(c, 525, o.cdesc_cap * 4);
+    o.crl = (int32_t *)ws(c, 526, o.cdesc_cap * 4);
     if (!o.rec_adc || !o.rec_label || !o.rec_tick || !o.run_rec || !o.run_i0 || !o.run_n || !o.run_k || !o.run_g ||
-        !o.run_tdc)
+        !o.run_tdc || !o.run_c || !o.cdesc || !o.ncr || !o.crl)
       return ws_fail(c);
     return 0;
   };
   // Fired-pixel log budget (bytes): the caller's, else half of the device memory free now plus what
   // the log already holds. When the whole call's log exceeds it, the queue is cut into passes of whole groups whose
-  // logs fit, and each pass runs kb_sipm then kb_wave on its SiPMs (records and runs accumulate over the passes).
+  // logs fit, and each pass runs kb_sipm then kb_wrec on its SiPMs (records and runs accumulate).
   size_t budget = opt->log_budget > 0 ? (size_t)opt->log_budget : 0;
   if (!budget) {
     size_t fr = 0, total = 0;
@@ -1466,7 +1629,8 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   int64_t nlog = 0;
   int npass = 1;
   // phases 1 and 2: photons -> fired-pixel logs -> records and reco waveforms. A log overflow grows the logs and runs
-  // everything again; a pool overflow grows the pools and runs kb_wave again (and kb_sipm, when passes reuse the log).
+  // everything again; a pool overflow grows the pools and runs the waveform kernel again (and kb_sipm, when passes
+  // reuse the log or the descriptor pool was full).
   for (int attempt = 0, factor = c->b_log_factor;; ++attempt, factor *= 2) {
     if (attempt > 8) return set_error(c, CRV_E_OVERFLOW, "batched chain: fired-pixel log still too short");
     if (L4)
@@ -1510,14 +1674,15 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     o.gcap = gcap;
     o.flog = (float4 *)ws(c, 538, log_chunks * 16 + 16);
     o.tbase = (double *)ws(c, 519, L4 * 8 + 8);
-    if (!o.flog || !o.tbase) return ws_fail(c);
+    dpass = (int32_t *)ws(c, 527, (size_t)(npass + 2) * 4);
+    if (!o.flog || !o.tbase || !dpass) return ws_fail(c);
     float4 *flog2 = nullptr;
     if (pipe && npass > 1) {
       flog2 = (float4 *)ws(c, 547, log_chunks * 16 + 16);
       if (!flog2) return ws_fail(c);
     }
     bool grow_log = false;
-    if (flog2) {  // pipelined passes: kb_sipm of pass i + 1 on `s` beside kb_wave of pass i on a side stream
+    if (flog2) {  // pipelined passes: kb_sipm of pass i + 1 on `s` beside the waveform kernels of pass i (side stream)
       const hipStream_t s2 = c->aux[0];
       while ((int)c->pev.size() < 6 * npass + 2) {
         hipEvent_t e;
@@ -1528,7 +1693,8 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
                  *t0w = S + 4 * npass, *t1w = S + 5 * npass, fk = S[6 * npass], jn = S[6 * npass + 1];
       for (int pa = 0;; ++pa) {
         if ((rc = pools())) return rc;
-        kb_zero<<<1, 64, 0, s>>>(counters + 2, 2);
+        kb_zero<<<1, 64, 0, s>>>(counters + 2, 5);
+        kb_zero<<<1, 64, 0, s>>>(dpass, 1);
         kb_zero<<<1, 64, 0, s>>>(c->flags + F_OVERFLOW, 1);
         CRV_HIP(c, hipEventRecord(fk, s));
         CRV_HIP(c, hipStreamWaitEvent(s2, fk, 0));
@@ -1544,12 +1710,11 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
                                         Queue{counters, tile, q1, qorder, q0}, nfired, c->flags);
           CRV_HIP(c, hipGetLastError());
           CRV_HIP(c, hipEventRecord(t1s[pi], s));
+          kb_dmark<<<1, 1, 0, s>>>(o.cdesc_count, dpass + pi + 1);
           CRV_HIP(c, hipEventRecord(S[pi], s));
           CRV_HIP(c, hipStreamWaitEvent(s2, S[pi], 0));
-          kb_zero<<<1, 64, 0, s2>>>(counters + 1, 1);
           CRV_HIP(c, hipEventRecord(t0w[pi], s2));
-          kb_wave<<<grid_w, 64, 0, s2>>>(p, c->t, c->d, g.task_key, nfired, op, Queue{counters + 1, tile, q1, qorder, q0},
-                                         c->flags);
+          kb_wrec<<<grid_r, 64, 0, s2>>>(p, c->t, c->d, g.task_key, op, dpass + pi, nullptr, c->flags);
           CRV_HIP(c, hipGetLastError());
           CRV_HIP(c, hipEventRecord(t1w[pi], s2));
           CRV_HIP(c, hipEventRecord(Wv[pi], s2));
@@ -1557,7 +1722,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
         CRV_HIP(c, hipEventRecord(jn, s2));
         CRV_HIP(c, hipStreamWaitEvent(s, jn, 0));
         CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
-        CRV_HIP(c, hipMemcpyAsync(cnt, counters + 2, 8, hipMemcpyDeviceToHost, s));
+        CRV_HIP(c, hipMemcpyAsync(cnt, counters + 2, 12, hipMemcpyDeviceToHost, s));
         CRV_HIP(c, hipStreamSynchronize(s));
         ms_sipm = ms_wave = 0.f;
         for (int pi = 0; pi < npass; ++pi) {
@@ -1576,10 +1741,11 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
           grow_log = true;
           break;
         }
-        if (cnt[0] <= o.rec_cap && cnt[1] <= o.run_cap) break;
+        if (cnt[0] <= o.rec_cap && cnt[1] <= o.run_cap && cnt[2] <= o.cdesc_cap) break;
         if (pa >= 3) return set_error(c, CRV_E_OVERFLOW, "batched chain: record or run pool still too small");
         c->b_rec_cap = std::max<int64_t>(o.rec_cap, (int64_t)cnt[0] + cnt[0] / 4 + 1024);
         c->b_run_cap = std::max<int64_t>(o.run_cap, (int64_t)cnt[1] + cnt[1] / 4 + 1024);
+        c->b_desc_cap = std::max<int64_t>(o.cdesc_cap, (int64_t)cnt[2] + cnt[2] / 4 + 1024);
       }
       if (!grow_log) {
         c->b_log_factor = factor;
@@ -1587,30 +1753,40 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
       }
       continue;
     }
-    for (int pa = 0;; ++pa) {
+    for (int pa = 0, desc_full = 0;; ++pa) {
       if ((rc = pools())) return rc;
+      // kb_sipm runs again when the passes reuse the log or its descriptors did not fit; otherwise only the
+      // waveform kernel runs again, over the descriptors of the last run
+      const bool sipm = pa == 0 || npass > 1 || desc_full;
       kb_zero<<<1, 64, 0, s>>>(counters + 2, 2);
+      kb_zero<<<1, 64, 0, s>>>(counters + 6, 1);
+      if (sipm) {
+        kb_zero<<<1, 64, 0, s>>>(counters + 4, 1);
+        kb_zero<<<1, 64, 0, s>>>(dpass, 1);
+        ms_sipm = 0.f;
+      }
       ms_wave = 0.f;
-      if (pa == 0 || npass > 1) ms_sipm = 0.f;
       for (int pi = 0; pi < npass && L4; ++pi) {
         const int64_t t0 = cuts[pi] * 64, t1 = std::min<int64_t>(cuts[pi + 1] * 64, L4);
         o.log0 = npass > 1 ? hoff[cuts[pi]] : 0;
-        if (pa == 0 || npass > 1) {
-          if ((rc = run_sipm(t0, t1)) < 0) return rc;
+        if (sipm) {
+          if ((rc = run_sipm(t0, t1, pi)) < 0) return rc;
           if (rc) {
             grow_log = true;
             break;
           }
         }
-        if ((rc = run_wave(t0, t1))) return rc;
+        if ((rc = run_wave(pi))) return rc;
       }
       if (grow_log) break;
-      CRV_HIP(c, hipMemcpyAsync(cnt, counters + 2, 8, hipMemcpyDeviceToHost, s));
+      CRV_HIP(c, hipMemcpyAsync(cnt, counters + 2, 12, hipMemcpyDeviceToHost, s));
       CRV_HIP(c, hipStreamSynchronize(s));
-      if (cnt[0] <= o.rec_cap && cnt[1] <= o.run_cap) break;
+      if (cnt[0] <= o.rec_cap && cnt[1] <= o.run_cap && cnt[2] <= o.cdesc_cap) break;
       if (pa >= 3) return set_error(c, CRV_E_OVERFLOW, "batched chain: record or run pool still too small");
+      desc_full = cnt[2] > o.cdesc_cap;
       c->b_rec_cap = std::max<int64_t>(o.rec_cap, (int64_t)cnt[0] + cnt[0] / 4 + 1024);
       c->b_run_cap = std::max<int64_t>(o.run_cap, (int64_t)cnt[1] + cnt[1] / 4 + 1024);
+      c->b_desc_cap = std::max<int64_t>(o.cdesc_cap, (int64_t)cnt[2] + cnt[2] / 4 + 1024);
     }
     if (!grow_log) {
       c->b_log_factor = factor;
@@ -1636,9 +1812,13 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     CRV_HIP(c, hipMemcpyAsync(hsum, sums, 16, hipMemcpyDeviceToHost, s));
   }
   const int64_t nrec = cnt[0], nruns = cnt[1];
+  c->b_last_desc = cnt[2];
+  int32_t nused = 0;
+  CRV_HIP(c, hipMemcpyAsync(&nused, o.rec_used, 4, hipMemcpyDeviceToHost, s));
   // the runs' places in the output: each SiPM's runs in run-number order
   int64_t *nrun64 = (int64_t *)ws(c, 579, (L4 + 1) * 8), *roff = (int64_t *)ws(c, 580, (L4 + 2) * 8);
   if (!nrun64 || !roff) return ws_fail(c);
+  if (L4) kb_run_order<<<nblocks(L4, 256), 256, 0, s>>>(L4, nullptr, o, nullptr);
   if (L4) kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(o.nrun, nrun64, L4);
   if ((rc = scan_i64(c, nrun64, roff, L4, 514, s))) return rc;
   // phase 3: pulse fits (slots grown and the fits run again when a waveform has more pulses)
@@ -1725,14 +1905,14 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
     timing->clusters_ms = ms[3];
     timing->total_ms = tot;
     timing->k_sipm_ms = km[0];    // kb_sipm (the passes of the last run)
-    timing->k_digi_ms = km[1];    // kb_wave (the passes of the last run)
+    timing->k_digi_ms = km[1];    // kb_wrec (the passes of the last run)
     timing->k_pulse_ms = km[2];   // kb_fit (last run)
     timing->k_coinc_ms = kc;
     timing->n_steps = n;
     timing->n_hit_counters = T;
     timing->n_photons = hsum[1];
     timing->n_charges = hsum[0];  // fired pixels (the charges, not materialised as a table)
-    timing->n_records = nrec;
+    timing->n_records = nused;    // records with digis (the pool's extent is opts->n_records)
     timing->n_waveforms = nruns;
     timing->n_pulses = np;
     timing->n_clusters = C.n;
@@ -1755,7 +1935,7 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
     return set_error(c, CRV_E_ARG, "batched chain: single-PE template too long");
   if (pl.tile % 64 || pl.tile <= 0) return set_error(c, CRV_E_ARG, "batched chain: tile must be a multiple of 64");
   if (pl.grid_s <= 0) pl.grid_s = resident_blocks(c, (const void *)kb_sipm);
-  if (pl.grid_w <= 0) pl.grid_w = resident_blocks(c, (const void *)kb_wave);
+  if (pl.grid_r <= 0) pl.grid_r = resident_blocks(c, (const void *)kb_wrec);
   StepGroups g;
   int rc = group_steps_fixed(c, pl, ar, g, s);
   if (rc) return rc;
@@ -1793,6 +1973,13 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
   o.run_k = (int32_t *)A(o.run_cap * 4);
   o.run_g = (int64_t *)A(o.run_cap * 8);
   o.run_tdc = (int64_t *)A(o.run_cap * 8);
+  o.run_c = (int32_t *)A(o.run_cap * 4);
+  o.cd0 = (int32_t *)A(L4 * 4 + 8);
+  o.ncls = (int32_t *)A(L4 * 4 + 8);
+  o.cdesc_cap = pl.desc_cap;
+  o.cdesc = (ClDesc *)A(o.cdesc_cap * sizeof(ClDesc));
+  o.ncr = (int32_t *)A(o.cdesc_cap * 4);
+  o.crl = (int32_t *)A(o.cdesc_cap * 4);
   int64_t *sums = (int64_t *)A(16), *fired64 = (int64_t *)A(L4 * 8 + 8);
   int64_t *nrun64 = (int64_t *)A((L4 + 1) * 8), *roff = (int64_t *)A((L4 + 2) * 8);
   const int fit_blocks = (int)std::min<int64_t>(nblocks(o.run_cap, 64), 8192);
@@ -1806,11 +1993,14 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
   void *ptrs[] = {nph, sipm_nph, gcap, gsize, goff, skey, skey2, sidx, order, nfired, counters, o.xcur, o.xcand,
                   o.xmult, o.lastt, o.gS, o.gHead, o.gTb, o.gRJ, o.nrun, o.flog, o.tbase, o.rec_adc, o.rec_label,
                   o.rec_tick, o.run_rec, o.run_i0, o.run_n, o.run_k, o.run_g, o.run_tdc, sums, fired64, nrun64, roff,
-                  o.yscr, o.npr, o.rmap, o.slot, npr64, poff, pbar, psipm};
+                  o.yscr, o.npr, o.rmap, o.slot, npr64, poff, pbar, psipm, o.run_c, o.cd0, o.ncls, o.cdesc, o.ncr,
+                  o.crl};
   for (void *q : ptrs)
     if (!q) return set_error(c, CRV_E_HIP, "graph: buffer allocation failed");
   o.rec_count = counters + 2;
   o.run_count = counters + 3;
+  o.cdesc_count = counters + 4;
+  o.rec_used = counters + 6;
   o.goff = goff;
   o.gcap = gcap;
   o.log0 = 0;
@@ -1843,12 +2033,12 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
                                    Queue{counters, pl.tile, L4, qorder, 0, pl.T_dev, pl.skip}, nfired, c->flags);
   CRV_HIP(c, hipGetLastError());
   CRV_HIP(c, mark(1));
+  kb_dmark<<<1, 1, 0, s>>>(o.cdesc_count, counters + 9);  // descriptors [counters[8] = 0, counters[9])
   CRV_HIP(c, mark(2));
-  kb_wave<<<pl.grid_w, 64, 0, s>>>(p, c->t, c->d, g.task_key, nfired, o,
-                                   Queue{counters + 1, pl.tile, L4, qorder, 0, pl.T_dev, pl.skip}, c->flags);
+  kb_wrec<<<pl.grid_r, 64, 0, s>>>(p, c->t, c->d, g.task_key, o, counters + 8, pl.skip, c->flags);
   CRV_HIP(c, hipGetLastError());
   CRV_HIP(c, mark(3));
-  kb_check_wave<<<1, 1, 0, s>>>(c->flags, counters + 2, o.rec_cap, o.run_cap, pl.skip);
+  kb_check_wave<<<1, 1, 0, s>>>(c->flags, counters + 2, o.rec_cap, o.run_cap, o.cdesc_cap, pl.skip);
   // photons drawn and pixels fired
   kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(nfired, fired64, L4);
   {
@@ -1859,6 +2049,7 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
     CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, fired64, sums, (int)L4, s));
     CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, sipm_nph, sums + 1, (int)L4, s));
   }
+  kb_run_order<<<nblocks(L4, 256), 256, 0, s>>>(L4, pl.T_dev, o, pl.skip);
   kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(o.nrun, nrun64, L4);
   if ((rc = scan_fixed(c, ar, nrun64, roff, L4, s))) return rc;
   kb_zero32<<<nblocks(o.run_cap + 1, 256), 256, 0, s>>>(o.npr, o.run_cap + 1);

@@ -291,3 +291,31 @@ def test_trim_returns_the_workspace_to_the_caller():
         for k, v in r0[tab].items():
             np.testing.assert_array_equal(r[tab][k], v, err_msg=f"{tab} {k}")
     assert CH.trim() > 0 and ctx.workspace_bytes() == 0
+
+
+def _twice_hit(n, seed):
+    """Cosmic events whose counters are each hit again 700 ns later: two kept charge clusters per SiPM."""
+    S = SY.cosmic_events(n, seed=seed)
+    S2 = {k: np.concatenate([v, v]) for k, v in S.items()}
+    S2["t"][len(S["t"]):] += 700.0
+    o = np.argsort(S2["event"], kind="stable")
+    return {k: v[o] for k, v in S2.items()}, len(S["t"])
+
+
+def test_pool_growth_does_not_change_results():
+    """Two kept clusters per SiPM: the first call on a fresh context outgrows the descriptor, record and run pools it
+    starts with (sized for one cluster per SiPM), grows them and runs the photon and waveform kernels again. The
+    tables equal a second call's, whose pools are large enough from the start, a pipelined fresh context's, and the
+    FP64 chain's within the tolerances above."""
+    c64, c32 = _cfgs()
+    cond, G = DT.nominal_conditions(), DT.sector_geom()
+    S, nsteps = _twice_hit(600, seed=31)
+    r1 = CH.run_chain(S, cond, c32, key=3, geom=G, records=True)
+    assert len(r1["records"]["label"]) > 4 * nsteps + 1024  # more than the starting pool (SiPMs + 1024): it grew
+    r2 = CH.run_chain(S, cond, c32, key=3, geom=G, records=True)
+    r3 = CH.run_chain(S, cond, dataclasses.replace(c32), key=3, geom=G, batch={"pipeline": 2}, records=True)
+    for r in (r2, r3):
+        for tab in ("pulses", "clusters", "records"):
+            for k, v in r1[tab].items():
+                np.testing.assert_array_equal(r[tab][k], v, err_msg=f"{tab} {k}")
+    _compare(r1, CH.run_chain(S, cond, c64, key=3, geom=G))
