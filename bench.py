@@ -39,7 +39,7 @@ PULSE_IN_B = 40        # event, bar, sipm, time, pe, seq read by the finder
 # FP32 batched chain kernels
 SIPM_STEP_B = 20       # per (step, SiPM) read by kb_sipm: photon count i32, z f64, t f64
 FIRE_B = 8             # fired-pixel log entry written by kb_sipm: time offset f32, overvoltage f32
-FIRE_IN_B = 8          # log entry read by the waveform kernels (kb_wcl + kb_wrec): time offset, overvoltage
+FIRE_IN_B = 8          # log entry read by the waveform kernel (kb_wrec): time offset, overvoltage
 RUN_B = 32             # reco-waveform descriptor: record, first sample, samples, run number i32; SiPM, start TDC i64
 SLOT_B = 48            # pulse slot written by kb_fit: time, location f64; 7 x f32; flags u32
 
@@ -272,7 +272,7 @@ def kernels_fp32(avg, cnt, fires):
                               cnt["n_clusters"], cnt["n_members"])
     return {
         "kb_sipm": (avg["k_sipm_ms"], SIPM_STEP_B * S_ + FIRE_B * fires),
-        "kb_wcl+kb_wrec": (avg["k_digi_ms"], FIRE_IN_B * fires + RECORD_B * W_ + RUN_B * R_),
+        "kb_wrec": (avg["k_digi_ms"], FIRE_IN_B * fires + RECORD_B * W_ + RUN_B * R_),
         "kb_fit": (avg["k_pulse_ms"], RUN_B * R_ + SAMPLE_B * 24 * R_ + SLOT_B * P_),
         "k_coinc": (avg["k_coinc_ms"], PULSE_IN_B * P_ + CLUSTER_B * C_ + MEMBER_B * M_),
     }

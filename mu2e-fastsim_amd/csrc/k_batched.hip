@@ -38,6 +38,11 @@ constexpr double PERIODD = 12.5;
 constexpr int F_PSLOT = 7;  // a reco waveform produced more pulses than its slots
 constexpr int TMPL_LDS = 1024;  // template samples staged in LDS by the waveform kernel
 constexpr int FWIN = 16;        // fit-window samples staged in LDS per lane by the fit kernel
+// the call's counters (work queue, record / run / descriptor / used-record counts) sit one 128-byte line apart:
+// atomics on counters that share a line serialise in one L2 channel
+constexpr int CL = 32;
+constexpr int C_QUEUE = 0, C_REC = 2 * CL, C_RUN = 3 * CL, C_DESC = 4 * CL, C_USED = 5 * CL, C_DRANGE = 6 * CL;
+constexpr int C_INTS = 7 * CL;
 
 __device__ __forceinline__ float u32f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * 0x1p-24f; }
 
@@ -311,7 +316,8 @@ struct Out {
   int32_t *rec_count;
   int32_t *rec_used;  // records that hold a waveform with digis (the others stay reserved with label -1)
   int64_t rec_cap;
-  // run pool: the reco waveforms (record, first sample, samples, SiPM, run number in the SiPM, start TDC)
+  // run pool: the reco waveforms (record, first sample, samples, run number (in its cluster, then in the SiPM after
+  // kb_run_rank), SiPM, start TDC)
   int32_t *run_rec, *run_i0, *run_n, *run_k;
   int64_t *run_g, *run_tdc;
   int32_t *run_count;
@@ -870,6 +876,7 @@ struct Lane {
   int64_t task;
   int nrun;   // reco waveforms of the cluster so far
   int32_t cl; // the cluster's descriptor (kb_wrec)
+  int32_t used;  // records with digis the lane wrote (added to rec_used once, when the lane is done)
   int W;
   const float *tmpl;
 };
@@ -1023,7 +1030,7 @@ __device__ __forceinline__ void wave_close(const crv_params &p, const Tables &t,
   for (int J = w.E; J < W; ++J) put_adc(o, L, w, J, (uint32_t)pedw);
   o.rec_label[w.rec] = L.evkey * CRV_CHAN_MULT + L.ch;
   o.rec_tick[w.rec] = w.kf;
-  atomicAdd(o.rec_used, 1);
+  ++L.used;
   // reco waveforms: runs of continuing digis (start TDC = window tick + first sample), in start order
   const int64_t tdc_base = (int64_t)llrint((w.t_start - L.td) / PERIODD);
   int pos = 0;
@@ -1072,6 +1079,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void kb
   Lane L;
   L.W = p.window_samples;
   L.tmpl = tmpl_s;
+  L.used = 0;
   for (int64_t d = d0 + (int64_t)blockIdx.x * 64 + lane; d < d1; d += lanes) {
     const ClDesc cd = o.cdesc[d];
     if (cd.q1 <= cd.q0) {
@@ -1118,6 +1126,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void kb
     if (wv.overlong) atomicAdd(&flags[F_OVERLONG], 1);
     o.ncr[d] = L.nrun;
   }
+  if (L.used) atomicAdd(o.rec_used, L.used);
 }
 
 // per SiPM: the offset of each cluster's reco waveforms within the SiPM (clusters in time order), and the SiPM's total
@@ -1131,6 +1140,14 @@ __global__ void kb_run_order(int64_t L4, const int32_t *__restrict__ Tdev, Out o
     acc += o.ncr[b0 + k];
   }
   o.nrun[g] = acc;
+}
+
+// per pool run: its run number within the cluster becomes its run number within the SiPM (the cluster's offset
+// added), which places it in the output
+__global__ void kb_run_rank(Out o, int64_t nruns, const int32_t *__restrict__ ndev, const int *__restrict__ skip) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if ((skip && *skip) || r >= (ndev ? min(nruns, (int64_t)*ndev) : nruns)) return;
+  o.run_k[r] += o.crl[o.run_c[r]];
 }
 
 // kb_fit, one lane per reco waveform of the run pool: pedestal, peaks, Gumbel fits, pulses -> the run's slots; the
@@ -1310,7 +1327,7 @@ __device__ __forceinline__ void fit_run(const crv_params &p, const Tables &t, co
     startb = fe;
   }
   if (np > o.pcap) atomicAdd(&flags[F_PSLOT], 1);
-  const int64_t fr = roff[g] + o.crl[o.run_c[r]] + o.run_k[r];
+  const int64_t fr = roff[g] + o.run_k[r];
   o.npr[fr] = min(np, o.pcap);
   o.rmap[fr] = (int32_t)r;
 }
@@ -1358,8 +1375,7 @@ __global__ void kb_gcaps(const int64_t *__restrict__ nph, const int32_t *__restr
 }
 
 __global__ void kb_zero(int *__restrict__ p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 __global__ void kb_widen(const int32_t *__restrict__ in, int64_t *__restrict__ out, int64_t n) {
@@ -1442,11 +1458,13 @@ __global__ void kb_pad_sipms(int64_t L4cap, const int32_t *__restrict__ Tdev, ui
 __global__ void kb_check_log(const int64_t *__restrict__ total, int64_t cap, int *__restrict__ skip) {
   if (*total > cap && atomicCAS(skip, 0, 1) == 0) skip[1] = 1;
 }
-__global__ void kb_check_wave(const int *__restrict__ flags, const int32_t *__restrict__ counts, int64_t rec_cap,
+__global__ void kb_check_wave(const int *__restrict__ flags, const int32_t *__restrict__ counters, int64_t rec_cap,
                               int64_t run_cap, int64_t desc_cap, int *__restrict__ skip) {
-  // counts[0] records, counts[1] runs, counts[2] cluster descriptors; a log overflow of kb_sipm shows in F_OVERFLOW
+  // a log overflow of kb_sipm shows in F_OVERFLOW
   if (flags[F_OVERFLOW] && atomicCAS(skip, 0, 1) == 0) skip[1] = 1;
-  if ((counts[0] > rec_cap || counts[1] > run_cap || counts[2] > desc_cap) && atomicCAS(skip, 0, 1) == 0) skip[1] = 2;
+  if ((counters[C_REC] > rec_cap || counters[C_RUN] > run_cap || counters[C_DESC] > desc_cap) &&
+      atomicCAS(skip, 0, 1) == 0)
+    skip[1] = 2;
 }
 __global__ void kb_check_fit(const int *__restrict__ flags, const int64_t *__restrict__ np, int64_t np_cap,
                              int *__restrict__ skip) {
@@ -1471,12 +1489,12 @@ __global__ void kb_pad_pulses(int64_t np_cap, const int64_t *__restrict__ np, co
 // the call's sizes for the host: pulses, photons, fired pixels, records, runs (clusters and members are written by
 // the finder's part)
 __global__ void kb_sizes(const int64_t *__restrict__ np, const int64_t *__restrict__ sums,
-                         const int32_t *__restrict__ counts, const int *__restrict__ skip, int64_t *__restrict__ sz) {
+                         const int32_t *__restrict__ counters, const int *__restrict__ skip, int64_t *__restrict__ sz) {
   sz[0] = *skip ? 0 : *np;
   sz[3] = sums[1];
   sz[4] = sums[0];
-  sz[5] = counts[4];  // records with digis
-  sz[6] = counts[1];
+  sz[5] = counters[C_USED];  // records with digis
+  sz[6] = counters[C_RUN];
 }
 
 // ---- host side ----
@@ -1513,7 +1531,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   uint32_t *skey = (uint32_t *)ws(c, 505, L4 * 4 + 8), *skey2 = (uint32_t *)ws(c, 506, L4 * 4 + 8);
   int32_t *sidx = (int32_t *)ws(c, 507, L4 * 4 + 8), *order = (int32_t *)ws(c, 508, L4 * 4 + 8);
   int32_t *nfired = (int32_t *)ws(c, 512, L4 * 4 + 8);
-  int *counters = (int *)ws(c, 540, 64);  // kb_sipm's queue, then record / run / descriptor / used-record counts
+  int *counters = (int *)ws(c, 540, C_INTS * 4);
   Out o{};
   o.xstride = (int64_t)grid_s * 64;
   o.xcur = (uint16_t *)ws(c, 543, (size_t)o.xstride * CMAXB * 2);
@@ -1531,10 +1549,10 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   if (!Tdev || !nph || !sipm_nph || !gcap || !gsize || !goff || !skey || !skey2 || !sidx || !order || !nfired || !counters ||
       !o.xcur || !o.xcand || !o.xmult || !o.lastt || !o.gS || !o.gHead || !o.gTb || !o.gRJ || !o.nrun)
     return ws_fail(c);
-  o.rec_count = counters + 2;
-  o.run_count = counters + 3;
-  o.cdesc_count = counters + 4;
-  o.rec_used = counters + 6;
+  o.rec_count = counters + C_REC;
+  o.run_count = counters + C_RUN;
+  o.cdesc_count = counters + C_DESC;
+  o.rec_used = counters + C_USED;
   const int32_t T32 = (int32_t)T;
   CRV_HIP(c, hipMemcpyAsync(Tdev, &T32, 4, hipMemcpyHostToDevice, s));
   kb_zero<<<1, 64, 0, s>>>(c->flags, F_NFLAGS);
@@ -1555,6 +1573,12 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   CRV_HIP(c, hipEventRecord(ev[1], s));
   int hf[F_NFLAGS];
   int32_t cnt[3] = {0, 0, 0};  // records, runs, cluster descriptors
+  auto read_counts = [&]() -> hipError_t {
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 3 && e == hipSuccess; ++k)
+      e = hipMemcpyAsync(cnt + k, counters + C_REC + k * CL, 4, hipMemcpyDeviceToHost, s);
+    return e;
+  };
   float ms_sipm = 0.f, ms_wave = 0.f;  // summed over the passes of the last attempt
   const int32_t *qorder = sorted ? order : nullptr;
   auto kernel_ms = [&](int k) {
@@ -1693,7 +1717,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
                  *t0w = S + 4 * npass, *t1w = S + 5 * npass, fk = S[6 * npass], jn = S[6 * npass + 1];
       for (int pa = 0;; ++pa) {
         if ((rc = pools())) return rc;
-        kb_zero<<<1, 64, 0, s>>>(counters + 2, 5);
+        kb_zero<<<1, 64, 0, s>>>(counters + C_REC, C_USED + 1 - C_REC);
         kb_zero<<<1, 64, 0, s>>>(dpass, 1);
         kb_zero<<<1, 64, 0, s>>>(c->flags + F_OVERFLOW, 1);
         CRV_HIP(c, hipEventRecord(fk, s));
@@ -1722,7 +1746,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
         CRV_HIP(c, hipEventRecord(jn, s2));
         CRV_HIP(c, hipStreamWaitEvent(s, jn, 0));
         CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
-        CRV_HIP(c, hipMemcpyAsync(cnt, counters + 2, 12, hipMemcpyDeviceToHost, s));
+        CRV_HIP(c, read_counts());
         CRV_HIP(c, hipStreamSynchronize(s));
         ms_sipm = ms_wave = 0.f;
         for (int pi = 0; pi < npass; ++pi) {
@@ -1758,10 +1782,11 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
       // kb_sipm runs again when the passes reuse the log or its descriptors did not fit; otherwise only the
       // waveform kernel runs again, over the descriptors of the last run
       const bool sipm = pa == 0 || npass > 1 || desc_full;
-      kb_zero<<<1, 64, 0, s>>>(counters + 2, 2);
-      kb_zero<<<1, 64, 0, s>>>(counters + 6, 1);
+      kb_zero<<<1, 64, 0, s>>>(counters + C_REC, 1);
+      kb_zero<<<1, 64, 0, s>>>(counters + C_RUN, 1);
+      kb_zero<<<1, 64, 0, s>>>(counters + C_USED, 1);
       if (sipm) {
-        kb_zero<<<1, 64, 0, s>>>(counters + 4, 1);
+        kb_zero<<<1, 64, 0, s>>>(counters + C_DESC, 1);
         kb_zero<<<1, 64, 0, s>>>(dpass, 1);
         ms_sipm = 0.f;
       }
@@ -1779,7 +1804,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
         if ((rc = run_wave(pi))) return rc;
       }
       if (grow_log) break;
-      CRV_HIP(c, hipMemcpyAsync(cnt, counters + 2, 12, hipMemcpyDeviceToHost, s));
+      CRV_HIP(c, read_counts());
       CRV_HIP(c, hipStreamSynchronize(s));
       if (cnt[0] <= o.rec_cap && cnt[1] <= o.run_cap && cnt[2] <= o.cdesc_cap) break;
       if (pa >= 3) return set_error(c, CRV_E_OVERFLOW, "batched chain: record or run pool still too small");
@@ -1819,6 +1844,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   int64_t *nrun64 = (int64_t *)ws(c, 579, (L4 + 1) * 8), *roff = (int64_t *)ws(c, 580, (L4 + 2) * 8);
   if (!nrun64 || !roff) return ws_fail(c);
   if (L4) kb_run_order<<<nblocks(L4, 256), 256, 0, s>>>(L4, nullptr, o, nullptr);
+  if (nruns) kb_run_rank<<<nblocks(nruns, 256), 256, 0, s>>>(o, nruns, nullptr, nullptr);
   if (L4) kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(o.nrun, nrun64, L4);
   if ((rc = scan_i64(c, nrun64, roff, L4, 514, s))) return rc;
   // phase 3: pulse fits (slots grown and the fits run again when a waveform has more pulses)
@@ -1948,7 +1974,7 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
   uint32_t *skey = (uint32_t *)A(L4 * 4 + 8), *skey2 = (uint32_t *)A(L4 * 4 + 8);
   int32_t *sidx = (int32_t *)A(L4 * 4 + 8), *order = (int32_t *)A(L4 * 4 + 8);
   int32_t *nfired = (int32_t *)A(L4 * 4 + 8);
-  int *counters = (int *)A(64);
+  int *counters = (int *)A(C_INTS * 4);
   Out o{};
   o.xstride = (int64_t)pl.grid_s * 64;
   o.xcur = (uint16_t *)A((size_t)o.xstride * CMAXB * 2);
@@ -1997,14 +2023,14 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
                   o.crl};
   for (void *q : ptrs)
     if (!q) return set_error(c, CRV_E_HIP, "graph: buffer allocation failed");
-  o.rec_count = counters + 2;
-  o.run_count = counters + 3;
-  o.cdesc_count = counters + 4;
-  o.rec_used = counters + 6;
+  o.rec_count = counters + C_REC;
+  o.run_count = counters + C_RUN;
+  o.cdesc_count = counters + C_DESC;
+  o.rec_used = counters + C_USED;
   o.goff = goff;
   o.gcap = gcap;
   o.log0 = 0;
-  kb_zero<<<1, 64, 0, s>>>(counters, 16);
+  kb_zero<<<1, 64, 0, s>>>(counters, C_INTS);
   kb_zero<<<1, 64, 0, s>>>(pl.skip, 2);
   const BStep st{g.edep, g.z, g.t};
   kb_count<<<nblocks(L4, 256), 256, 0, s>>>(p, c->t, c->d, st, pl.T_dev, g.task_key, g.task_off, g.perm, g.evfirst,
@@ -2033,12 +2059,12 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
                                    Queue{counters, pl.tile, L4, qorder, 0, pl.T_dev, pl.skip}, nfired, c->flags);
   CRV_HIP(c, hipGetLastError());
   CRV_HIP(c, mark(1));
-  kb_dmark<<<1, 1, 0, s>>>(o.cdesc_count, counters + 9);  // descriptors [counters[8] = 0, counters[9])
+  kb_dmark<<<1, 1, 0, s>>>(o.cdesc_count, counters + C_DRANGE + 1);  // descriptors [0, counters[C_DRANGE + 1])
   CRV_HIP(c, mark(2));
-  kb_wrec<<<pl.grid_r, 64, 0, s>>>(p, c->t, c->d, g.task_key, o, counters + 8, pl.skip, c->flags);
+  kb_wrec<<<pl.grid_r, 64, 0, s>>>(p, c->t, c->d, g.task_key, o, counters + C_DRANGE, pl.skip, c->flags);
   CRV_HIP(c, hipGetLastError());
   CRV_HIP(c, mark(3));
-  kb_check_wave<<<1, 1, 0, s>>>(c->flags, counters + 2, o.rec_cap, o.run_cap, o.cdesc_cap, pl.skip);
+  kb_check_wave<<<1, 1, 0, s>>>(c->flags, counters, o.rec_cap, o.run_cap, o.cdesc_cap, pl.skip);
   // photons drawn and pixels fired
   kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(nfired, fired64, L4);
   {
@@ -2050,6 +2076,7 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
     CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, sipm_nph, sums + 1, (int)L4, s));
   }
   kb_run_order<<<nblocks(L4, 256), 256, 0, s>>>(L4, pl.T_dev, o, pl.skip);
+  kb_run_rank<<<nblocks(o.run_cap, 256), 256, 0, s>>>(o, o.run_cap, o.run_count, pl.skip);
   kb_widen<<<nblocks(L4, 256), 256, 0, s>>>(o.nrun, nrun64, L4);
   if ((rc = scan_fixed(c, ar, nrun64, roff, L4, s))) return rc;
   kb_zero32<<<nblocks(o.run_cap + 1, 256), 256, 0, s>>>(o.npr, o.run_cap + 1);
@@ -2067,7 +2094,7 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
                                                          o.run_count, pl.skip);
   CRV_HIP(c, hipGetLastError());
   kb_pad_pulses<<<nblocks(pl.np_cap, 256), 256, 0, s>>>(pl.np_cap, poff + o.run_cap, pl.skip, P.ev);
-  kb_sizes<<<1, 1, 0, s>>>(poff + o.run_cap, sums, counters + 2, pl.skip, pl.sizes);
+  kb_sizes<<<1, 1, 0, s>>>(poff + o.run_cap, sums, counters, pl.skip, pl.sizes);
   CRV_HIP(c, hipGetLastError());
   return clusters_fixed(c, pl, ar, P.ev, pbar, psipm, P.time, P.pe, P.seq, s);
 }
