@@ -106,6 +106,32 @@ typedef struct crv_charges {
  * (response/sipm.py). FP64, one launch per stage. */
 int crv_charges_run(crv_ctx *ctx, const crv_steps *steps, crv_charges *out, crv_stream stream);
 
+/* Photon arrivals at the SiPMs (device arrays): SiPM label (event key * 32768 + channel), time, and an id carried to
+ * the charges. A photon's draws are keyed by (label, its position in the list). */
+typedef struct crv_photon_list {
+  const int64_t *label;
+  const double *time;
+  const int64_t *photon;
+  int64_t n;
+} crv_photon_list;
+
+/* Dark noise and trap afterpulses (reference response/sipm.py:26-47 SiPMConfig thermal_rate, trap*_prob and
+ * trap*_lifetime; simulate's t_start, t_end and thermal_chans, sipm.py:102-121). */
+typedef struct crv_sipm_noise {
+  double thermal_rate;                  /* dark discharges per ns at full overvoltage (0: none) */
+  double trap0_prob, trap1_prob;        /* per firing (0: none) */
+  double trap0_lifetime, trap1_lifetime;/* ns */
+  double t_start, t_end;                /* dark noise in [t_start, t_end); photons and afterpulses after t_end dropped */
+  const int64_t *thermal_labels;        /* device array of distinct SiPM labels that get dark noise */
+  int64_t n_thermal;
+} crv_sipm_noise;
+
+/* Replaces response/sipm.py:102-198 simulate (with thermal_chans, t_start, t_end): photon arrivals plus dark noise ->
+ * SiPM charges with cross-talk and trap afterpulses, sorted by (label, time); photon -1 marks dark discharges and
+ * their descendants. FP64. out->n_photons counts the photons kept and the dark discharges. */
+int crv_sipm_run(crv_ctx *ctx, const crv_photon_list *photons, const crv_sipm_noise *noise, crv_charges *out,
+                 crv_stream stream);
+
 /* Zero-suppressed digis (reference chain.digis return dict) and the 128-sample ADC record of every charge cluster
  * that has a digi. */
 typedef struct crv_digis {

@@ -110,7 +110,7 @@ void *ws(Ctx *c, int slot, size_t bytes);
 
 // device flag words
 enum Flag { F_UNSORTED = 0, F_COUNTER = 1, F_CAPPED = 2, F_OVERFLOW = 3, F_CASCADE = 4, F_OVERLONG = 5, F_BADCHAN = 6,
-            F_NFLAGS = 8 };
+            F_BADINPUT = 8, F_NFLAGS = 10 };
 
 int set_error(Ctx *c, int code, const std::string &msg);
 // CRV_E_HIP with the workspace request that could not be allocated
@@ -133,6 +133,8 @@ struct StepGroups {  // hit counters: steps grouped by (event, counter)
 };
 int group_steps(Ctx *c, const crv_steps *st, StepGroups &g, hipStream_t s);
 int charges_stage(Ctx *c, StepGroups &g, crv_charges *out, hipStream_t s);
+// photon list + dark noise -> charges with cross-talk and trap afterpulses (crv_sipm_run)
+int sipm_list_stage(Ctx *c, const crv_photon_list *ph, const crv_sipm_noise *nz, crv_charges *out, hipStream_t s);
 int digis_stage(Ctx *c, const int64_t *label, const double *time, const double *charge, int64_t n, crv_digis *out,
                 hipStream_t s);
 int waveforms_stage(Ctx *c, const int64_t *label, const int64_t *starttdc, const int16_t *adc, int64_t n,

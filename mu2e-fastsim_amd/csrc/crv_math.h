@@ -117,7 +117,8 @@ CRV_HD double fm_asin(double x) {
 // Philox4x32-10 and the draw layout: key = (seed, kind + 16 * evkey_hi), counter = (evkey_lo, c1, c2, c3).
 
 enum Kind : uint32_t {
-  KIND_COUNT = 1, KIND_DELAY = 2, KIND_PIXEL = 3, KIND_XTALK = 4, KIND_XTFIRE = 5, KIND_TDC0 = 6, KIND_NOISE = 7
+  KIND_COUNT = 1, KIND_DELAY = 2, KIND_PIXEL = 3, KIND_XTALK = 4, KIND_XTFIRE = 5, KIND_TDC0 = 6, KIND_NOISE = 7,
+  KIND_THERMAL_N = 8, KIND_THERMAL = 9, KIND_TRAP = 10, KIND_AFTER = 11  // dark noise and trap afterpulses
 };
 
 struct U4 {

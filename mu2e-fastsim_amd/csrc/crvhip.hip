@@ -579,6 +579,16 @@ int crv_charges_run(crv_ctx *ctx, const crv_steps *steps, crv_charges *out, crv_
   return charges_stage(c, g, out, s);
 }
 
+int crv_sipm_run(crv_ctx *ctx, const crv_photon_list *photons, const crv_sipm_noise *noise, crv_charges *out,
+                 crv_stream stream) {
+  if (int rc = guard(ctx)) return rc;
+  Ctx *c = &ctx->c;
+  if (!photons || !noise || !out) return set_error(c, CRV_E_ARG, "NULL argument");
+  if (!c->have_params) return set_error(c, CRV_E_STATE, "parameters not set");
+  if (!c->t.cdf) return set_error(c, CRV_E_STATE, "photon map not uploaded");
+  return sipm_list_stage(c, photons, noise, out, (hipStream_t)stream);
+}
+
 int crv_digis_run(crv_ctx *ctx, const int64_t *label, const double *time, const double *charge, int64_t n,
                   crv_digis *out, crv_stream stream) {
   if (int rc = guard(ctx)) return rc;

@@ -13,13 +13,12 @@
 #include <hipcub/hipcub.hpp>
 
 #include "crv_internal.h"
+#include "crv_sipm_dev.h"
 
 namespace crv {
 
 namespace {
 
-constexpr int CMAX = 64;     // cascade candidates per generation and lane
-constexpr int MAX_GEN = 31;  // cascade generations the draw layout encodes
 
 __global__ void k_step_keys(const uint32_t *__restrict__ ev, const uint32_t *__restrict__ counter, int64_t n,
                             uint64_t *__restrict__ key, int32_t *__restrict__ idx, int32_t *__restrict__ evfirst,
@@ -104,20 +103,7 @@ struct Gen {  // one photon stream of a (step, SiPM): ordered exponential delays
   int32_t *r;
 };
 
-__device__ __forceinline__ bool bit_get(const uint32_t *bm, int lane, int pix) {
-  return (bm[(pix >> 5) * 64 + lane] >> (pix & 31)) & 1u;
-}
-__device__ __forceinline__ void bit_set(uint32_t *bm, int lane, int pix) { bm[(pix >> 5) * 64 + lane] |= 1u << (pix & 31); }
-__device__ __forceinline__ bool dead_pix(const uint32_t *dead, int pix) { return (dead[pix >> 5] >> (pix & 31)) & 1u; }
 
-__device__ __forceinline__ int upper_bound(const double *__restrict__ a, int n, double x) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (a[mid] <= x) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
 
 __global__ __launch_bounds__(64) void k_sipm(crv_params p, Tables t, Derived dv, Steps st, int64_t T,
                                              const uint64_t *__restrict__ task_key,
@@ -127,10 +113,7 @@ __global__ __launch_bounds__(64) void k_sipm(crv_params p, Tables t, Derived dv,
                                              double *__restrict__ c_v, int32_t *__restrict__ c_row,
                                              int16_t *__restrict__ c_pix, int32_t *__restrict__ c_count,
                                              int *__restrict__ flags) {
-  __shared__ uint32_t bm[CRV_DEAD_WORDS * 64];
-  __shared__ uint16_t cur[CMAX * 64];
-  __shared__ uint16_t cand[CMAX * 64];
-  __shared__ uint8_t mult[CMAX * 64];
+  __shared__ SipmLds L;
   const int lane = threadIdx.x;
   int64_t g = (int64_t)blockIdx.x * 64 + lane;
   if (g >= T * 4) return;
@@ -140,11 +123,10 @@ __global__ __launch_bounds__(64) void k_sipm(crv_params p, Tables t, Derived dv,
   uint32_t ev = (uint32_t)(key >> 13);
   int ch = (int)(key & 8191u) * 4 + s;
   int a = task_off[tk], b = task_off[tk + 1];
-  int64_t base = coff[g], cap = coff[g + 1] - base;
-  int32_t cnt = 0;
+  LaneRecs R{c_time, c_v, c_row, c_pix, coff[g], coff[g + 1] - coff[g], 0};
   c_count[g] = 0;
   if (ch >= t.n_channels || (t.status[ch] & 0b101) != 0) return;
-  for (int w = 0; w < CRV_DEAD_WORDS; ++w) bm[w * 64 + lane] = 0u;
+  for (int w = 0; w < CRV_DEAD_WORDS; ++w) L.bm[w * 64 + lane] = 0u;
   Draw dr{(uint32_t)p.seed, p.evkey_hi};
   const double t_end = p.photon_window_hi;
   // start the photon streams
@@ -203,99 +185,24 @@ __global__ __launch_bounds__(64) void k_sipm(crv_params p, Tables t, Derived dv,
       py += unif(u.z) >= 0.5 ? 1 : 0;
     }
     int pix = px * CRV_NPIX + py;  // px, py <= 40: the photon map is at most 40 x 40 bins
-    if (dead_pix(t.dead, pix)) continue;
-    double e = 0.0;
-    if (bit_get(bm, lane, pix)) {
-      for (int32_t q = cnt - 1; q >= 0; --q)
-        if (c_pix[base + q] == pix) {
-          e = fm_exp(-(tph - c_time[base + q]) / p.time_constant);
-          break;
-        }
-    }
-    double v = p.overvoltage * (1.0 - e);
+    if (px_dead(t.dead, pix)) continue;
+    double v = p.overvoltage * (1.0 - recovery(p, L, lane, R, pix, tph));
     double pf = p.p1 * (1.0 - fm_exp(-v / p.p2));
     double pr = 1.0 - (1.0 - pf);
     if (!(unif(u.w) < pr)) continue;
-    if (cnt >= cap) { overflow = true; break; }
-    c_time[base + cnt] = tph;
-    c_v[base + cnt] = v;
-    c_row[base + cnt] = row;
-    c_pix[base + cnt] = (int16_t)pix;
-    ++cnt;
-    bit_set(bm, lane, pix);
-    // cross-talk cascade, breadth-first
-    int ncur = 1, gnum = 0;
-    cur[lane] = (uint16_t)pix;
-    while (ncur > 0) {
-      int ncand = 0;
-      for (int ai = 0; ai < ncur; ++ai) {
-        int f = cur[ai * 64 + lane];
-        int fx = f / CRV_NPIX, fy = f % CRV_NPIX;
-        U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)f));
-        const uint32_t uw[4] = {ux.x, ux.y, ux.z, ux.w};
-        const int nxs[4] = {fx - 1, fx + 1, fx, fx};
-        const int nys[4] = {fy, fy, fy - 1, fy + 1};
-        const bool oks[4] = {fx > 0, fx + 1 < p.n_pixels_x, fy > 0, fy + 1 < p.n_pixels_y};
-        for (int nb = 0; nb < 4; ++nb) {
-          if (!oks[nb] || !(unif(uw[nb]) < dv.q_ct)) continue;
-          int c = nxs[nb] * CRV_NPIX + nys[nb];
-          int at = 0;
-          while (at < ncand && cand[at * 64 + lane] < c) ++at;
-          if (at < ncand && cand[at * 64 + lane] == c) {
-            mult[at * 64 + lane] += 1;
-          } else if (ncand < CMAX) {
-            for (int q = ncand; q > at; --q) {
-              cand[q * 64 + lane] = cand[(q - 1) * 64 + lane];
-              mult[q * 64 + lane] = mult[(q - 1) * 64 + lane];
-            }
-            cand[at * 64 + lane] = (uint16_t)c;
-            mult[at * 64 + lane] = 1;
-            ++ncand;
-          } else {
-            overflow = true;
-          }
-        }
-      }
-      if (ncand == 0 || overflow) break;
-      ++gnum;
-      if (gnum > MAX_GEN) { atomicAdd(&flags[F_CASCADE], 1); overflow = true; break; }
-      ncur = 0;
-      for (int ai = 0; ai < ncand; ++ai) {
-        int c = cand[ai * 64 + lane];
-        int m = mult[ai * 64 + lane];
-        if (dead_pix(t.dead, c)) continue;
-        double ec = 0.0;
-        if (bit_get(bm, lane, c)) {
-          for (int32_t q = cnt - 1; q >= 0; --q)
-            if (c_pix[base + q] == c) {
-              ec = fm_exp(-(tph - c_time[base + q]) / p.time_constant);
-              break;
-            }
-        }
-        double vc = p.overvoltage * (1.0 - ec);
-        double pfc = p.p1 * (1.0 - fm_exp(-vc / p.p2));
-        double q1 = 1.0 - pfc;
-        double qm = q1;
-        for (int mm = 2; mm <= m; ++mm) qm = qm * q1;
-        double prc = 1.0 - qm;
-        double uc = unif(dr(KIND_XTFIRE, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)c)).x);
-        if (!(uc < prc)) continue;
-        if (cnt >= cap) { overflow = true; break; }
-        c_time[base + cnt] = tph;
-        c_v[base + cnt] = vc;
-        c_row[base + cnt] = row;
-        c_pix[base + cnt] = (int16_t)c;
-        ++cnt;
-        bit_set(bm, lane, c);
-        cur[ncur * 64 + lane] = (uint16_t)c;
-        ++ncur;
-      }
-      if (overflow) break;
+    if (!add_record(R, tph, v, row, pix)) {
+      overflow = true;
+      break;
     }
-    if (overflow) break;
+    px_set(L, lane, pix);
+    if (!cascade(p, t, dv, dr, ev, (uint32_t)ch, j, (uint32_t)r, tph, row, pix, R, L, lane, flags,
+                 [](int, int, double) {})) {
+      overflow = true;
+      break;
+    }
   }
   if (overflow) atomicAdd(&flags[F_OVERFLOW], 1);
-  c_count[g] = cnt;
+  c_count[g] = R.cnt;
 }
 
 __global__ void k_charges_out(crv_params p, int64_t T, const uint64_t *__restrict__ task_key,

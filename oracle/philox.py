@@ -23,6 +23,11 @@ KIND_XTALK = 4     # four cross-talk proposals of a fired pixel
 KIND_XTFIRE = 5    # fire decision of a cross-talk candidate pixel
 KIND_TDC0 = 6      # FEB clock phase per (event, ROC, FEB)
 KIND_NOISE = 7     # electronics noise, two samples per call
+KIND_THERMAL_N = 8  # dark-noise count per SiPM (Poisson, the photon-count sampler)
+KIND_THERMAL = 9    # dark discharge k of a SiPM: time, pixel x, pixel y
+KIND_TRAP = 10      # trap capture of a firing: trap 0 test and lifetime, trap 1 test and lifetime
+KIND_AFTER = 11     # fire decision of a trap afterpulse: trap 0, trap 1
+DARK_J = 0xFFFFFFFF  # step ordinal of dark discharges in the photon identity (photons have smaller ones)
 
 
 def philox4x32(c0, c1, c2, c3, k0, k1):

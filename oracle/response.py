@@ -97,28 +97,68 @@ def photons(steps, status, dev, opt, seed):
 # uniform sources for the SiPM simulation
 
 class PhiloxSource:
-    """Draws keyed by photon identity: ids = (evkey, chan, j, rank) arrays per photon (input order)."""
+    """Draws keyed by photon identity: ids = (evkey, chan, j, rank) arrays per photon (input order). Dark discharges
+    take the identity (evkey, chan, DARK_J, k), k their index within the SiPM, appended by `thermal`."""
 
     def __init__(self, seed, ids):
         self.seed = seed
         self.ev, self.ch, self.j, self.r = (np.asarray(a, np.int64) for a in ids)
         self._root = None
+        self._dark = None
+
+    def subset(self, keep):
+        return PhiloxSource(self.seed, (self.ev[keep], self.ch[keep], self.j[keep], self.r[keep]))
+
+    def _key(self, gid, gen, pix):
+        return self.ev[gid], self.ch[gid], self.j[gid], PX.pack_photon(self.r[gid], gen, pix)
+
+    def thermal(self, labels, lam, t_start, t_end):
+        """A Poisson(lam) count of dark discharges per SiPM label (the photon-count sampler on the KIND_THERMAL_N
+        draws (evkey, chan, 0, call)), discharge k at t_start + (t_end - t_start) u (KIND_THERMAL draw
+        (evkey, chan, k, 0), x word)."""
+        labels = np.asarray(labels, np.int64)
+        ev, ch = labels // CHAN_MULT, labels % CHAN_MULT
+        n, capped = R.poisson(np.full(len(labels), lam, np.float64),
+                              lambda call: PX.draw(self.seed, PX.KIND_THERMAL_N, ev, ch, 0, call))
+        if capped.any() or (n > 0xFFFF).any():
+            raise RuntimeError("dark-noise count beyond the draw layout (65535 per SiPM)")
+        rep = np.repeat(np.arange(len(labels)), n)
+        k = np.arange(len(rep)) - np.repeat(np.cumsum(n) - n, n)
+        u = PX.draw(self.seed, PX.KIND_THERMAL, ev[rep], ch[rep], k, 0)
+        self._dark = (u[1], u[2])
+        self.ev, self.ch = np.r_[self.ev, ev[rep]], np.r_[self.ch, ch[rep]]
+        self.j, self.r = np.r_[self.j, np.full(len(rep), PX.DARK_J, np.int64)], np.r_[self.r, k]
+        return labels[rep], t_start + (t_end - t_start) * u[0]
 
     def pixels(self, n):
         u = PX.draw(self.seed, PX.KIND_PIXEL, self.ev, self.ch, self.j, PX.pack_photon(self.r))
         self._root = u[3]
         return u[0], u[1], u[2]
 
+    def dark_pixels(self, n, nx, ny):
+        """Uniform pixel of each dark discharge: floor(u nx), floor(u ny) of its KIND_THERMAL draw (y, z words)."""
+        ux, uy = self._dark
+        return np.floor(ux * nx).astype(np.int64), np.floor(uy * ny).astype(np.int64)
+
     def fire(self, gid, gen, pix):
         if gen == 0:
             return self._root[gid]
-        return PX.draw(self.seed, PX.KIND_XTFIRE, self.ev[gid], self.ch[gid], self.j[gid],
-                       PX.pack_photon(self.r[gid], gen, pix))[0]
+        return PX.draw(self.seed, PX.KIND_XTFIRE, *self._key(gid, gen, pix))[0]
 
     def xtalk(self, gid, gen, pix):
-        u = PX.draw(self.seed, PX.KIND_XTALK, self.ev[gid], self.ch[gid], self.j[gid],
-                    PX.pack_photon(self.r[gid], gen, pix))
-        return np.stack(u, 1)
+        return np.stack(PX.draw(self.seed, PX.KIND_XTALK, *self._key(gid, gen, pix)), 1)
+
+    def traps(self, gid, gen, pix, qs):
+        """Per trap type: (captured mask, lifetime uniforms of the captured) from the firing's KIND_TRAP draw."""
+        u = PX.draw(self.seed, PX.KIND_TRAP, *self._key(gid, gen, pix))
+        out = []
+        for i, q in enumerate(qs):
+            tr = u[2 * i] < q
+            out.append((tr, u[2 * i + 1][tr]))
+        return out
+
+    def after(self, gid, gen, pix, typ):
+        return PX.draw(self.seed, PX.KIND_AFTER, *self._key(np.array([gid]), gen, pix))[typ][0]
 
 
 class SequentialSource:
@@ -127,14 +167,35 @@ class SequentialSource:
     def __init__(self, rng):
         self.rng = rng
 
+    def subset(self, keep):
+        return self
+
+    def thermal(self, labels, lam, t_start, t_end):
+        n = self.rng.poisson(lam, len(labels))
+        tl = np.repeat(np.asarray(labels, np.int64), n)
+        return tl, t_start + (t_end - t_start) * self.rng.random(len(tl))
+
     def pixels(self, n):
         return self.rng.random(n), self.rng.random(n), self.rng.random(n)
+
+    def dark_pixels(self, n, nx, ny):
+        return self.rng.integers(0, nx, n), self.rng.integers(0, ny, n)
 
     def fire(self, gid, gen, pix):
         return self.rng.random(len(gid))
 
     def xtalk(self, gid, gen, pix):
         return self.rng.random((len(gid), 4))
+
+    def traps(self, gid, gen, pix, qs):
+        out = []
+        for q in qs:
+            tr = self.rng.random(len(gid)) < q
+            out.append((tr, self.rng.random(int(tr.sum()))))
+        return out
+
+    def after(self, gid, gen, pix, typ):
+        return self.rng.random()
 
 
 # ---------------------------------------------------------------------------------------------------------------------
@@ -173,27 +234,39 @@ def _pow_int(q, m):
 MAX_GEN = 31
 
 
-def sipm_simulate(label, time, photon, pmap, cfg, source, t_end=1e5):
-    """SiPM charges for photon arrivals (reference response/sipm.py:102-198, no thermal noise / traps).
+def sipm_simulate(label, time, photon, pmap, cfg, source, t_end=1e5, t_start=0.0, thermal_labels=None):
+    """SiPM charges for photon arrivals (reference response/sipm.py:102-198).
 
     label [P] SiPM label per photon, time [P], photon [P] carried id (step row); source: PhiloxSource built on the
-    same P photons (in this order) or SequentialSource. Returns dict chan, time, charge, pe, photon sorted by
-    (chan, time) with the record order breaking ties.
+    same P photons (in this order) or SequentialSource. thermal_labels: SiPMs that get dark noise at
+    cfg.thermal_rate over [t_start, t_end) (sipm.py:113-121: Poisson(rate / p_fire(Vov) * (t_end - t_start))
+    discharges per SiPM at uniform times and pixels, photon id -1, merged into the photons' time order).
+    cfg.trap0_prob / trap1_prob: each firing is captured by trap i with probability prob_i / p_fire(Vov) and
+    releases an afterpulse on its pixel after an Exp(lifetime_i) delay (sipm.py:140, :164-167); the afterpulses of a
+    SiPM are tried after all its photons, in time order, with the pixel's recovery at that point and no cross-talk
+    (sipm.py:179-193). Returns dict chan, time, charge, pe, photon sorted by (chan, time), record order on ties.
     """
     label = np.asarray(label, np.int64)
     time = np.asarray(time, np.float64)
     photon = np.asarray(photon, np.int64)
     keep = time <= t_end
-    gid_all = np.flatnonzero(keep)
     label, time, photon = label[keep], time[keep], photon[keep]
-    if isinstance(source, PhiloxSource):
-        src_keep = keep
-        source = PhiloxSource(source.seed, (source.ev[src_keep], source.ch[src_keep], source.j[src_keep],
-                                            source.r[src_keep]))
+    source = source.subset(keep)
+    pf_full = p_fire(cfg, np.float64(cfg.overvoltage))
+    ndark = 0
+    if cfg.thermal_rate > 0 and thermal_labels is not None:
+        lam = cfg.thermal_rate / pf_full * (t_end - t_start)
+        tl, tt = source.thermal(thermal_labels, lam, t_start, t_end)
+        ndark = len(tl)
+        label, time = np.r_[label, tl], np.r_[time, tt]
+        photon = np.r_[photon, np.full(ndark, -1, np.int64)]
     n = len(label)
     cum = cdf_of_map(pmap)
     ub, uhx, uhy = source.pixels(n)
     px, py = pixel_ids(cum, np.shape(pmap)[1], ub, uhx, uhy, cfg.pixel_id)
+    if ndark:
+        dx, dy = source.dark_pixels(ndark, cfg.n_pixels_x, cfg.n_pixels_y)
+        px, py = np.r_[px[:n - ndark], dx], np.r_[py[:n - ndark], dy]
     dead = np.zeros((NPIX, NPIX), bool)
     for x, y in cfg.inactive:
         dead[x, y] = True
@@ -211,7 +284,10 @@ def sipm_simulate(label, time, photon, pmap, cfg, source, t_end=1e5):
     S = len(chans)
     last = np.full((S, NPIX * NPIX), -np.inf)
     q_ct = q_crosstalk(cfg)
+    q_trap = (cfg.trap0_prob / pf_full, cfg.trap1_prob / pf_full)
+    life = (cfg.trap0_lifetime, cfg.trap1_lifetime)
     rec = {k: [] for k in ("slot", "time", "v", "photon")}
+    traps = []  # per capture: slot, pixel, release time, photon, and the firing's identity (gid, gen) and trap type
     for k in range(int(cnt.max())):
         nn = int((cnt > k).sum())
         idx = start[:nn] + k
@@ -230,6 +306,10 @@ def sipm_simulate(label, time, photon, pmap, cfg, source, t_end=1e5):
             sl, pix, t, ph, v, g = sl[fire], pix[fire], t[fire], ph[fire], v[fire], g[fire]
             last[sl, pix] = t
             rec["slot"].append(sl), rec["time"].append(t), rec["v"].append(v), rec["photon"].append(ph)
+            if q_trap[0] > 0 or q_trap[1] > 0:
+                for typ, (tr, ul) in enumerate(source.traps(g, gen, pix, q_trap)):
+                    traps.append((sl[tr], pix[tr], t[tr] - life[typ] * F.log(ul), ph[tr], g[tr],
+                                  np.full(int(tr.sum()), gen), np.full(int(tr.sum()), typ)))
             x, y = pix // NPIX, pix % NPIX
             nx = np.stack([x - 1, x + 1, x, x], 1)
             ny = np.stack([y, y, y - 1, y + 1], 1)
@@ -246,6 +326,17 @@ def sipm_simulate(label, time, photon, pmap, cfg, source, t_end=1e5):
             gen += 1
             if gen > MAX_GEN and len(front[0]):
                 raise RuntimeError("SiPM cross-talk cascade deeper than the draw layout allows")
+    if traps:
+        ts, tp, tt, tph, tg, tgen, ttyp = (np.concatenate([x[i] for x in traps]) for i in range(7))
+        for i in np.lexsort((tt, ts)):
+            sl, pp, t1 = ts[i], tp[i], tt[i]
+            if t1 > t_end:
+                continue
+            v = cfg.overvoltage * (1.0 - F.exp(-(t1 - last[sl, pp]) / cfg.time_constant))
+            if source.after(tg[i], tgen[i], pp, ttyp[i]) < p_fire(cfg, v):
+                last[sl, pp] = t1
+                rec["slot"].append(np.array([sl])), rec["time"].append(np.array([t1]))
+                rec["v"].append(np.array([v])), rec["photon"].append(np.array([tph[i]]))
     slot = np.concatenate(rec["slot"])
     v = np.concatenate(rec["v"])
     out = {"chan": chans[slot], "time": np.concatenate(rec["time"]), "charge": cfg.capacitance * v,

@@ -17,3 +17,9 @@ def pytest_configure(config):
 def golden():
     import numpy as np
     return np.load(os.path.join(ROOT, "tests", "golden", "reference_vectors.npz"))
+
+
+@pytest.fixture(scope="session")
+def noise_golden():
+    import numpy as np
+    return np.load(os.path.join(ROOT, "tests", "golden", "sipm_noise_vectors.npz"))

@@ -5,7 +5,8 @@ reco/waveform.py, reco/pulsefit.py, reco/coinc.py, reco/synth.py) and records th
 fixtures next to this script. The fixtures hold data only (inputs and outputs); the tests compare the oracle with
 them, and the GPU box never sees the reference.
 
-  python tests/golden/make_golden.py
+  python tests/golden/make_golden.py            (reference_vectors.npz)
+  python tests/golden/make_golden.py noise      (sipm_noise_vectors.npz: dark noise and trap afterpulses)
 
 Pinned here: SiPM charges on a shared uniform stream (the oracle's SequentialSource consumes the reference's draw
 order), digis on a shared normal stream, pulse reconstruction, digi grouping, coincidence clusters.
@@ -61,6 +62,45 @@ def sipm_cases():
         out.update({pre + "chan_in": chan, pre + "time_in": t, pre + "photon_in": photon, pre + "pmap": pmap,
                     pre + "seed": np.int64(seed), pre + "t_end": np.float64(c["t_end"]),
                     pre + "cross_talk": np.float64(cfg.cross_talk), pre + "pixel_id": np.array(cfg.pixel_id)})
+        out.update({pre + k: v for k, v in Q.items()})
+    return out
+
+
+def sipm_noise_cases():
+    """Reference sipm.simulate with dark noise (thermal_chans over [t_start, t_end)) and trap afterpulses, on fewer
+    than its 20000-SiPM chunk so that one chunk holds every SiPM."""
+    out = {}
+    rng = np.random.default_rng(23)
+    cases = {
+        "d": dict(n=120, mean=30, cfg=dict(cross_talk=0.05, thermal_rate=0.004), pmap="ones", win=(100.0, 1500.0),
+                  extra=40),
+        "e": dict(n=150, mean=40, cfg=dict(cross_talk=0.1, trap0_prob=0.08, trap1_prob=0.05), pmap="random",
+                  win=(0.0, 5000.0), extra=0),
+        "f": dict(n=100, mean=50, cfg=dict(cross_talk=0.3, thermal_rate=0.01, trap0_prob=0.1, trap1_prob=0.1,
+                                           trap0_lifetime=7.0, trap1_lifetime=40.0, pixel_id="truncate"),
+                  pmap="random", win=(200.0, 1300.0), extra=25),
+    }
+    for name, c in cases.items():
+        n = rng.poisson(c["mean"], c["n"])
+        labels = np.arange(c["n"]) * 7 + 3
+        chan = np.repeat(labels, n)
+        lo, hi = c["win"]
+        t = np.repeat(rng.uniform(lo - 50, hi, c["n"]), n) + rng.normal(0, 3, len(chan))
+        perm = rng.permutation(len(chan))
+        chan, t = chan[perm], t[perm]
+        photon = np.arange(len(chan)) + 1000
+        pmap = np.ones((40, 40)) if c["pmap"] == "ones" else rng.random((40, 40)) ** 4
+        thermal = np.r_[labels, np.arange(c["extra"]) * 7 + 5]
+        cfg = RSI.SiPMConfig(**c["cfg"])
+        seed = 200 + ord(name)
+        Q = RSI.simulate(chan, t, photon, pmap, cfg, np.random.default_rng(seed), t_end=hi, t_start=lo,
+                         thermal_chans=thermal)
+        pre = f"noise_{name}_"
+        out.update({pre + "chan_in": chan, pre + "time_in": t, pre + "photon_in": photon, pre + "pmap": pmap,
+                    pre + "thermal": thermal, pre + "seed": np.int64(seed), pre + "t_start": np.float64(lo),
+                    pre + "t_end": np.float64(hi), pre + "pixel_id": np.array(cfg.pixel_id)})
+        for k in ("cross_talk", "thermal_rate", "trap0_prob", "trap1_prob", "trap0_lifetime", "trap1_lifetime"):
+            out[pre + k] = np.float64(getattr(cfg, k))
         out.update({pre + k: v for k, v in Q.items()})
     return out
 
@@ -220,4 +260,10 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["noise"]:
+        path = os.path.join(HERE, "sipm_noise_vectors.npz")
+        out = sipm_noise_cases()
+        np.savez_compressed(path, **out)
+        print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
+    else:
+        main()

@@ -32,6 +32,39 @@ def test_sipm_matches_reference_on_shared_uniforms(golden, name):
     np.testing.assert_allclose(Q["charge"], g("charge"), rtol=1e-14, atol=0)
 
 
+def _noise_cfg(g):
+    return CF.SiPMConfig(**{k: float(g(k)) for k in ("cross_talk", "thermal_rate", "trap0_prob", "trap1_prob",
+                                                       "trap0_lifetime", "trap1_lifetime")},
+                         pixel_id=str(g("pixel_id")))
+
+
+@pytest.mark.parametrize("name", ["d", "e", "f"])
+def test_sipm_noise_matches_reference_on_shared_uniforms(noise_golden, name):
+    """Dark noise (d), trap afterpulses (e) and both with heavy cross-talk (f) against the reference's sipm.simulate
+    on the same numpy Generator stream. Afterpulse release times go through the oracle's log (numpy's differs by an
+    ulp), so times and PE are compared at 1e-14."""
+    g = lambda k: noise_golden[f"noise_{name}_{k}"]
+    cfg = _noise_cfg(g)
+    Q = ORS.sipm_simulate(g("chan_in"), g("time_in"), g("photon_in"), g("pmap"), cfg,
+                          ORS.SequentialSource(np.random.default_rng(int(g("seed")))), t_end=float(g("t_end")),
+                          t_start=float(g("t_start")), thermal_labels=g("thermal"))
+    assert len(Q["chan"]) == len(g("chan")) > 100
+    np.testing.assert_array_equal(Q["chan"], g("chan"))
+    np.testing.assert_array_equal(Q["photon"], g("photon"))
+    np.testing.assert_allclose(Q["time"], g("time"), rtol=1e-14, atol=0)
+    np.testing.assert_allclose(Q["pe"], g("pe"), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(Q["charge"], g("charge"), rtol=1e-13, atol=1e-28)
+    if cfg.thermal_rate > 0:
+        assert (g("photon") == -1).sum() > 20  # dark discharges present
+    if cfg.trap0_prob > 0:  # the afterpulses are there: without traps the same stream gives other charges
+        import dataclasses
+        Q0 = ORS.sipm_simulate(g("chan_in"), g("time_in"), g("photon_in"), g("pmap"),
+                               dataclasses.replace(cfg, trap0_prob=0.0, trap1_prob=0.0),
+                               ORS.SequentialSource(np.random.default_rng(int(g("seed")))), t_end=float(g("t_end")),
+                               t_start=float(g("t_start")), thermal_labels=g("thermal"))
+        assert len(Q0["chan"]) < len(Q["chan"])
+
+
 def test_template_resampling_matches_reference(golden):
     raw = golden["tmpl_raw"]
     t = OWF.single_pe_template(raw[:, 0], raw[:, 1], 0.5, 1.047, 100.0)
