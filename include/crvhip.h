@@ -107,13 +107,31 @@ typedef struct crv_charges {
 int crv_charges_run(crv_ctx *ctx, const crv_steps *steps, crv_charges *out, crv_stream stream);
 
 /* Photon arrivals at the SiPMs (device arrays): SiPM label (event key * 32768 + channel), time, and an id carried to
- * the charges. A photon's draws are keyed by (label, its position in the list). */
+ * the charges. A photon's draws are keyed by (label, j, rank): the step ordinal and rank crv_photons_run gives it,
+ * or, when j is NULL, its position in the list and rank 0. */
 typedef struct crv_photon_list {
   const int64_t *label;
   const double *time;
   const int64_t *photon;
+  const uint32_t *j, *rank; /* optional (NULL) */
   int64_t n;
 } crv_photon_list;
+
+/* Photon arrivals of CrvSteps: the photon half of crv_charges_run (reference response/photons.py:102-224 with the
+ * closed-form optics, status mask chain.py:199-204). Every photoelectron of a live channel, in (label, step ordinal,
+ * rank) order, with its step row and draw identity; with `shift` (device array indexed by the steps' event), times
+ * become fmod(t + shift[event], period) folded into [0, period) (reference onspill.py:117-118 and :145-148). */
+typedef struct crv_photons {
+  int64_t *label;
+  double *time;
+  int64_t *photon; /* step row */
+  uint32_t *j, *rank;
+  int64_t capacity, n;
+  int64_t capped; /* Poisson draws that hit the rejection bound */
+} crv_photons;
+
+int crv_photons_run(crv_ctx *ctx, const crv_steps *steps, const double *shift, double period, crv_photons *out,
+                    crv_stream stream);
 
 /* Dark noise and trap afterpulses (reference response/sipm.py:26-47 SiPMConfig thermal_rate, trap*_prob and
  * trap*_lifetime; simulate's t_start, t_end and thermal_chans, sipm.py:102-121). */

@@ -48,7 +48,12 @@ class Charges(C.Structure):
 
 
 class PhotonList(C.Structure):
-    _fields_ = [("label", P), ("time", P), ("photon", P), ("n", i64)]
+    _fields_ = [("label", P), ("time", P), ("photon", P), ("j", P), ("rank", P), ("n", i64)]
+
+
+class Photons(C.Structure):
+    _fields_ = [("label", P), ("time", P), ("photon", P), ("j", P), ("rank", P), ("capacity", i64), ("n", i64),
+                ("capped", i64)]
 
 
 class SipmNoise(C.Structure):
@@ -107,7 +112,7 @@ EXPORTS = ("crv_ctx_create", "crv_ctx_destroy", "crv_set_params", "crv_last_erro
            "crv_reco_waveforms_run", "crv_pulses_run", "crv_clusters_run", "crv_chain_run", "crv_chain_fetch",
            "crv_chain_records", "crv_chain_run_batched", "crv_ctx_workspace", "crv_ctx_trim",
            "crv_pulses_run_noalloc", "crv_graph_create", "crv_graph_launch", "crv_graph_fetch", "crv_graph_stats",
-           "crv_graph_destroy", "crv_sipm_run")
+           "crv_graph_destroy", "crv_sipm_run", "crv_photons_run")
 
 _lib = None
 
@@ -130,6 +135,7 @@ def load(path=LIB_PATH):
     lib.crv_upload_geometry.argtypes = [P, i32, P, P, P, P, P, P, i32] + [P] * 8
     lib.crv_charges_run.argtypes = [P, C.POINTER(Steps), C.POINTER(Charges), P]
     lib.crv_sipm_run.argtypes = [P, C.POINTER(PhotonList), C.POINTER(SipmNoise), C.POINTER(Charges), P]
+    lib.crv_photons_run.argtypes = [P, C.POINTER(Steps), P, f64, C.POINTER(Photons), P]
     lib.crv_digis_run.argtypes = [P, P, P, P, i64, C.POINTER(Digis), P]
     lib.crv_reco_waveforms_run.argtypes = [P, P, P, P, i64, C.POINTER(Waveforms), P]
     lib.crv_pulses_run.argtypes = [P] + [P] * 6 + [i64] + [P] * 4 + [C.POINTER(Pulses), P]

@@ -92,11 +92,20 @@ def _host(t, n):
 
 
 def step_table(steps, geom=None):
-    """(evkey, counter, edep, z, t) of a step table, sorted stably by event key (returns also the order). z is the
-    step's position along its counter from the counter centre: the `z` column, or, for the reference's CrvStep columns,
-    the midpoint of (x0, y0, z0)-(x1, y1, z1) placed in its counter by a CRVGeometry `geom` (without one, z0 / z1 are
-    taken as counter-local)."""
+    """(evkey, counter, edep, z, t) of a step table, sorted stably by event key (returns also the order); rows with
+    a non-finite time are left out. See step_columns for z and t."""
     ev = event_key(steps)
+    counter, edep, z, t = step_columns(steps, geom)
+    good = np.isfinite(t)
+    order = np.flatnonzero(good)[np.argsort(ev[good], kind="stable")]
+    return ev[order], counter[order], edep[order], z[order], t[order], order
+
+
+def step_columns(steps, geom=None):
+    """(counter, edep, z, t) per step row. z is the step's position along its counter from the counter centre: the `z`
+    column, or, for the reference's CrvStep columns, the midpoint of (x0, y0, z0)-(x1, y1, z1) placed in its counter by
+    a CRVGeometry `geom` (without one, z0 / z1 are taken as counter-local); t is the `t` column or the midpoint of
+    t0 and t1."""
     counter = np.asarray(steps["counter"] if "counter" in steps else steps["bar"], np.int64)
     edep = np.asarray(steps["edep"], np.float64)
     if "z" in steps:
@@ -110,9 +119,7 @@ def step_table(steps, geom=None):
         z = 0.5 * (np.asarray(steps["z0"], float) + np.asarray(steps["z1"], float))
     t = np.asarray(steps["t"], np.float64) if "t" in steps else 0.5 * (np.asarray(steps["t0"], float) +
                                                                        np.asarray(steps["t1"], float))
-    good = np.isfinite(t)
-    order = np.flatnonzero(good)[np.argsort(ev[good], kind="stable")]
-    return ev[order], counter[order], edep[order], z[order], t[order], order
+    return counter, edep, z, t
 
 
 def _check_response_geometry(geom, cfg):

@@ -105,7 +105,7 @@ def simulate(chan, time, photon, pmap, cfg, rng=None, t_end=1e5, t_start=0.0, th
     dev = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to("cuda")
     d_label, d_time, d_ph = dev(label, np.int64), dev(t, np.float64), dev(ph, np.int64)
     d_tl = dev(tl, np.int64) if thermal else None
-    pl = L.PhotonList(L.ptr(d_label), L.ptr(d_time), L.ptr(d_ph), len(label))
+    pl = L.PhotonList(L.ptr(d_label), L.ptr(d_time), L.ptr(d_ph), None, None, len(label))
     nz = L.SipmNoise(float(cfg.thermal_rate) if thermal else 0.0, float(cfg.trap0_prob), float(cfg.trap1_prob),
                      float(cfg.trap0_lifetime), float(cfg.trap1_lifetime), float(t_start), float(t_end),
                      L.ptr(d_tl), len(tl) if thermal else 0)

@@ -133,6 +133,8 @@ struct StepGroups {  // hit counters: steps grouped by (event, counter)
 };
 int group_steps(Ctx *c, const crv_steps *st, StepGroups &g, hipStream_t s);
 int charges_stage(Ctx *c, StepGroups &g, crv_charges *out, hipStream_t s);
+// the photoelectrons of the steps as a list (crv_photons_run), optionally shifted per event and folded by a period
+int photons_stage(Ctx *c, StepGroups &g, const double *shift, double period, crv_photons *out, hipStream_t s);
 // photon list + dark noise -> charges with cross-talk and trap afterpulses (crv_sipm_run)
 int sipm_list_stage(Ctx *c, const crv_photon_list *ph, const crv_sipm_noise *nz, crv_charges *out, hipStream_t s);
 int digis_stage(Ctx *c, const int64_t *label, const double *time, const double *charge, int64_t n, crv_digis *out,

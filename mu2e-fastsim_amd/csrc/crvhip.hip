@@ -579,6 +579,19 @@ int crv_charges_run(crv_ctx *ctx, const crv_steps *steps, crv_charges *out, crv_
   return charges_stage(c, g, out, s);
 }
 
+int crv_photons_run(crv_ctx *ctx, const crv_steps *steps, const double *shift, double period, crv_photons *out,
+                    crv_stream stream) {
+  if (int rc = guard(ctx)) return rc;
+  Ctx *c = &ctx->c;
+  if (!steps || !out) return set_error(c, CRV_E_ARG, "NULL argument");
+  if (int rc = ready(c, false)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  StepGroups g;
+  int rc = group_steps(c, steps, g, s);
+  if (rc) return rc;
+  return photons_stage(c, g, shift, period, out, s);
+}
+
 int crv_sipm_run(crv_ctx *ctx, const crv_photon_list *photons, const crv_sipm_noise *noise, crv_charges *out,
                  crv_stream stream) {
   if (int rc = guard(ctx)) return rc;

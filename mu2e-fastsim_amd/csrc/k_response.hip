@@ -10,6 +10,8 @@
 // stream's delays are generated in ascending order as exponential order statistics), so no photon sort is needed.
 // The lane keeps an "ever fired" bitmap of its 41 x 41 pixels in LDS and looks a fired pixel's last discharge up
 // in its own charge records (written in time order), newest first.
+#include <cmath>
+
 #include <hipcub/hipcub.hpp>
 
 #include "crv_internal.h"
@@ -205,6 +207,49 @@ __global__ __launch_bounds__(64) void k_sipm(crv_params p, Tables t, Derived dv,
   c_count[g] = R.cnt;
 }
 
+// photoelectrons of each (hit counter, SiPM) lane: its steps in task order, each step's photons in rank order (the
+// ordered delays of k_sipm's streams), at poff[g] onwards; times shifted and folded when `shift` is given
+__global__ void k_photons_out(crv_params p, Steps st, int64_t T, const uint64_t *__restrict__ task_key,
+                              const int32_t *__restrict__ task_off, const int32_t *__restrict__ perm,
+                              const int32_t *__restrict__ evfirst, const int32_t *__restrict__ nph,
+                              const int64_t *__restrict__ poff, const double *__restrict__ shift, double period,
+                              crv_photons out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= T * 4) return;
+  const int64_t tk = g >> 2;
+  const int s = (int)(g & 3);
+  const uint64_t key = task_key[tk];
+  const uint32_t ev = (uint32_t)(key >> 13);
+  const int ch = (int)(key & 8191u) * 4 + s;
+  const int64_t label = ((((int64_t)p.evkey_hi) << 32) | (int64_t)ev) * CRV_CHAN_MULT + ch;
+  const Draw dr{(uint32_t)p.seed, p.evkey_hi};
+  int64_t o = poff[g];
+  for (int k = task_off[tk]; k < task_off[tk + 1]; ++k) {
+    const int32_t N = nph[(int64_t)k * 4 + s];
+    if (N <= 0) continue;
+    const int32_t i = perm[k];
+    const uint32_t j = (uint32_t)(i - evfirst[i]);
+    const double tb = st.t[i] + dist_to_end(p, st.z[i], s) * p.inv_speed;
+    double S = 0.0;
+    U4 uu{0u, 0u, 0u, 0u};
+    for (int32_t r = 0; r < N; ++r, ++o) {
+      if ((r & 3) == 0) uu = dr(KIND_DELAY, ev, (uint32_t)ch, j, (uint32_t)(r >> 2));
+      const uint32_t x = (r & 3) == 0 ? uu.x : (r & 3) == 1 ? uu.y : (r & 3) == 2 ? uu.z : uu.w;
+      S = S + (-fm_log(unif(x))) / (double)(N - r);
+      double tt = tb + p.tau * S;
+      if (shift) {
+        tt = fmod(tt + shift[ev], period);
+        if (tt < 0) tt += period;
+      }
+      out.label[o] = label;
+      out.time[o] = tt;
+      out.photon[o] = i;
+      out.j[o] = j;
+      out.rank[o] = (uint32_t)r;
+    }
+  }
+}
+
 __global__ void k_charges_out(crv_params p, int64_t T, const uint64_t *__restrict__ task_key,
                               const int64_t *__restrict__ coff, const int64_t *__restrict__ ooff,
                               const double *__restrict__ c_time, const double *__restrict__ c_v,
@@ -364,6 +409,43 @@ int group_steps_fixed(Ctx *c, Plan &pl, Arena &ar, StepGroups &g, hipStream_t s)
   void *tmp3 = ar.get(tb3);
   if (!tmp3) return set_error(c, CRV_E_HIP, "graph: scan buffer allocation failed");
   CRV_HIP(c, hipcub::DeviceScan::ExclusiveSum(tmp3, tb3, cnt, g.task_off, (int)(n + 1), s));
+  return CRV_OK;
+}
+
+int photons_stage(Ctx *c, StepGroups &g, const double *shift, double period, crv_photons *out, hipStream_t s) {
+  const int64_t T = g.T, L = T * 4;
+  out->capped = 0;
+  if (shift && !(std::isfinite(period) && period > 0))
+    return set_error(c, CRV_E_ARG, "photons: the fold period must be finite and positive");
+  if (T == 0) {
+    out->n = 0;
+    return CRV_OK;
+  }
+  Steps st{g.ev, g.counter, g.edep, g.z, g.t};
+  int32_t *nph = (int32_t *)ws(c, 10, g.n_steps * 4 * 4);
+  int64_t *sipm_nph = (int64_t *)ws(c, 11, (L + 1) * 8);
+  int64_t *poff = (int64_t *)ws(c, 13, (L + 1) * 8);
+  if (!nph || !sipm_nph || !poff) return ws_fail(c);
+  CRV_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int) * F_NFLAGS, s));
+  k_photon_count<<<nblocks(L, 256), 256, 0, s>>>(c->p, c->t, c->d, st, T, g.task_key, g.task_off, g.perm, g.evfirst,
+                                                 nph, sipm_nph, c->flags);
+  CRV_HIP(c, hipGetLastError());
+  int rc = scan_i64(c, sipm_nph, poff, L, 20, s);
+  if (rc) return rc;
+  int64_t nphot = 0;
+  if ((rc = read_i64(c, poff + L, &nphot, s))) return rc;
+  int hf[F_NFLAGS];
+  CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
+  CRV_HIP(c, hipStreamSynchronize(s));
+  if (hf[F_BADCHAN]) return set_error(c, CRV_E_ARG, "channel beyond the uploaded conditions");
+  if (hf[F_OVERFLOW]) return set_error(c, CRV_E_OVERFLOW, "photons: more than 65535 photoelectrons from one step");
+  out->capped = hf[F_CAPPED];
+  out->n = nphot;
+  if (nphot > out->capacity) return set_error(c, CRV_E_CAPACITY, "photons: output capacity too small");
+  k_photons_out<<<nblocks(L, 256), 256, 0, s>>>(c->p, st, T, g.task_key, g.task_off, g.perm, g.evfirst, nph, poff,
+                                                shift, period, *out);
+  CRV_HIP(c, hipGetLastError());
+  CRV_HIP(c, hipStreamSynchronize(s));
   return CRV_OK;
 }
 

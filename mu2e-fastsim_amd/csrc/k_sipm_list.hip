@@ -9,8 +9,9 @@
 //    probability prob_i / p_fire(Vov) and releases an afterpulse on its pixel after an Exp(lifetime_i) delay;
 //  * a SiPM's afterpulses are tried after all its entries, in time order, against the pixel's last discharge at that
 //    point, with p_fire(v) and no cross-talk; those after t_end are dropped.
-// Draws are keyed by the entry's identity: a photon is (label, list position, rank 0), dark discharge k of a SiPM is
-// (label, DARK_J, k); cascade, trap and afterpulse draws add the generation and the pixel.
+// Draws are keyed by the entry's identity: a photon is (label, j, rank) as crv_photons_run gives it (or its list
+// position and rank 0), dark discharge k of a SiPM is (label, DARK_J, k); cascade, trap and afterpulse draws add the
+// generation and the pixel.
 //
 // Layout: entries (photons in list order, then dark discharges) are sorted by (label, time, entry) with two stable
 // radix sorts, cut into SiPMs by a run-length encoding of the labels, and one lane simulates one SiPM in the record
@@ -56,20 +57,19 @@ __device__ __forceinline__ void label_split(int64_t label, uint32_t &ev, uint32_
   ch = (uint32_t)(label % CRV_CHAN_MULT);
 }
 
-__global__ void k_list_entries(const int64_t *__restrict__ label, const double *__restrict__ time,
-                               const int64_t *__restrict__ photon, int64_t n, double t_end, Entries E,
-                               int *__restrict__ flags) {
+__global__ void k_list_entries(crv_photon_list ph, double t_end, Entries E, int *__restrict__ flags) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t lb = label[i];
-  const double t = time[i];
-  if (lb < 0 || !isfinite(t)) atomicAdd(&flags[F_BADINPUT], 1);
+  if (i >= ph.n) return;
+  const int64_t lb = ph.label[i];
+  const double t = ph.time[i];
+  const uint32_t j = ph.j ? ph.j[i] : (uint32_t)i, r = ph.j ? ph.rank[i] : 0u;
+  if (lb < 0 || !isfinite(t) || j == DARK_J || r > 0xFFFFu) atomicAdd(&flags[F_BADINPUT], 1);
   E.label[i] = t <= t_end ? lb : NO_LABEL;
   E.time[i] = t;
-  E.j[i] = (uint32_t)i;
-  E.r[i] = 0u;
+  E.j[i] = j;
+  E.r[i] = r;
   E.pix[i] = -1;
-  E.photon[i] = photon[i];
+  E.photon[i] = ph.photon[i];
 }
 
 __global__ void k_dark_count(crv_params p, const int64_t *__restrict__ tl, int64_t nt, double lam,
@@ -282,6 +282,7 @@ int sipm_list_stage(Ctx *c, const crv_photon_list *ph, const crv_sipm_noise *nz,
   const int64_t n = ph->n, nt = nz->n_thermal;
   if (n < 0 || nt < 0) return set_error(c, CRV_E_ARG, "sipm: negative photon or thermal-label count");
   if (n && (!ph->label || !ph->time || !ph->photon)) return set_error(c, CRV_E_ARG, "sipm: NULL photon array");
+  if (n && !ph->j != !ph->rank) return set_error(c, CRV_E_ARG, "sipm: photon j and rank go together");
   if (nt && !nz->thermal_labels) return set_error(c, CRV_E_ARG, "sipm: NULL thermal_labels");
   auto finite_nonneg = [](double x) { return std::isfinite(x) && x >= 0.0; };
   if (!std::isfinite(nz->t_end)) return set_error(c, CRV_E_ARG, "sipm: t_end must be finite");
@@ -324,8 +325,7 @@ int sipm_list_stage(Ctx *c, const crv_photon_list *ph, const crv_sipm_noise *nz,
   Entries en{(int64_t *)ws(c, 600, E * 8), (double *)ws(c, 601, E * 8), (uint32_t *)ws(c, 602, E * 4),
              (uint32_t *)ws(c, 603, E * 4), (int16_t *)ws(c, 604, E * 2), (int64_t *)ws(c, 605, E * 8)};
   if (!en.label || !en.time || !en.j || !en.r || !en.pix || !en.photon) return ws_fail(c);
-  if (n)
-    k_list_entries<<<nblocks(n, 256), 256, 0, s>>>(ph->label, ph->time, ph->photon, n, nz->t_end, en, c->flags);
+  if (n) k_list_entries<<<nblocks(n, 256), 256, 0, s>>>(*ph, nz->t_end, en, c->flags);
   if (D) k_dark_fill<<<nblocks(nt, 256), 256, 0, s>>>(p, nz->thermal_labels, nt, nd, doff, n, nz->t_start, nz->t_end, en);
   CRV_HIP(c, hipGetLastError());
   // (label, time, entry) order: a stable sort by time, then a stable sort by label
@@ -358,7 +358,8 @@ int sipm_list_stage(Ctx *c, const crv_photon_list *ph, const crv_sipm_noise *nz,
   int hf[F_NFLAGS];
   CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
   CRV_HIP(c, hipStreamSynchronize(s));
-  if (hf[F_BADINPUT]) return set_error(c, CRV_E_ARG, "sipm: negative label or non-finite photon time");
+  if (hf[F_BADINPUT])
+    return set_error(c, CRV_E_ARG, "sipm: negative label, non-finite photon time, or a rank above 65535");
   if (hf[F_COUNTER]) return set_error(c, CRV_E_OVERFLOW, "sipm: more than 65535 dark discharges in one SiPM");
   out->capped = hf[F_CAPPED];
   const int64_t S = nr - (lastlab == (uint64_t)NO_LABEL ? 1 : 0);
