@@ -361,8 +361,9 @@ def finder_geometry(geom, counters_cfg):
 
 def clusters(W, cond, geom, counters_cfg, cfg, timers=None, detail=False, rng=None, caps=None):
     """Reco waveforms -> pulses -> coincidence clusters (reference chain.py:296-362 output, one dict in a list).
-    `rng` feeds only the reference's background-pulse model, which this chain has no setting for; `caps` (fixed
-    device shapes of the reference's JAX path) is refused."""
+    With cfg.background, background pulses drawn from `rng` for every event of W join the signal pulses in (event,
+    bar, SiPM, time) order before the finder (reference chain.py:337-342); `caps` (fixed device shapes of the
+    reference's JAX path) is refused."""
     timers = timers if timers is not None else {}
     CT.validate_chain(cfg)
     if caps is not None:
@@ -378,6 +379,12 @@ def clusters(W, cond, geom, counters_cfg, cfg, timers=None, detail=False, rng=No
     idx = np.arange(len(ev)) - np.searchsorted(ev, ev)
     bar, sipm = np.asarray(W["bar"])[wf], np.asarray(W["sipm"])[wf]
     time, pe, seq = P["time"][o], P["pe"][o], P["seq"][o]
+    from . import background as BG
+    smp = BG.sampler(cfg, rng if rng is not None else np.random.default_rng())
+    if smp is not None:
+        B = smp(np.unique(np.asarray(W["ev"])))
+        ev, idx, bar, sipm, time, pe, seq, _ = BG.merge_pulses(ev, idx, bar, sipm, time, pe, seq, B)
+        timers["background_pulses"] = timers.get("background_pulses", 0) + len(B["ev"])
     t0 = _time.perf_counter()
     out = find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G)
     timers["coinc_s"] = timers.get("coinc_s", 0.0) + _time.perf_counter() - t0
@@ -466,6 +473,8 @@ def run_chain(steps, cond, cfg, key=0, geom=None, batch=None, timers=None, recor
     records of the clusters with digis sorted by (label, FEB tick)."""
     import ctypes
     CT.validate_chain(cfg)
+    if cfg.background is not None:
+        raise ValueError("background: the one-call chain has no background pulses; run the stages (clusters) instead")
     G = geom if isinstance(geom, DT.SectorGeom) else finder_geometry(None, None) if geom is None else None
     if G is None:
         raise ValueError("geom: run_chain takes a SectorGeom or None (build one with finder_geometry)")
@@ -526,6 +535,8 @@ class ChainGraph:
     def __init__(self, sample, cond, cfg, key=0, geom=None, batch=None, max_steps=None, margin=0.25):
         import dataclasses
         CT.validate_chain(cfg)
+        if cfg.background is not None:
+            raise ValueError("background: the captured chain has no background pulses; run the stages instead")
         cfg = cfg if cfg.precision == "fp32" else dataclasses.replace(cfg, precision="fp32")
         self.cfg, self.cond = cfg, cond
         self.G = geom if geom is not None else DT.sector_geom()

@@ -137,6 +137,8 @@ class ChainConfig:
     feb_phase_key: str = "roc_feb"
     window_samples: int = 128
     photon_map: np.ndarray = field(default_factory=lambda: np.ones((40, 40)))
+    background: object = None         # crvhip.background.BackgroundModel: out-of-time pulses added before clustering
+    background_scale: float = 1.0     # multiplies the background model's net rate
     backend: str = "hip"
     precision: str = "fp64"
 

@@ -244,3 +244,33 @@ def test_empty_inputs():
     W = CH.reco_waveforms(D, cond)
     out, timers = CH.clusters(W, cond, G, None, cfg, detail=True)
     assert len(out[0]["cl_event"]) == 0 and timers["pulses"] == 0
+
+
+def test_clusters_with_background_pulses_match(setup):
+    """cfg.background: background pulses drawn for every event join the signal pulses in (event, bar, SiPM, time)
+    order before the finder (reference chain.py:337-342); the HIP finder on the merged table equals the oracle's."""
+    from dataclasses import replace
+    from crvhip import background as BG
+    cfg, cond, G, tmpl, S, Qo = setup
+    rng = np.random.default_rng(17)
+    sizes = rng.integers(1, 4, 400)
+    grp = np.repeat(np.arange(400), sizes)
+    n = len(grp)
+    bars = np.flatnonzero(np.asarray(G.sector) >= 0)
+    lib = {"group": grp, "bar": rng.choice(bars, n), "sipm": rng.integers(0, 4, n), "pe": rng.uniform(2, 40, n),
+           "dt": np.concatenate([np.r_[0.0, np.sort(rng.uniform(0, 25, k - 1))] for k in sizes]),
+           "seq": np.zeros(n, np.int64)}
+    M = BG.BackgroundModel.from_partials([dict(events=1000, exposure_ns=1000 * 1e5, groups=3000, cnt_sum=3000,
+                                               cnt_sumsq=3000 + 9000 + 4500, lib=lib)])
+    cfgb = replace(cfg, background=M, background_scale=1.0)
+    Wo = OC.reco_waveforms(OC.digis(Qo, cond, cfg, 5, tmpl), cond)
+    timers = {}
+    out, _ = CH.clusters(Wo, cond, G, None, cfgb, timers=timers, detail=True, rng=np.random.default_rng(9))
+    assert timers["background_pulses"] > 100
+    P = out[0]["pulses"]
+    ref = OC.find_clusters(P["ev"], P["bar"], P["sipm"], P["time"], P["pe"], P["seq"], cfg, cond, G)
+    _cluster_compare(out[0], ref)
+    # the same draws as the sampler on its own: the merged table holds the signal pulses plus exactly these
+    sig, _ = CH.clusters(Wo, cond, G, None, cfg, detail=True)
+    B = M.sample(np.unique(np.asarray(Wo["ev"])), cfg.window, np.random.default_rng(9))
+    assert len(P["ev"]) == len(sig[0]["pulses"]["ev"]) + len(B["ev"])
