@@ -197,7 +197,14 @@ def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, aut
         if rc not in (0, L.CRV_E_CAPACITY):
             ctx.check(rc)
         need_p, need_c, need_m = max(need_p, empty.P.n), max(need_c, empty.C.n), max(need_m, empty.C.n_mem)
-    out = Outputs(L, torch, need_p, max(need_c, need_m))
+    try:
+        out = Outputs(L, torch, need_p, max(need_c, need_m))
+    except torch.OutOfMemoryError:
+        # the library's cached workspace of the sizing pass holds the device: give it back (the warmup pass below
+        # allocates it again, outside the timed region)
+        ctx.trim()
+        torch.cuda.empty_cache()
+        out = Outputs(L, torch, need_p, max(need_c, need_m))
 
     def step():
         acc = {}
@@ -210,7 +217,7 @@ def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, aut
                 acc[f] = acc.get(f, 0) + getattr(tm, f)
         return acc
 
-    for _ in range(max(warmup - 1, 0)):
+    for _ in range(max(warmup, 1)):  # full passes into the caller's buffers after the sizing pass
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -306,6 +313,7 @@ def workload_leg(workload, n_events, chunk, seed, steps, warmup, batch, autotune
     ds = SG.device_steps(Sd)
     del Sd
     torch.cuda.synchronize()
+    torch.cuda.empty_cache()  # the generator's temporaries: cached by torch, they would starve the library's hipMalloc
     gen_s = time.perf_counter() - t0
     v, ms, avg, cnt, ex = measure("fp32", ds, ds.event, n_events, seed + 1, steps, warmup, batch, chunk, autotune,
                                   graph=graph)
@@ -409,6 +417,9 @@ def main():
         from crvhip import synth_gpu as SG
         Sd = SG.events(args.workload, args.events, args.seed)
         ds = SG.device_steps(Sd)
+        del Sd
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
         evkey = ds.event
     else:
         ds = DeviceSteps(S["evkey"], S["counter"], S["edep"], S["z"], S["t"])

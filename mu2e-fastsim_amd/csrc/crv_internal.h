@@ -162,8 +162,10 @@ struct Arena {
   std::vector<std::pair<void *, size_t>> blocks;
   size_t next = 0;
   bool replay = false;
+  size_t failed = 0;  // bytes of the last request that could not be met
   void *get(size_t bytes);
   void release();
+  std::string note() const;  // " (request, held, device free)" for allocation errors
 };
 
 // Capacities of a fixed-capacity batched call and its device-side counts. Every size a kernel or a library call needs

@@ -124,10 +124,19 @@ void *Arena::get(size_t bytes) {
   void *p = nullptr;
   if (hipMalloc(&p, bytes) != hipSuccess) {
     (void)hipGetLastError();
+    failed = bytes;
     return nullptr;
   }
   blocks.emplace_back(p, bytes);
   return p;
+}
+
+std::string Arena::note() const {
+  size_t held = 0, fr = 0, total = 0;
+  for (const auto &b : blocks) held += b.second;
+  (void)hipMemGetInfo(&fr, &total);
+  return " (request " + std::to_string(failed >> 20) + " MiB, the graph holds " + std::to_string(held >> 20) +
+         " MiB, device free " + std::to_string(fr >> 20) + " of " + std::to_string(total >> 20) + " MiB)";
 }
 
 void Arena::release() {
@@ -148,14 +157,14 @@ int scan_fixed(Ctx *c, Arena &ar, const int64_t *in, int64_t *out, int64_t n, hi
   size_t tb = 0;
   CRV_HIP(c, hipcub::DeviceScan::InclusiveSum(nullptr, tb, in, out + 1, (int)n, s));
   void *tmp = ar.get(tb);
-  if (!tmp) return set_error(c, CRV_E_HIP, "graph: scan buffer allocation failed");
+  if (!tmp) return set_error(c, CRV_E_HIP, "graph: scan buffer allocation failed" + ar.note());
   CRV_HIP(c, hipcub::DeviceScan::InclusiveSum(tmp, tb, in, out + 1, (int)n, s));
   return CRV_OK;
 }
 
 int widen_scan_fixed(Ctx *c, Arena &ar, const int32_t *in, int64_t *out, int64_t n, hipStream_t s) {
   int64_t *w = (int64_t *)ar.get((n + 1) * 8);
-  if (!w) return set_error(c, CRV_E_HIP, "graph: scan buffer allocation failed");
+  if (!w) return set_error(c, CRV_E_HIP, "graph: scan buffer allocation failed" + ar.note());
   if (n) k_i32_to_i64<<<nblocks(n, 256), 256, 0, s>>>(in, w, n);
   CRV_HIP(c, hipGetLastError());
   return scan_fixed(c, ar, w, out, n, s);
@@ -848,6 +857,9 @@ int crv_graph_create(crv_ctx *ctx, const crv_steps *sample, int64_t max_steps, c
   pl.tile = so.tile;
   pl.sort_tasks = so.sort_tasks;
   pl.grid_s = pl.grid_r = so.grid;
+  // the graph takes every buffer from its own arenas: the sizing run's workspace goes back to the device first, so
+  // the device never holds two copies of a call's buffers (an uncaptured fallback allocates it again)
+  if ((rc = crv_ctx_trim(ctx, nullptr))) return rc;
   auto fail = [&](int code, const std::string &m) {
     (void)crv_graph_destroy(G);
     return set_error(c, code, m);
@@ -891,7 +903,7 @@ int crv_graph_create(crv_ctx *ctx, const crv_steps *sample, int64_t max_steps, c
   for (auto q : dp) ok = ok && *q;
   for (auto q : bp) ok = ok && *q;
   for (auto q : cp) ok = ok && *q;
-  if (!ok) return fail(CRV_E_HIP, "graph: plan buffer allocation failed");
+  if (!ok) return fail(CRV_E_HIP, "graph: plan buffer allocation failed" + F.note());
   if (hipStreamCreateWithFlags(&G->cs, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&G->e0) != hipSuccess || hipEventCreate(&G->e1) != hipSuccess)
     return fail(CRV_E_HIP, "graph: stream or event creation failed");

@@ -2022,7 +2022,7 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
                   o.yscr, o.npr, o.rmap, o.slot, npr64, poff, pbar, psipm, o.run_c, o.cd0, o.ncls, o.cdesc, o.ncr,
                   o.crl};
   for (void *q : ptrs)
-    if (!q) return set_error(c, CRV_E_HIP, "graph: buffer allocation failed");
+    if (!q) return set_error(c, CRV_E_HIP, "graph: buffer allocation failed" + ar.note());
   o.rec_count = counters + C_REC;
   o.run_count = counters + C_RUN;
   o.cdesc_count = counters + C_DESC;
@@ -2044,7 +2044,7 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
     size_t tb = 0;
     CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, skey, skey2, sidx, order, (int)L4, 0, sbits, s));
     void *tmp = A(tb);
-    if (!tmp) return set_error(c, CRV_E_HIP, "graph: sort buffer allocation failed");
+    if (!tmp) return set_error(c, CRV_E_HIP, "graph: sort buffer allocation failed" + ar.note());
     CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, skey, skey2, sidx, order, (int)L4, 0, sbits, s));
   }
   const int32_t *qorder = sorted ? order : nullptr;
@@ -2071,7 +2071,7 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
     size_t tb = 0;
     CRV_HIP(c, hipcub::DeviceReduce::Sum(nullptr, tb, fired64, sums, (int)L4, s));
     void *tmp = A(tb);
-    if (!tmp) return set_error(c, CRV_E_HIP, "graph: reduce buffer allocation failed");
+    if (!tmp) return set_error(c, CRV_E_HIP, "graph: reduce buffer allocation failed" + ar.note());
     CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, fired64, sums, (int)L4, s));
     CRV_HIP(c, hipcub::DeviceReduce::Sum(tmp, tb, sipm_nph, sums + 1, (int)L4, s));
   }

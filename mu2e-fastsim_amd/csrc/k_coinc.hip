@@ -2251,7 +2251,7 @@ int clusters_fixed(Ctx *c, Plan &pl, Arena &ar, const int64_t *ev, const int32_t
   for (auto q : bd) ok = ok && *q;
   for (auto q : bi) ok = ok && *q;
   for (auto q : bu) ok = ok && *q;
-  if (!ok) return set_error(c, CRV_E_HIP, "graph: finder buffer allocation failed");
+  if (!ok) return set_error(c, CRV_E_HIP, "graph: finder buffer allocation failed" + ar.note());
   size_t tb = 0;
   CRV_HIP(c, hipcub::DeviceRunLengthEncode::Encode(nullptr, tb, ev, evk, cnt, nruns, (int)n, s));
   void *tmp = A(tb);

@@ -387,7 +387,7 @@ int group_steps_fixed(Ctx *c, Plan &pl, Arena &ar, StepGroups &g, hipStream_t s)
   g.task_off = (int32_t *)ar.get((n + 1) * 4);
   int32_t *nruns = (int32_t *)ar.get(8);
   if (!key || !key_s || !idx || !g.perm || !g.evfirst || !g.task_key || !cnt || !g.task_off || !nruns)
-    return set_error(c, CRV_E_HIP, "graph: step grouping allocation failed");
+    return set_error(c, CRV_E_HIP, "graph: step grouping allocation failed" + ar.note());
   k_zero_flags<<<1, 64, 0, s>>>(c->flags, F_NFLAGS);
   k_step_keys_fixed<<<nblocks(n, 256), 256, 0, s>>>(pl.in_event, pl.in_counter, pl.n_dev, n, key, idx, g.evfirst,
                                                    c->flags);
@@ -395,19 +395,19 @@ int group_steps_fixed(Ctx *c, Plan &pl, Arena &ar, StepGroups &g, hipStream_t s)
   size_t tb = 0;
   CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key_s, idx, g.perm, (int)n, 0, 46, s));
   void *tmp = ar.get(tb);
-  if (!tmp) return set_error(c, CRV_E_HIP, "graph: sort buffer allocation failed");
+  if (!tmp) return set_error(c, CRV_E_HIP, "graph: sort buffer allocation failed" + ar.note());
   CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, key_s, idx, g.perm, (int)n, 0, 46, s));
   size_t tb2 = 0;
   CRV_HIP(c, hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, key_s, g.task_key, cnt, nruns, (int)n, s));
   void *tmp2 = ar.get(tb2);
-  if (!tmp2) return set_error(c, CRV_E_HIP, "graph: run-length buffer allocation failed");
+  if (!tmp2) return set_error(c, CRV_E_HIP, "graph: run-length buffer allocation failed" + ar.note());
   CRV_HIP(c, hipcub::DeviceRunLengthEncode::Encode(tmp2, tb2, key_s, g.task_key, cnt, nruns, (int)n, s));
   k_fix_tasks<<<nblocks(n + 1, 256), 256, 0, s>>>(nruns, pl.n_dev, n, pl.T_dev, cnt);
   CRV_HIP(c, hipGetLastError());
   size_t tb3 = 0;
   CRV_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, cnt, g.task_off, (int)(n + 1), s));
   void *tmp3 = ar.get(tb3);
-  if (!tmp3) return set_error(c, CRV_E_HIP, "graph: scan buffer allocation failed");
+  if (!tmp3) return set_error(c, CRV_E_HIP, "graph: scan buffer allocation failed" + ar.note());
   CRV_HIP(c, hipcub::DeviceScan::ExclusiveSum(tmp3, tb3, cnt, g.task_off, (int)(n + 1), s));
   return CRV_OK;
 }
