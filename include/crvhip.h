@@ -133,6 +133,57 @@ typedef struct crv_photons {
 int crv_photons_run(crv_ctx *ctx, const crv_steps *steps, const double *shift, double period, crv_photons *out,
                     crv_stream stream);
 
+/* Stock lookup-table optics (reference response/lookup.py:65-96 LookupTable, read from the v6 file by
+ * crvhip.lookup.load): one table's geometry constants, Cerenkov maps, bin edges and, for each of its three tables
+ * (scintillation in the scintillator, Cerenkov in the scintillator, Cerenkov in the fibre), the per-bin arrival
+ * probability and the delay weights padded to dmax per bin; inv_speed / atten_length as
+ * response/photons.py:230-254 table_propagation measures them. Host arrays; the library keeps device copies. */
+typedef struct crv_lookup_table {
+  double half_thickness, half_width, half_length, fiber_separation, hole_radius_x, hole_radius_y, fiber_radius,
+      corner_radius;
+  double inv_speed, atten_length;
+  const double *edges[7]; /* x, y, z, beta, theta, phi, r */
+  int32_t n_edges[7];
+  const double *ck[2];    /* [n, 2] (beta, photons per mm): scintillator, fibre */
+  int32_t n_ck[2];
+  const float *prob[3];   /* [bins] */
+  const uint8_t *delays[3]; /* [bins * dmax] */
+  int32_t dmax[3];
+} crv_lookup_table;
+
+/* Per bar (response/photons.py:305-356 iter_rows inputs): the table index of its sector (-1: not simulated), the
+ * sector's reflector (0: both ends read; otherwise SiPMs 0 and 2 only, reflector < 0 writing SiPM s to channel
+ * s + 1), its yield (photons per MeV), the propagation ratios of ResponseConfig.speed_ratio / atten_ratio, its
+ * centre (mm) and local_to_world (the world axes of thickness, width, length). */
+typedef struct crv_lookup_bars {
+  int32_t nbars;
+  const int32_t *table, *reflector;
+  const double *yield, *speed_ratio, *atten_ratio;
+  const double *center;        /* [nbars * 3] */
+  const int32_t *local_to_world; /* [nbars * 3] */
+} crv_lookup_bars;
+
+/* Upload 1-16 tables and the bar arrays (replaces any earlier upload on the context). */
+int crv_lookup_upload(crv_ctx *ctx, const crv_lookup_table *tables, int32_t n_tables, const crv_lookup_bars *bars);
+
+/* CrvSteps for the lookup optics: SoA device arrays sorted by event (reference CrvStep columns, README.md:30-40). */
+typedef struct crv_lookup_steps {
+  const uint32_t *event;        /* event key low 32 bits (the upper 32 in crv_params.evkey_hi) */
+  const int32_t *bar;
+  const double *t0, *t1;
+  const double *pos0, *pos1;    /* [n * 3] world (x, y, z) of the step's start and end, mm */
+  const double *edep, *len;     /* MeV, mm */
+  const double *p0, *p1;        /* momentum magnitude at the start and end, MeV */
+  const double *mass, *charge;  /* particle mass (MeV) and charge (e) of the step's pdg (photons.py:65-77) */
+  int64_t n;
+} crv_lookup_steps;
+
+/* Replaces response/photons.py:359-371 make_photons (the substep loop of :163-224) with the status mask of
+ * chain.py:199-204: the photoelectrons of the steps in (step, SiPM, rank) order, a row's photons ranked substep by
+ * substep, then by type, then by arrival; photon = the step's row in the call, (j, rank) its draw identity for
+ * crv_sipm_run. FP64; CRV_E_CAPACITY with out->n set when the output is too small. */
+int crv_lookup_photons_run(crv_ctx *ctx, const crv_lookup_steps *steps, crv_photons *out, crv_stream stream);
+
 /* Dark noise and trap afterpulses (reference response/sipm.py:26-47 SiPMConfig thermal_rate, trap*_prob and
  * trap*_lifetime; simulate's t_start, t_end and thermal_chans, sipm.py:102-121). */
 typedef struct crv_sipm_noise {

@@ -56,6 +56,23 @@ class Photons(C.Structure):
                 ("capped", i64)]
 
 
+class LookupTableC(C.Structure):
+    _fields_ = [(k, f64) for k in ("half_thickness", "half_width", "half_length", "fiber_separation", "hole_radius_x",
+                                   "hole_radius_y", "fiber_radius", "corner_radius", "inv_speed", "atten_length")] + \
+        [("edges", P * 7), ("n_edges", i32 * 7), ("ck", P * 2), ("n_ck", i32 * 2), ("prob", P * 3), ("delays", P * 3),
+         ("dmax", i32 * 3)]
+
+
+class LookupBars(C.Structure):
+    _fields_ = [("nbars", i32), ("table", P), ("reflector", P), ("yield_", P), ("speed_ratio", P),
+                ("atten_ratio", P), ("center", P), ("local_to_world", P)]
+
+
+class LookupSteps(C.Structure):
+    _fields_ = [("event", P), ("bar", P), ("t0", P), ("t1", P), ("pos0", P), ("pos1", P), ("edep", P), ("len", P),
+                ("p0", P), ("p1", P), ("mass", P), ("charge", P), ("n", i64)]
+
+
 class SipmNoise(C.Structure):
     _fields_ = [(k, f64) for k in ("thermal_rate", "trap0_prob", "trap1_prob", "trap0_lifetime", "trap1_lifetime",
                                    "t_start", "t_end")] + [("thermal_labels", P), ("n_thermal", i64)]
@@ -112,7 +129,7 @@ EXPORTS = ("crv_ctx_create", "crv_ctx_destroy", "crv_set_params", "crv_last_erro
            "crv_reco_waveforms_run", "crv_pulses_run", "crv_clusters_run", "crv_chain_run", "crv_chain_fetch",
            "crv_chain_records", "crv_chain_run_batched", "crv_ctx_workspace", "crv_ctx_trim",
            "crv_pulses_run_noalloc", "crv_graph_create", "crv_graph_launch", "crv_graph_fetch", "crv_graph_stats",
-           "crv_graph_destroy", "crv_sipm_run", "crv_photons_run")
+           "crv_graph_destroy", "crv_sipm_run", "crv_photons_run", "crv_lookup_upload", "crv_lookup_photons_run")
 
 _lib = None
 
@@ -136,6 +153,8 @@ def load(path=LIB_PATH):
     lib.crv_charges_run.argtypes = [P, C.POINTER(Steps), C.POINTER(Charges), P]
     lib.crv_sipm_run.argtypes = [P, C.POINTER(PhotonList), C.POINTER(SipmNoise), C.POINTER(Charges), P]
     lib.crv_photons_run.argtypes = [P, C.POINTER(Steps), P, f64, C.POINTER(Photons), P]
+    lib.crv_lookup_upload.argtypes = [P, C.POINTER(LookupTableC), i32, C.POINTER(LookupBars)]
+    lib.crv_lookup_photons_run.argtypes = [P, C.POINTER(LookupSteps), C.POINTER(Photons), P]
     lib.crv_digis_run.argtypes = [P, P, P, P, i64, C.POINTER(Digis), P]
     lib.crv_reco_waveforms_run.argtypes = [P, P, P, P, i64, C.POINTER(Waveforms), P]
     lib.crv_pulses_run.argtypes = [P] + [P] * 6 + [i64] + [P] * 4 + [C.POINTER(Pulses), P]

@@ -161,16 +161,18 @@ class DeviceSteps:
 def charges(steps, parts, geom, cond, cfg, rng, key=None, tabs=None, pmap=None, timers=None, pdg=None, caps=None):
     """CrvSteps -> SiPM charges: dict chan (label), time, charge, pe, photon (step row), sorted by (label, time).
 
-    The closed-form optics of cfg.optics replace the reference's lookup tables and Cerenkov yield, so `tabs` and
-    `pdg` must be None (`parts` only feeds the reference's pdg lookup and is not read). `pmap` replaces
-    cfg.photon_map for this call. `geom`: None or a SectorGeom (synthetic detector), or a CRVGeometry whose counters
-    all have the optics' length (steps placed by their world coordinates)."""
+    Without `tabs` the closed-form optics of cfg.optics apply: `pdg` must be None (`parts` is not read) and `geom` is
+    None or a SectorGeom (synthetic detector), or a CRVGeometry whose counters all have the optics' length (steps
+    placed by their world coordinates). With `tabs` ({name: crvhip.lookup.LookupTable}) the stock lookup-table optics
+    apply (reference response/photons.py, the photons of crvhip.photons on the device): `geom` must be a CRVGeometry,
+    the steps carry the CrvStep columns (bar, edep, t0, t1, x0..z1, px0, py0, pz0, p1, len) and the particle of each
+    step comes from `pdg` or from `parts` (joined on the step's sp). `pmap` replaces cfg.photon_map for this call."""
     timers = timers if timers is not None else {}
     CT.validate_chain(cfg)
     if caps is not None:
         raise ValueError("caps: capacities need the batched chain")
     if tabs is not None:
-        raise ValueError("tabs: lookup-table optics are not supported; the closed-form cfg.optics apply")
+        return _charges_lookup(steps, parts, geom, cond, cfg, rng, key, tabs, pmap, timers, pdg)
     if pdg is not None:
         raise ValueError("pdg: the closed-form optics have no particle-dependent (Cerenkov) yield; pass pdg=None")
     _check_response_geometry(geom, cfg)
@@ -203,6 +205,88 @@ def charges(steps, parts, geom, cond, cfg, rng, key=None, tabs=None, pmap=None, 
     timers["charges"] = timers.get("charges", 0) + n
     if out.capped:
         timers["poisson_capped"] = timers.get("poisson_capped", 0) + int(out.capped)
+    return Q
+
+
+def _pdg_of(steps, parts, pdg):
+    """pdg per step: given, or joined from the particle table on (run, subrun, event, sp) = (..., key), 0 if absent
+    (reference chain.py:162-164)."""
+    n = len(np.asarray(steps["bar"]))
+    if pdg is not None:
+        pdg = np.asarray(pdg)
+        if pdg.shape != (n,) or (n and pdg.dtype.kind not in "iu"):
+            raise ValueError("pdg must be an integer array aligned with the steps")
+        return pdg.astype(np.int64)
+    if parts is None:
+        raise ValueError("the lookup optics need the particle of each step: pass pdg or parts")
+    import pandas as pd
+    keys = [k for k in ("run", "subrun", "event") if k in steps and k in parts]
+    st = pd.DataFrame({k: np.asarray(steps[k]) for k in keys + ["sp"]})
+    pt = pd.DataFrame({**{k: np.asarray(parts[k]) for k in keys}, "sp": np.asarray(parts["key"]),
+                       "pdg": np.asarray(parts["pdg"])})
+    return st.merge(pt, on=keys + ["sp"], how="left").pdg.fillna(0).to_numpy().astype(np.int64)
+
+
+def _charges_lookup(steps, parts, geom, cond, cfg, rng, key, tabs, pmap, timers, pdg):
+    """charges() through the lookup-table optics: crv_lookup_photons_run, then crv_sipm_run without dark noise or
+    trap afterpulses (validate_chain refuses both, as the reference's contract does), photons after cfg.window[1]
+    dropped (reference chain.py:193-213)."""
+    from . import photons as PH
+    if not isinstance(tabs, dict) or not tabs or not all(hasattr(T, "tables") and hasattr(T, "edges")
+                                                         for T in tabs.values()):
+        raise ValueError("tabs: expected a non-empty dict {name: crvhip.lookup.LookupTable}")
+    if not DT.is_crv_geometry(geom):
+        raise ValueError("geom: the lookup-table optics need a CRVGeometry (bar centres and axes)")
+    pdg = _pdg_of(steps, parts, pdg)
+    ev = event_key(steps)
+    bar = np.asarray(steps["bar"], np.int64)
+    nb = len(np.asarray(geom.sector))
+    if len(bar) and (bar.min() < 0 or bar.max() >= nb):
+        raise ValueError(f"step bar outside the geometry's {nb} bars")
+    if nb * 4 > CHAN_MULT:
+        raise ValueError(f"channel {nb * 4 - 1} >= CHAN_MULT {CHAN_MULT}: SiPM labels would overlap the next event")
+    ctx = context(cfg, cond)
+    ctx.use_photon_map(cfg, pmap)
+    PH.upload(ctx, tabs, geom, cfg.response)
+    ls = PH.LookupSteps(ev, steps, pdg)
+    ctx.set_params(cfg, _seed(key, rng), ls.evkey_hi)
+    t0 = _time.perf_counter()
+    torch = _torch()
+    empty = {k: np.zeros(0, np.int64 if k in ("chan", "photon") else np.float64)
+             for k in ("chan", "time", "charge", "pe", "photon")}
+    if ls.n == 0:
+        return empty
+    pb, po = PH.device_photons(ctx, ls)
+    nph = po.n
+    timers["photons_s"] = timers.get("photons_s", 0.0) + _time.perf_counter() - t0
+    timers["photons"] = timers.get("photons", 0) + int(nph)
+    if po.capped:
+        timers["poisson_capped"] = timers.get("poisson_capped", 0) + int(po.capped)
+    if nph == 0:
+        return empty
+    t0 = _time.perf_counter()
+    s = cfg.sipm
+    pl = L.PhotonList(L.ptr(pb["label"]), L.ptr(pb["time"]), L.ptr(pb["photon"]), L.ptr(pb["j"]), L.ptr(pb["rank"]),
+                      nph)
+    nz = L.SipmNoise(0.0, float(s.trap0_prob), float(s.trap1_prob), float(s.trap0_lifetime), float(s.trap1_lifetime),
+                     0.0, float(cfg.window[1]), None, 0)
+    cap = 2 * nph + 4096
+    for _ in range(2):
+        qb = {k: _empty(cap, dt) for k, dt in (("label", torch.int64), ("time", torch.float64),
+                                               ("charge", torch.float64), ("pe", torch.float64),
+                                               ("photon", torch.int64))}
+        qo = L.Charges(*[L.ptr(qb[k]) for k in ("label", "time", "charge", "pe", "photon")], cap, 0, 0, 0)
+        rc = ctx.lib.crv_sipm_run(ctx.h, C_ref(pl), C_ref(nz), C_ref(qo), None)
+        if rc == L.CRV_E_CAPACITY:
+            cap = qo.n
+            continue
+        ctx.check(rc)
+        break
+    m = qo.n
+    Q = {"chan": _host(qb["label"], m), "time": _host(qb["time"], m), "charge": _host(qb["charge"], m),
+         "pe": _host(qb["pe"], m), "photon": ls.order[_host(qb["photon"], m)] if m else np.zeros(0, np.int64)}
+    timers["sipm_s"] = timers.get("sipm_s", 0.0) + _time.perf_counter() - t0
+    timers["charges"] = timers.get("charges", 0) + m
     return Q
 
 

@@ -104,9 +104,29 @@ class FinderConfig:
 
 
 @dataclass(frozen=True)
+class SectorConfig:
+    """Lookup-table optics of one sector (reference response/photons.py:31-35): table name, reflector, photons/MeV."""
+    table: str
+    reflector: int
+    yield_per_mev: float
+
+
+# the stock extracted-detector sectors (reference response/photons.py:38-43, Offline epilog_extracted_v05.fcl)
+EXTRACTED_V05 = {"EX": SectorConfig("6000_0", 0, 39794.0), "T1": SectorConfig("2370_1", -1, 28610.0),
+                 "T2": SectorConfig("3200_0", 0, 28610.0),
+                 **{f"M{i}": SectorConfig("2370_1", 1, 28610.0) for i in range(1, 9)}}
+
+
+@dataclass(frozen=True)
 class ResponseConfig:
+    """photon_yield_scale multiplies every arrival probability (the reference's default is 0.94; the closed-form optics
+    of this build are calibrated at 1.0). sectors, speed_ratio and atten_ratio configure the lookup-table optics
+    (charges(..., tabs=...)); the closed-form optics ignore them."""
     photon_yield_scale: float = 1.0
     photon_yield_variation_scale: float = 1.0
+    sectors: dict = field(default_factory=lambda: dict(EXTRACTED_V05))
+    speed_ratio: dict = field(default_factory=dict)
+    atten_ratio: dict = field(default_factory=dict)
 
 
 @dataclass(frozen=True)

@@ -60,6 +60,25 @@ struct Tables {
   const uint8_t *tside0 = nullptr, *tside1 = nullptr;
 };
 
+// A v6 lookup table on the device (crv_lookup_upload): constants, edges, Cerenkov maps, per-bin probabilities and
+// cumulative delay weights (int32 [bins * dmax]).
+struct LkTab {
+  double ht, hw, hl, fs, hrx, hry, fr, cr, inv_v, att;
+  const double *edges[7];
+  int32_t ne[7], nb[7];
+  const double *ck[2];
+  int32_t nck[2];
+  const float *prob[3];
+  const int32_t *cum[3], *dsum[3];
+  int32_t dmax[3];
+};
+struct LkBars {
+  int32_t nbars = 0;
+  const int32_t *table = nullptr, *refl = nullptr;
+  const double *yield = nullptr, *speed = nullptr, *atten = nullptr, *center = nullptr;
+  const int32_t *l2w = nullptr;
+};
+
 // Derived scalars of the parameters (computed on the host with the same portable functions as the device).
 struct Derived {
   double yscale;  // pe_per_mev / A(ref_distance)
@@ -96,6 +115,11 @@ struct Ctx {
   int ws_fail_slot = -1;                  // the workspace request that failed last
   size_t ws_fail_bytes = 0;
   uint8_t *scratch = nullptr;  // this device's fixed scratch array of the allocation-free pulse entry point
+  // lookup-table optics (crv_lookup_upload)
+  std::vector<void *> lk_owned;
+  const LkTab *lk_tabs = nullptr;
+  int32_t lk_ntab = 0;
+  LkBars lk_bars{};
 };
 
 // Workspace slots: each stage takes fixed slots below WS_STAGE_SLOTS (charges 0-25, digis 30-55, pulses 60-81,
@@ -135,6 +159,10 @@ int group_steps(Ctx *c, const crv_steps *st, StepGroups &g, hipStream_t s);
 int charges_stage(Ctx *c, StepGroups &g, crv_charges *out, hipStream_t s);
 // the photoelectrons of the steps as a list (crv_photons_run), optionally shifted per event and folded by a period
 int photons_stage(Ctx *c, StepGroups &g, const double *shift, double period, crv_photons *out, hipStream_t s);
+// lookup-table optics (k_lookup.hip)
+int lookup_upload(Ctx *c, const crv_lookup_table *tabs, int32_t ntab, const crv_lookup_bars *bars);
+void lookup_free(Ctx *c);
+int lookup_photons_stage(Ctx *c, const crv_lookup_steps *st, crv_photons *out, hipStream_t s);
 // photon list + dark noise -> charges with cross-talk and trap afterpulses (crv_sipm_run)
 int sipm_list_stage(Ctx *c, const crv_photon_list *ph, const crv_sipm_noise *nz, crv_charges *out, hipStream_t s);
 int digis_stage(Ctx *c, const int64_t *label, const double *time, const double *charge, int64_t n, crv_digis *out,

@@ -393,6 +393,7 @@ int crv_ctx_destroy(crv_ctx *ctx) {
   for (void *p : ctx->c.ws)
     if (p) (void)hipFree(p);
   for (void *p : ctx->c.owned) (void)hipFree(p);
+  lookup_free(&ctx->c);
   if (ctx->c.flags) (void)hipFree(ctx->c.flags);
   for (auto &e : ctx->c.ev) (void)hipEventDestroy(e);
   for (auto &e : ctx->c.kev) (void)hipEventDestroy(e);
@@ -599,6 +600,20 @@ int crv_photons_run(crv_ctx *ctx, const crv_steps *steps, const double *shift, d
   int rc = group_steps(c, steps, g, s);
   if (rc) return rc;
   return photons_stage(c, g, shift, period, out, s);
+}
+
+int crv_lookup_upload(crv_ctx *ctx, const crv_lookup_table *tables, int32_t n_tables, const crv_lookup_bars *bars) {
+  if (int rc = guard(ctx)) return rc;
+  return lookup_upload(&ctx->c, tables, n_tables, bars);
+}
+
+int crv_lookup_photons_run(crv_ctx *ctx, const crv_lookup_steps *steps, crv_photons *out, crv_stream stream) {
+  if (int rc = guard(ctx)) return rc;
+  Ctx *c = &ctx->c;
+  if (!steps || !out) return set_error(c, CRV_E_ARG, "NULL argument");
+  if (!c->have_params) return set_error(c, CRV_E_STATE, "parameters not set");
+  if (!c->t.status) return set_error(c, CRV_E_STATE, "conditions not uploaded");
+  return lookup_photons_stage(c, steps, out, (hipStream_t)stream);
 }
 
 int crv_sipm_run(crv_ctx *ctx, const crv_photon_list *photons, const crv_sipm_noise *noise, crv_charges *out,

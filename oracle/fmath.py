@@ -11,6 +11,7 @@ sequence of IEEE-754 double operations (no fused multiply-add), so both give ide
 * sin / cos: reduction by pi/2 in three 30-bit parts (Cody-Waite), Taylor series on |r| <= pi/4, quadrant swap.
 * asin: the Maclaurin series of asin to z^24 (z = x^2 <= 1/4) on |x| <= 1/2, asin x = pi/2 - 2 asin(sqrt((1-x)/2))
   above.
+* acos: pi/2 - asin x on |x| <= 1/2; 2 asin(sqrt((1-|x|)/2)) above (pi minus it for x < 0), the same series.
 
 Each function works elementwise on numpy arrays with one numpy ufunc per IEEE operation (numpy never contracts
 a*b+c), so the op order is the one written here. The coefficients are rationals rounded once to double
@@ -126,4 +127,15 @@ def asin(x):
     hi = PI_2 - 2.0 * (s * _horner(ASIN_C, w))
     r = np.where(a <= 0.5, lo, hi)
     r = np.where(x < 0, -r, r)
+    return np.where(a > 1.0, np.nan, r)
+
+
+def acos(x):
+    x = np.asarray(x, np.float64)
+    a = np.abs(x)
+    mid = PI_2 - x * _horner(ASIN_C, x * x)
+    w = (1.0 - a) * 0.5
+    s = np.sqrt(np.maximum(w, 0.0))
+    v = 2.0 * (s * _horner(ASIN_C, w))
+    r = np.where(a <= 0.5, mid, np.where(x > 0, v, (2.0 * PI_2) - v))
     return np.where(a > 1.0, np.nan, r)

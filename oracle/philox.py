@@ -27,6 +27,9 @@ KIND_THERMAL_N = 8  # dark-noise count per SiPM (Poisson, the photon-count sampl
 KIND_THERMAL = 9    # dark discharge k of a SiPM: time, pixel x, pixel y
 KIND_TRAP = 10      # trap capture of a firing: trap 0 test and lifetime, trap 1 test and lifetime
 KIND_AFTER = 11     # fire decision of a trap afterpulse: trap 0, trap 1
+KIND_LCOUNT = 12    # lookup optics: photon number of one type per (step, SiPM) (Poisson or Gauss)
+KIND_LBINOM = 13    # lookup optics: Binomial arrivals per substep, one uniform per type and inversion chunk
+KIND_LDELAY = 14    # lookup optics: delay of each arriving photon, four per call
 DARK_J = 0xFFFFFFFF  # step ordinal of dark discharges in the photon identity (photons have smaller ones)
 
 

@@ -136,3 +136,57 @@ def ordered_delays(n, tau, uniforms_of):
 
 def normal(u1, u2):
     return np.sqrt(-2.0 * F.log(u1)) * F.cos(F.TWO_PI * u2)
+
+
+BINOM_CHUNK_MEAN = 16.0  # largest -size * log(1 - q) of one inversion chunk: P(0) >= e^-16, no underflow
+BINOM_KMAX = 1024        # inversion search bound per chunk (the chunk mean is <= 16: never reached in double)
+BINOM_MAX_CHUNKS = 1 << 16  # chunks per draw the draw layout holds (a mean of up to ~1M arrivals)
+
+
+def binomial_chunk(n, q):
+    """Chunk size of the Binomial(n, q) inversion, q <= 1/2: the largest m with m * -log(1 - q) <= 16 (at least 1,
+    at most n)."""
+    lq = F.log(1.0 - q)
+    lim = BINOM_CHUNK_MEAN / -lq
+    return np.where(lim >= n.astype(np.float64), n, np.maximum(np.floor(lim), 1.0).astype(np.int64)), lq
+
+
+def binomial(n, p, uniform_of):
+    """Binomial(n, p) draws (arrays; p in [0, 1]). uniform_of(chunk) -> uniform array [len(n)] of each draw's chunk.
+
+    Inversion by sequential search (P(0) = (1 - q)^m, P(k + 1) = P(k) * q / (1 - q) * (m - k) / (k + 1)) on chunks of
+    m trials whose mean is at most 16, summed over the chunks of n (a sum of Binomial(m_i, q) is Binomial(sum m_i,
+    q)); q = min(p, 1 - p) and k -> n - k when p > 1/2. Returns (k int64, ok bool): ok is False when a draw needs
+    more chunks than the draw layout holds."""
+    n = np.asarray(n, np.int64)
+    p = np.asarray(p, np.float64)
+    flip = p > 0.5
+    q = np.where(flip, 1.0 - p, p)
+    k = np.zeros(len(n), np.int64)
+    ok = np.ones(len(n), bool)
+    act = np.flatnonzero((n > 0) & (q > 0.0))
+    if len(act):
+        na, qa = n[act], q[act]
+        m, lq = binomial_chunk(na, qa)
+        nch = (na + m - 1) // m
+        ok[act] = nch <= BINOM_MAX_CHUNKS
+        r = qa / (1.0 - qa)
+        tot = np.zeros(len(act), np.int64)
+        for c in range(int(nch.max())):
+            g = np.flatnonzero(nch > c)
+            size = np.minimum(m[g], na[g] - c * m[g])
+            u = uniform_of(c)[act[g]]
+            f = F.exp(size.astype(np.float64) * lq[g])
+            cdf = f.copy()
+            kk = np.zeros(len(g), np.int64)
+            go = (u > cdf) & (kk < size)
+            while go.any():
+                h = np.flatnonzero(go)
+                f[h] = ((f[h] * r[g[h]]) * (size[h] - kk[h]).astype(np.float64)) / (kk[h] + 1).astype(np.float64)
+                kk[h] += 1
+                cdf[h] = cdf[h] + f[h]
+                go = (u > cdf) & (kk < size) & (kk < BINOM_KMAX)
+            tot[g] += kk
+        k[act] = np.where(flip[act], na - tot, tot)
+    k = np.where(flip & (q == 0.0), n, k)  # p == 1: every trial
+    return k, ok
