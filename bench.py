@@ -37,7 +37,7 @@ SAMPLE_B = 2           # one ADC word read by the pulse kernel
 WAVEFORM_META_B = 40   # start, length, offset, calibration of a reco waveform
 PULSE_IN_B = 40        # event, bar, sipm, time, pe, seq read by the finder
 # FP32 batched chain kernels
-SIPM_STEP_B = 20       # per (step, SiPM) read by kb_sipm: photon count i32, z f64, t f64
+SIPM_STEP_B = 32       # per step read by kb_sipm: z, t f64 and the photon counts of its 4 SiPMs i32
 FIRE_B = 8             # fired-pixel log entry written by kb_sipm: time offset f32, overvoltage f32
 FIRE_IN_B = 8          # log entry read by the waveform kernel (kb_wrec): time offset, overvoltage
 RUN_B = 32             # reco-waveform descriptor: record, first sample, samples, run number i32; SiPM, start TDC i64
@@ -288,7 +288,14 @@ def kernels_fp32(avg, cnt, fires):
 # HBM bytes per event actually moved by a kernel on the cosmic workload: rocprofv3 FETCH_SIZE + WRITE_SIZE passes
 # over 1M events, each scaled by the factor that makes a plain 1 GiB device copy read and write 1 GiB
 # (profiles/chain_fp32_cosmic1m_traffic.txt); used for roofline.traffic of the same workload only
-PMC_BYTES_PER_EVENT = {"cosmic": {"kb_sipm": 88487.6, "kb_wave": 37401.1, "kb_fit": 24707.7}}
+PMC_BYTES_PER_EVENT = {"cosmic": {"kb_sipm": 91055.4, "kb_wrec": 36030.6, "kb_fit": 24895.2}}
+# VALU issue and active lanes of the chain's kernels on the cosmic workload, from rocprofv3 SQ counters over 1M events
+# (profiles/chain_fp32_cosmic1m_valu.txt): busy = 4 SQ_ACTIVE_INST_VALU / 1024 SIMDs / GRBM_GUI_ACTIVE cycles per XCD,
+# lanes = SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)
+PMC_VALU = {"cosmic": {"kb_sipm": {"busy": 0.754, "lanes": 0.346, "valu_per_event": 33982},
+                       "kb_wrec": {"busy": 0.861, "lanes": 0.496, "valu_per_event": 12772},
+                       "kb_fit": {"busy": 1.087, "lanes": 0.663, "valu_per_event": 10223},
+                       "k_coinc_wave<64>": {"busy": 0.554, "lanes": 0.538, "valu_per_event": 7183}}}
 
 
 def roofline(kernels, workload=None, events=0, per_call=1):
@@ -298,9 +305,13 @@ def roofline(kernels, workload=None, events=0, per_call=1):
     kms, kbytes = kernels[dom]
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     per_event = PMC_BYTES_PER_EVENT.get(workload, {}).get(dom)
-    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": per_event * events if per_event else None,
-            "bytes_per_launch": kbytes / per_call, "ms_per_launch": kms / per_call}
+    out = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": per_event * events if per_event else None,
+           "bytes_per_launch": kbytes / per_call, "ms_per_launch": kms / per_call}
+    valu = PMC_VALU.get(workload, {}).get(dom)
+    if valu:  # the kernel is issue-bound: its VALU issue and active-lane fraction from the counter profile
+        out["valu"] = {**valu, "source": "rocprofv3 SQ counters, same workload (profiles/chain_fp32_cosmic1m_valu.txt)"}
+    return out
 
 
 def workload_leg(workload, n_events, chunk, seed, steps, warmup, batch, autotune, graph=False):
