@@ -314,6 +314,83 @@ def roofline(kernels, workload=None, events=0, per_call=1):
     return out
 
 
+LK_STEP_B = 128        # per step read by the lookup optics: event, bar i32; t0, t1, edep, len, p0, p1, mass, charge,
+#                        start and end positions f64
+PHOTON_B = 32          # per photon written by the lookup optics (label, time, step row f64/i64; step ordinal, rank i32)
+SIPM_CHARGE_B = 40     # per charge written by the SiPM stage (label, time, charge, pe, photon)
+
+
+def lookup_leg(n_events, seed, steps):
+    """The stock lookup-table optics on the synthetic detector (one made-up v6 table, crvhip.synth_lookup): the cosmic
+    workload's steps as CrvStep rows through crv_lookup_photons_run (photons at the SiPMs, FP64), then crv_sipm_run
+    (charges), inputs resident in HBM; each call is timed on the host (the ABI returns once its stream has finished).
+    A sizing pass and a warmup precede the `steps` timed passes."""
+    import torch
+    from crvhip import _lib as L, chain as CH, config as CF, detector as DT, photons as PH, synth as SY
+    from crvhip import synth_lookup as SL, template as TP
+    S = SY.cosmic_events(n_events, seed=seed)
+    cols, pdg = SL.crv_steps(S)
+    del S
+    T = SL.table()
+    G, _ = DT.synthetic_crv_geometry()
+    cfg = CF.ChainConfig(response=SL.response_config())
+    ctx = L.Context(cfg, DT.nominal_conditions(), None, TP.load(cfg.waveform), 0)
+    PH.upload(ctx, {T.name: T}, G, cfg.response)
+    ls = PH.LookupSteps(CH.event_key(cols), cols, pdg)
+    ctx.set_params(cfg, seed + 1, ls.evkey_hi)
+    pb, po = PH.device_photons(ctx, ls)  # sizing pass: the photon buffers at the size the call needs
+    nph = po.n
+    pl = L.PhotonList(L.ptr(pb["label"]), L.ptr(pb["time"]), L.ptr(pb["photon"]), L.ptr(pb["j"]), L.ptr(pb["rank"]),
+                      nph)
+    sc = cfg.sipm
+    nz = L.SipmNoise(0.0, 0.0, 0.0, float(sc.trap0_lifetime), float(sc.trap1_lifetime), 0.0, float(cfg.window[1]),
+                     None, 0)
+    cap = 2 * nph + 4096
+    for _ in range(2):
+        qb = {k: torch.empty(cap, dtype=dt, device="cuda") for k, dt in (
+            ("label", torch.int64), ("time", torch.float64), ("charge", torch.float64), ("pe", torch.float64),
+            ("photon", torch.int64))}
+        qo = L.Charges(*[L.ptr(qb[k]) for k in ("label", "time", "charge", "pe", "photon")], cap, 0, 0, 0)
+        rc = ctx.lib.crv_sipm_run(ctx.h, ctypes.byref(pl), ctypes.byref(nz), ctypes.byref(qo), None)
+        if rc == L.CRV_E_CAPACITY:
+            cap = qo.n
+            continue
+        ctx.check(rc)
+        break
+    st = ls.struct()
+    t_ph, t_si = [], []
+    for it in range(steps + 1):  # a warmup, then the timed passes
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.check(ctx.lib.crv_lookup_photons_run(ctx.h, ctypes.byref(st), ctypes.byref(po), None))
+        t1 = time.perf_counter()
+        ctx.check(ctx.lib.crv_sipm_run(ctx.h, ctypes.byref(pl), ctypes.byref(nz), ctypes.byref(qo), None))
+        t2 = time.perf_counter()
+        if it:
+            t_ph.append(t1 - t0)
+            t_si.append(t2 - t1)
+    assert po.n == nph, "the photons of a pass changed"
+    ph_s, si_s = float(np.mean(t_ph)), float(np.mean(t_si))
+    ph_bytes = LK_STEP_B * ls.n + PHOTON_B * nph
+    si_bytes = PHOTON_B * nph + SIPM_CHARGE_B * qo.n
+    out = {"value": n_events / (ph_s + si_s), "unit": "events/s", "events": n_events, "steps": steps,
+           "ms_per_step": (ph_s + si_s) * 1e3, "dtype": "f64",
+           "config": {"workload": "cosmic events as CrvStep rows, one synthetic v6 lookup table "
+                                  f"({sum(t.delays.nbytes for t in T.tables.values()) >> 20} MiB of delay weights)",
+                      "stages": "crv_lookup_photons_run -> crv_sipm_run"},
+           "stage_ms": {"photons": ph_s * 1e3, "sipm": si_s * 1e3},
+           "roofline": {"bound": "hbm", "kernel": "lookup photons stage (count, scan, fill)",
+                        "achieved": ph_bytes / ph_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ph_bytes / ph_s / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "bytes_per_launch": ph_bytes, "ms_per_launch": ph_s * 1e3},
+           "sipm_stage": {"achieved": si_bytes / si_s / 1e9, "unit": "GB/s", "bytes_per_launch": si_bytes},
+           "per_event": {"steps": ls.n / n_events, "photons": nph / n_events, "charges": qo.n / n_events}}
+    ctx.close()
+    del pb, qb, ls
+    torch.cuda.empty_cache()
+    return out
+
+
 def workload_leg(workload, n_events, chunk, seed, steps, warmup, batch, autotune, graph=False):
     """One extra workload (device-generated, resident in HBM) through the FP32 batched chain in calls of `chunk`
     events: events/s over `steps` timed passes after the sizing pass, with its own dominant-kernel roofline."""
@@ -366,6 +443,9 @@ def main():
                     help="uneven leg: run every chunk through crv_chain_run_batched instead of one captured graph")
     ap.add_argument("--overlay-events", type=int, default=200_000, help="high-occupancy overlay leg (0: skip)")
     ap.add_argument("--overlay-chunk", type=int, default=20_000)
+    ap.add_argument("--lookup-events", type=int, default=100_000,
+                    help="lookup-table optics leg: cosmic events as CrvStep rows through the photon and SiPM stages "
+                         "(0: skip)")
     ap.add_argument("--workload", default="cosmic")
     ap.add_argument("--graph", action="store_true", help="headline: replay one captured graph per chunk")
     ap.add_argument("--seed", type=int, default=0)
@@ -497,6 +577,9 @@ def main():
             legs["overlay"] = workload_leg("overlay", args.overlay_events, args.overlay_chunk, args.seed, 1, 1,
                                            {**batch, "tile": 128, "sort_tasks": 2}, False)
             progress(f"overlay {args.overlay_events}: {legs['overlay']['value']:.0f} events/s")
+        if args.lookup_events:
+            legs["lookup"] = lookup_leg(args.lookup_events, args.seed, 3)
+            progress(f"lookup {args.lookup_events}: {legs['lookup']['value']:.0f} events/s")
     S_, W_, P_, C_, M_ = (cnt32["n_steps"], cnt32["n_records"], cnt32["n_pulses"], cnt32["n_clusters"],
                           cnt32["n_members"])
     b_event = (STEP_B * S_ + RECORD_B * W_ + PULSE_B * P_ + CLUSTER_B * C_ + MEMBER_B * M_) / args.events
