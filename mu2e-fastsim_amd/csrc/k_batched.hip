@@ -1381,32 +1381,31 @@ __global__ __launch_bounds__(64) void kb_fit(crv_params p, Tables t, const uint6
 
 // fired-pixel log capacity per lane of each group of 64 SiPMs in queue order, from the largest photon count of the
 // group (3 factor / 4 of it + 16; factor doubles on every re-run); the group takes 64 times that
+// log capacity per group of 64 queue positions: one wavefront per group, each lane one SiPM (coalesced queue reads),
+// the group's largest photon count by a wavefront max
 __global__ void kb_gcaps(const int64_t *__restrict__ nph, const int32_t *__restrict__ order, int64_t L4,
-                         int64_t *__restrict__ gcap, int64_t *__restrict__ gsize, int factor, int exact_order,
+                         int64_t *__restrict__ gcap, int64_t *__restrict__ gsize, int factor,
                          const int32_t *__restrict__ Tdev = nullptr) {
-  const int64_t grp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (grp >= (L4 + 63) / 64) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t grp = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (grp >= (L4 + 63) / 64) return;  // (whole wavefronts: every lane of a group returns together)
   if (Tdev && grp * 64 >= 4 * (int64_t)*Tdev) {  // fixed-capacity calls: groups past the call hold nothing
-    gcap[grp] = 0;
-    gsize[grp] = 0;
+    if (lane == 0) {
+      gcap[grp] = 0;
+      gsize[grp] = 0;
+    }
     return;
   }
   if (Tdev) L4 = min(L4, 4 * (int64_t)*Tdev);
+  const int64_t gq = grp * 64 + lane;
   int64_t m = 0;
-  // sorted longest first by the exact count (clamped to 32 bits): the group's first SiPM has its largest count; the
-  // octave orders keep a bucket's SiPMs in their given order, so there the group's largest count is looked up
-  const int64_t first = order && exact_order ? nph[order[grp * 64]] : -1;
-  if (first >= 0 && first < 0xFFFFFFFFll) {
-    m = first * 3 * factor / 4 + 16;
-  } else {
-    for (int64_t gq = grp * 64; gq < L4 && gq < grp * 64 + 64; ++gq) {
-      const int64_t v = nph[order ? order[gq] : gq];
-      m = max(m, v * 3 * factor / 4 + 16);  // fired pixels: about 42 % of the photons, binomial spread
-    }
+  if (gq < L4) m = nph[order ? order[gq] : gq] * 3 * factor / 4 + 16;  // fired pixels: about 42 % of the photons
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (int64_t)__shfl_xor(m, o));
+  if (lane == 0) {
+    m = (m + 1) & ~(int64_t)1;
+    gcap[grp] = m;
+    gsize[grp] = m / 2 * 64;  // chunks of two entries
   }
-  m = (m + 1) & ~(int64_t)1;
-  gcap[grp] = m;
-  gsize[grp] = m / 2 * 64;  // chunks of two entries
 }
 
 __global__ void kb_zero(int *__restrict__ p, int n) {
@@ -1693,7 +1692,7 @@ int batched_stage(Ctx *c, const crv_steps *steps, crv_batch_opts *opt, crv_timin
   for (int attempt = 0, factor = c->b_log_factor;; ++attempt, factor *= 2) {
     if (attempt > 8) return set_error(c, CRV_E_OVERFLOW, "batched chain: fired-pixel log still too short");
     if (L4)
-      kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, qorder, L4, gcap, gsize, factor, opt->sort_tasks == 1);
+      kb_gcaps<<<nblocks(ngrp * 64, 256), 256, 0, s>>>(sipm_nph, qorder, L4, gcap, gsize, factor);
     if ((rc = scan_i64(c, gsize, goff, ngrp, 510, s))) return rc;
     if (attempt == 0) CRV_HIP(c, hipMemcpyAsync(hf, c->flags, sizeof(hf), hipMemcpyDeviceToHost, s));
     if ((rc = read_i64(c, goff + ngrp, &nlog, s))) return rc;  // (synchronises the stream)
@@ -2083,8 +2082,7 @@ int batched_fixed(Ctx *c, Plan &pl, Arena &ar, hipStream_t s) {
     CRV_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, skey, skey2, sidx, order, (int)L4, 0, sbits, s));
   }
   const int32_t *qorder = sorted ? order : nullptr;
-  kb_gcaps<<<nblocks(ngrp, 256), 256, 0, s>>>(sipm_nph, qorder, L4, gcap, gsize, pl.log_factor,
-                                              pl.sort_tasks == 1, pl.T_dev);
+  kb_gcaps<<<nblocks(ngrp * 64, 256), 256, 0, s>>>(sipm_nph, qorder, L4, gcap, gsize, pl.log_factor, pl.T_dev);
   CRV_HIP(c, hipGetLastError());
   if ((rc = scan_fixed(c, ar, gsize, goff, ngrp, s))) return rc;
   kb_check_log<<<1, 1, 0, s>>>(goff + ngrp, pl.log_cap, pl.skip);
