@@ -28,7 +28,6 @@ namespace {
 
 constexpr int CMAXB = 16;   // cascade candidates per generation and lane
 constexpr int MAXGEN = 31;  // cascade generations
-constexpr int CASCADE_BATCH = 8;  // waiting cascades that run together (4: 77.4 ms, 16: 76.7, 32: 96.4 on 1M cosmic)
 // bits of the 7 x 7 cascade window (bit (dx + 3) * 7 + dy + 3) with |dx| = 3 or |dy| = 3
 constexpr uint64_t WIN_BORDER = 0x1FE0C183060FFull;
 constexpr int RINGB = 16;   // open samples of the waveform window
@@ -552,161 +551,8 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
             o.gRJ[gi] = (uint64_t)j << 32;
           }
         }
-        bool bad = false;  // the log or the cascade lists overflowed
-        // cross-talk cascade of a fired photon (pixel pix, step ordinal j, rank r, time offset toff)
-        auto cascade = [&](const int pix, const uint32_t j, const int r, const float toff) {
-            // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order. While a
-            // generation's pixels stay inside the 7 x 7 window around the photon's pixel, the generation, its candidates
-            // and their multiplicities are bit masks over the window in registers (bit (dx + 3) * 7 + dy + 3; ascending
-            // bit = ascending pixel); a generation with a pixel on the window's border continues in the per-lane lists
-            // in global scratch below. Both visit the same draws in the same order.
-            const int cx = pix / CRV_NPIX, cy = pix % CRV_NPIX;
-            uint64_t cur = 1ull << 24;  // generation 0: the photon's pixel at the window's centre
-            int gnum = 0;
-            bool window = true;
-            while (!bad) {
-              if (cur & WIN_BORDER) {
-                window = false;  // neighbours could leave the window
-                break;
-              }
-              uint64_t m0 = 0, m1 = 0, m2 = 0;  // candidate multiplicity, bit planes of a 3-bit count per position
-              for (uint64_t rem = cur; rem; rem &= rem - 1) {
-                const int bq = __ffsll((long long)rem) - 1;
-                const int fx = cx + bq / 7 - 3, fy = cy + bq % 7 - 3;
-                const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j,
-                                 pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)(fx * CRV_NPIX + fy)));
-                uint64_t add = 0;
-                if (fx > 0 && u32f(ux.x) < qct) add |= 1ull << (bq - 7);
-                if (fx + 1 < p.n_pixels_x && u32f(ux.y) < qct) add |= 1ull << (bq + 7);
-                if (fy > 0 && u32f(ux.z) < qct) add |= 1ull << (bq - 1);
-                if (fy + 1 < p.n_pixels_y && u32f(ux.w) < qct) add |= 1ull << (bq + 1);
-                const uint64_t c0 = m0 & add;
-                m0 ^= add;
-                const uint64_t c1 = m1 & c0;
-                m1 ^= c0;
-                m2 ^= c1;
-              }
-              const uint64_t cand = m0 | m1 | m2;
-              if (!cand) break;
-              if (__popcll(cand) > CMAXB || gnum + 1 > MAXGEN) {
-                atomicAdd(&flags[F_CASCADE], 1);
-                bad = true;
-                break;
-              }
-              ++gnum;
-              uint64_t nxt = 0;
-              for (uint64_t rem = cand; rem; rem &= rem - 1) {
-                const int bq = __ffsll((long long)rem) - 1;
-                const int c = (cx + bq / 7 - 3) * CRV_NPIX + cy + bq % 7 - 3;
-                const int m = (int)((m0 >> bq) & 1u) + 2 * (int)((m1 >> bq) & 1u) + 4 * (int)((m2 >> bq) & 1u);
-                const bool c_set = bget(bm, lane, c);
-                if (c_set && dead_px(t, c)) continue;
-                float vc = Vov;
-                if (c_set) vc = Vov * (1.f - __expf(-(toff - lastt[c * xs]) * inv_tau));
-                const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
-                float qm = q1;
-                for (int mm = 2; mm <= m; ++mm) qm *= q1;
-                const float uc =
-                    u32f(dr(KIND_XTFIRE, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)c)).x);
-                if (!(uc < 1.f - qm)) continue;
-                if (nlog >= lcap) {
-                  bad = true;
-                  break;
-                }
-                log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, c, toff, vc);
-                bset(bm, lane, c);
-                nxt |= 1ull << bq;
-              }
-              cur = nxt;
-              if (!cur) break;
-            }
-            if (bad) return;
-            if (window) return;
-            int ncur = 0;
-            for (uint64_t rem = cur; rem; rem &= rem - 1, ++ncur) {
-              const int bq = __ffsll((long long)rem) - 1;
-              xcur[ncur * xs] = (uint16_t)((cx + bq / 7 - 3) * CRV_NPIX + cy + bq % 7 - 3);
-            }
-            while (ncur > 0 && !bad) {
-              int ncand = 0;
-              for (int ai = 0; ai < ncur; ++ai) {
-                const int f = (int)xcur[ai * xs];
-                const int fx = f / CRV_NPIX, fy = f % CRV_NPIX;
-                const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)f));
-                const uint32_t uw[4] = {ux.x, ux.y, ux.z, ux.w};
-    #pragma unroll
-                for (int nb = 0; nb < 4; ++nb) {
-                  const int nx = nb == 0 ? fx - 1 : nb == 1 ? fx + 1 : fx;
-                  const int ny = nb == 2 ? fy - 1 : nb == 3 ? fy + 1 : fy;
-                  const bool ok =
-                      nb == 0 ? fx > 0 : nb == 1 ? fx + 1 < p.n_pixels_x : nb == 2 ? fy > 0 : fy + 1 < p.n_pixels_y;
-                  if (!ok || !(u32f(uw[nb]) < qct)) continue;
-                  const int c = nx * CRV_NPIX + ny;
-                  int at = 0;
-                  while (at < ncand && xcand[at * xs] < c) ++at;
-                  if (at < ncand && xcand[at * xs] == c) {
-                    xmult[at * xs] += 1;
-                  } else if (ncand < CMAXB) {
-                    for (int q = ncand; q > at; --q) {
-                      xcand[q * xs] = xcand[(q - 1) * xs];
-                      xmult[q * xs] = xmult[(q - 1) * xs];
-                    }
-                    xcand[at * xs] = (uint16_t)c;
-                    xmult[at * xs] = 1;
-                    ++ncand;
-                  } else {
-                    atomicAdd(&flags[F_CASCADE], 1);
-                    bad = true;
-                  }
-                }
-              }
-              if (ncand == 0 || bad) break;
-              if (++gnum > MAXGEN) {
-                atomicAdd(&flags[F_CASCADE], 1);
-                bad = true;
-                break;
-              }
-              ncur = 0;
-              for (int ai = 0; ai < ncand; ++ai) {
-                const int c = xcand[ai * xs];
-                const int m = xmult[ai * xs];
-                const bool c_set = bget(bm, lane, c);
-                if (c_set && dead_px(t, c)) continue;
-                float vc = Vov;
-                if (c_set) vc = Vov * (1.f - __expf(-(toff - lastt[c * xs]) * inv_tau));
-                const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
-                float qm = q1;
-                for (int mm = 2; mm <= m; ++mm) qm *= q1;
-                const float uc =
-                    u32f(dr(KIND_XTFIRE, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)c)).x);
-                if (!(uc < 1.f - qm)) continue;
-                if (nlog >= lcap) {
-                  bad = true;
-                  break;
-                }
-                log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, c, toff, vc);
-                bset(bm, lane, c);
-                xcur[ncur * xs] = (uint16_t)c;
-                ++ncur;
-              }
-            }
-        };
-        bool pend = false;  // a cascade waits
-        int c_pix = 0, c_r = 0;
-        uint32_t c_j = 0;
-        float c_toff = 0.f;
+        bool bad = false;
         for (;;) {
-          {
-            const uint64_t wp = __ballot(pend);
-            if (wp && (__popcll(wp) >= CASCADE_BATCH || __ballot(!pend) == 0)) {  // (wave-uniform)
-              if (pend) {
-                pend = false;
-                cascade(c_pix, c_j, c_r, c_toff);
-              }
-            }
-          }
-          if (bad) break;
-          if (pend) continue;
           // next photon: the earliest stream head (lowest step first on ties)
           int kb = -1;
           double tph = 0.0;
@@ -833,23 +679,142 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           }
           log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, pix, toff, v);
           bset(bm, lane, pix);
-          // cross-talk: the photon's first generation of proposals. A photon whose proposals hold a candidate waits
-          // (pend) with its cascade until CASCADE_BATCH lanes of the wave wait or no other lane can go on; the cascades
-          // then run together (the same draws in the same order as one after the other: a waiting lane takes no
-          // photon before its cascade is done).
-          {
-            const int cx = pix / CRV_NPIX, cy = pix % CRV_NPIX;
-            const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, 0u, (uint32_t)pix));
-            const bool any = (cx > 0 && u32f(ux.x) < qct) || (cx + 1 < p.n_pixels_x && u32f(ux.y) < qct) ||
-                             (cy > 0 && u32f(ux.z) < qct) || (cy + 1 < p.n_pixels_y && u32f(ux.w) < qct);
-            if (any) {
-              pend = true;
-              c_pix = pix;
-              c_j = j;
-              c_r = r;
-              c_toff = toff;
+          // cross-talk cascade, breadth-first, candidates of a generation in ascending pixel order. While a
+          // generation's pixels stay inside the 7 x 7 window around the photon's pixel, the generation, its candidates
+          // and their multiplicities are bit masks over the window in registers (bit (dx + 3) * 7 + dy + 3; ascending
+          // bit = ascending pixel); a generation with a pixel on the window's border continues in the per-lane lists
+          // in global scratch below. Both visit the same draws in the same order.
+          const int cx = pix / CRV_NPIX, cy = pix % CRV_NPIX;
+          uint64_t cur = 1ull << 24;  // generation 0: the photon's pixel at the window's centre
+          int gnum = 0;
+          bool window = true;
+          while (!bad) {
+            if (cur & WIN_BORDER) {
+              window = false;  // neighbours could leave the window
+              break;
+            }
+            uint64_t m0 = 0, m1 = 0, m2 = 0;  // candidate multiplicity, bit planes of a 3-bit count per position
+            for (uint64_t rem = cur; rem; rem &= rem - 1) {
+              const int bq = __ffsll((long long)rem) - 1;
+              const int fx = cx + bq / 7 - 3, fy = cy + bq % 7 - 3;
+              const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j,
+                               pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)(fx * CRV_NPIX + fy)));
+              uint64_t add = 0;
+              if (fx > 0 && u32f(ux.x) < qct) add |= 1ull << (bq - 7);
+              if (fx + 1 < p.n_pixels_x && u32f(ux.y) < qct) add |= 1ull << (bq + 7);
+              if (fy > 0 && u32f(ux.z) < qct) add |= 1ull << (bq - 1);
+              if (fy + 1 < p.n_pixels_y && u32f(ux.w) < qct) add |= 1ull << (bq + 1);
+              const uint64_t c0 = m0 & add;
+              m0 ^= add;
+              const uint64_t c1 = m1 & c0;
+              m1 ^= c0;
+              m2 ^= c1;
+            }
+            const uint64_t cand = m0 | m1 | m2;
+            if (!cand) break;
+            if (__popcll(cand) > CMAXB || gnum + 1 > MAXGEN) {
+              atomicAdd(&flags[F_CASCADE], 1);
+              bad = true;
+              break;
+            }
+            ++gnum;
+            uint64_t nxt = 0;
+            for (uint64_t rem = cand; rem; rem &= rem - 1) {
+              const int bq = __ffsll((long long)rem) - 1;
+              const int c = (cx + bq / 7 - 3) * CRV_NPIX + cy + bq % 7 - 3;
+              const int m = (int)((m0 >> bq) & 1u) + 2 * (int)((m1 >> bq) & 1u) + 4 * (int)((m2 >> bq) & 1u);
+              const bool c_set = bget(bm, lane, c);
+              if (c_set && dead_px(t, c)) continue;
+              float vc = Vov;
+              if (c_set) vc = Vov * (1.f - __expf(-(toff - lastt[c * xs]) * inv_tau));
+              const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
+              float qm = q1;
+              for (int mm = 2; mm <= m; ++mm) qm *= q1;
+              const float uc =
+                  u32f(dr(KIND_XTFIRE, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)c)).x);
+              if (!(uc < 1.f - qm)) continue;
+              if (nlog >= lcap) {
+                bad = true;
+                break;
+              }
+              log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, c, toff, vc);
+              bset(bm, lane, c);
+              nxt |= 1ull << bq;
+            }
+            cur = nxt;
+            if (!cur) break;
+          }
+          if (bad) break;
+          if (window) continue;
+          int ncur = 0;
+          for (uint64_t rem = cur; rem; rem &= rem - 1, ++ncur) {
+            const int bq = __ffsll((long long)rem) - 1;
+            xcur[ncur * xs] = (uint16_t)((cx + bq / 7 - 3) * CRV_NPIX + cy + bq % 7 - 3);
+          }
+          while (ncur > 0 && !bad) {
+            int ncand = 0;
+            for (int ai = 0; ai < ncur; ++ai) {
+              const int f = (int)xcur[ai * xs];
+              const int fx = f / CRV_NPIX, fy = f % CRV_NPIX;
+              const U4 ux = dr(KIND_XTALK, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)f));
+              const uint32_t uw[4] = {ux.x, ux.y, ux.z, ux.w};
+  #pragma unroll
+              for (int nb = 0; nb < 4; ++nb) {
+                const int nx = nb == 0 ? fx - 1 : nb == 1 ? fx + 1 : fx;
+                const int ny = nb == 2 ? fy - 1 : nb == 3 ? fy + 1 : fy;
+                const bool ok =
+                    nb == 0 ? fx > 0 : nb == 1 ? fx + 1 < p.n_pixels_x : nb == 2 ? fy > 0 : fy + 1 < p.n_pixels_y;
+                if (!ok || !(u32f(uw[nb]) < qct)) continue;
+                const int c = nx * CRV_NPIX + ny;
+                int at = 0;
+                while (at < ncand && xcand[at * xs] < c) ++at;
+                if (at < ncand && xcand[at * xs] == c) {
+                  xmult[at * xs] += 1;
+                } else if (ncand < CMAXB) {
+                  for (int q = ncand; q > at; --q) {
+                    xcand[q * xs] = xcand[(q - 1) * xs];
+                    xmult[q * xs] = xmult[(q - 1) * xs];
+                  }
+                  xcand[at * xs] = (uint16_t)c;
+                  xmult[at * xs] = 1;
+                  ++ncand;
+                } else {
+                  atomicAdd(&flags[F_CASCADE], 1);
+                  bad = true;
+                }
+              }
+            }
+            if (ncand == 0 || bad) break;
+            if (++gnum > MAXGEN) {
+              atomicAdd(&flags[F_CASCADE], 1);
+              bad = true;
+              break;
+            }
+            ncur = 0;
+            for (int ai = 0; ai < ncand; ++ai) {
+              const int c = xcand[ai * xs];
+              const int m = xmult[ai * xs];
+              const bool c_set = bget(bm, lane, c);
+              if (c_set && dead_px(t, c)) continue;
+              float vc = Vov;
+              if (c_set) vc = Vov * (1.f - __expf(-(toff - lastt[c * xs]) * inv_tau));
+              const float q1 = vc == Vov ? 1.f - pf_full : 1.f - p1 * (1.f - __expf(-vc * inv_p2));
+              float qm = q1;
+              for (int mm = 2; mm <= m; ++mm) qm *= q1;
+              const float uc =
+                  u32f(dr(KIND_XTFIRE, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, (uint32_t)gnum, (uint32_t)c)).x);
+              if (!(uc < 1.f - qm)) continue;
+              if (nlog >= lcap) {
+                bad = true;
+                break;
+              }
+              log_fire(o, lb, nlog, pend_t, pend_v, lastt, xs, c, toff, vc);
+              bset(bm, lane, c);
+              xcur[ncur * xs] = (uint16_t)c;
+              ++ncur;
             }
           }
+          if (bad) break;
         }
         if (nlog & 1) o.flog[lb + (nlog >> 1) * 64] = make_float4(pend_t, pend_v, 0.f, 0.f);
         if (bad) {
