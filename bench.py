@@ -217,7 +217,7 @@ def measure(precision, ds, evkey, n_ev, seed, steps, warmup, batch, chunk=0, aut
                 acc[f] = acc.get(f, 0) + getattr(tm, f)
         return acc
 
-    for _ in range(max(warmup, 1)):  # full passes into the caller's buffers after the sizing pass
+    for _ in range(warmup):  # full passes into the caller's buffers after the sizing pass
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -441,8 +441,9 @@ def main():
     ap.add_argument("--uneven-chunk", type=int, default=500_000)
     ap.add_argument("--no-graph", action="store_true",
                     help="uneven leg: run every chunk through crv_chain_run_batched instead of one captured graph")
-    ap.add_argument("--overlay-events", type=int, default=200_000, help="high-occupancy overlay leg (0: skip)")
-    ap.add_argument("--overlay-chunk", type=int, default=20_000)
+    ap.add_argument("--overlay-events", type=int, default=1_000_000, help="high-occupancy overlay leg (0: skip)")
+    ap.add_argument("--overlay-chunk", type=int, default=10_000,
+                    help="overlay events per call (the 1M overlay events' steps alone take 89 GB of HBM)")
     ap.add_argument("--lookup-events", type=int, default=100_000,
                     help="lookup-table optics leg: cosmic events as CrvStep rows through the photon and SiPM stages "
                          "(0: skip)")
@@ -568,13 +569,13 @@ def main():
     legs = {}
     if args.workload == "cosmic":
         if args.uneven_events:
-            legs["uneven"] = workload_leg("uneven", args.uneven_events, args.uneven_chunk, args.seed, 1, 1,
+            legs["uneven"] = workload_leg("uneven", args.uneven_events, args.uneven_chunk, args.seed, 1, 0,
                                           {**batch, "tile": args.tile, "sort_tasks": args.sort}, autotune,
                                           graph=not args.no_graph)
             progress(f"uneven {args.uneven_events}: {legs['uneven']['value']:.0f} events/s")
         if args.overlay_events:
             # the overlay's queue options as the autotuner picked them for this workload (tile 128, octave order)
-            legs["overlay"] = workload_leg("overlay", args.overlay_events, args.overlay_chunk, args.seed, 1, 1,
+            legs["overlay"] = workload_leg("overlay", args.overlay_events, args.overlay_chunk, args.seed, 1, 0,
                                            {**batch, "tile": 128, "sort_tasks": 2}, False)
             progress(f"overlay {args.overlay_events}: {legs['overlay']['value']:.0f} events/s")
         if args.lookup_events:
