@@ -17,9 +17,12 @@ crv/art/CrvFastSim/src/CrvRecoTableDumper_module.cc:97-185) for the trees this c
 * calib    run, channel, pedestal, pulsearea, pulseheight, timeoffset, status, roc, feb, febch
 
 `idx` is the row's index in its event's collection; a cluster's `pulses` are the `idx` of its member pulses in the
-pulse tree; `flags` bit 0 is the failed fit (the bit the reference decodes, reco/cli.py:47); `layers` is the number
-of distinct layers among the cluster's member pulses; `wfidx` holds the pulse's reco-waveform number in the event
-(this chain keeps no art digi collection to point into); spill, evlen and pbtime are 0 (no onspill folding).
+pulse tree; `flags` bit 0 is the failed fit (the bit the reference decodes, reco/cli.py:47); `layers` holds the
+sorted distinct layers of the cluster's member pulses (reference reco/coinc.py:391, read back as a jagged int column by
+CrvCoincidenceClustersFromTable_module.cc:69); `wfidx` holds the pulse's reco-waveform number in the event (this chain
+keeps no art digi collection to point into); spill, evlen and pbtime are 0 (no onspill folding). Column types are the
+dumper's branch types (CrvRecoTableDumper_module.cc:59-92): unsigned run, subrun, event; int counters and indices;
+float PE, height, slope, positions and calibration constants; double times; short ADC words.
 
 Storage: a path ending in `.npz` is one archive with keys `<tree>/<column>`, a jagged column stored flat with its
 per-row counts under `<tree>/n<column>` (the counted-array convention the reference's table reader accepts,
@@ -42,7 +45,29 @@ TREES = {
                  "sidehits1"),
     "calib": ("run", "channel", "pedestal", "pulsearea", "pulseheight", "timeoffset", "status", "roc", "feb", "febch"),
 }
-JAGGED = {"digis": ("adc",), "pulses": ("wfidx",), "clusters": ("pulses",)}
+JAGGED = {"digis": ("adc",), "pulses": ("wfidx",), "clusters": ("layers", "pulses")}
+# the dumper's branch types (reference CrvRecoTableDumper_module.cc:59-92; jagged columns: the element type)
+_U, _I, _F, _D = np.uint32, np.int32, np.float32, np.float64
+_IDS = {"run": _U, "subrun": _U, "event": _U}
+_CHAN = {"idx": _I, "bar": _I, "sipm": _I}
+DTYPES = {
+    "events": {**_IDS, "spill": _I, "evlen": _D, "pbtime": _D, "ndigi": _I, "npulse": _I, "ncluster": _I},
+    "digis": {**_IDS, **_CHAN, "starttdc": _I, "nzs": _I, "roc": _I, "feb": _I, "febch": _I, "adc": np.int16},
+    "pulses": {**_IDS, **_CHAN, **{k: _F for k in ("pe", "peph", "height", "beta", "chi2", "ped")}, "time": _D,
+               "letime": _D, "seq": _I, "flags": _I, "pedfromdb": _I, "wfidx": _I},
+    "clusters": {**_IDS, "idx": _I, "sectortype": _I, "start": _D, "end": _D, "pe": _F, "slope": _F, "layers": _I,
+                 "pulses": _I, "poscalc": _I, "avgtime": _D, **{k: _F for k in ("x", "y", "z", "sidepe0", "sidepe1")},
+                 "sidetime0": _D, "sidetime1": _D, "sidehits0": _I, "sidehits1": _I},
+    "calib": {"run": _U, "channel": _I, **{k: _F for k in ("pedestal", "pulsearea", "pulseheight", "timeoffset")},
+              "status": _I, "roc": _I, "feb": _I, "febch": _I},
+}
+
+
+def _typed(tree, cols):
+    """A tree's columns in the dumper's types."""
+    dt = DTYPES[tree]
+    return {c: Jagged(v.values.astype(dt[c]), v.counts) if isinstance(v, Jagged) else np.asarray(v).astype(dt[c])
+            for c, v in cols.items()}
 
 
 class Jagged:
@@ -144,12 +169,13 @@ def pulses_tree(P, run=0):
     n = len(ev)
     wf = np.asarray(P["wf"], np.int64)
     wf_in_event = wf - np.repeat(_first_in_event(ev, wf), np.diff(np.r_[_starts(ev), n])) if n else wf
-    return {"run": np.full(n, run, np.int64), "subrun": sub, "event": evt, "idx": _rank_in_event(ev),
+    out = {"run": np.full(n, run, np.int64), "subrun": sub, "event": evt, "idx": _rank_in_event(ev),
             "bar": chan // 4, "sipm": chan % 4,
             **{k: np.asarray(P[k], np.float64) for k in ("pe", "peph", "height", "beta", "chi2", "ped", "time",
                                                           "letime")},
             "seq": np.asarray(P["seq"], np.int64), "flags": np.asarray(P["failed"], np.int64) & 1,
             "pedfromdb": np.asarray(P["pedfromdb"], bool), "wfidx": Jagged(wf_in_event, np.ones(n, np.int64))}
+    return _typed("pulses", out)
 
 
 def _starts(ev):
@@ -170,25 +196,26 @@ def clusters_tree(C, layer_of_bar, pulse_bar, pulse_ev, run=0):
     mem_cl = np.asarray(C["mem_cl"], np.int64)
     mem_idx = np.asarray(C["mem_idx"], np.int64)
     counts = np.bincount(mem_cl, minlength=nc)[:nc] if nc else np.zeros(0, np.int64)
-    # layers: distinct layers of each cluster's member pulses
+    # layers: the sorted distinct layers of each cluster's member pulses
     pev = np.asarray(pulse_ev, np.int64)  # sorted by event
     rows = np.searchsorted(pev, np.repeat(ev, counts), side="left") + mem_idx
     lay = np.asarray(layer_of_bar)[np.asarray(pulse_bar, np.int64)[rows]] if len(rows) else rows
-    nlay = np.zeros(nc, np.int64)
+    layers = Jagged(np.zeros(0, np.int64), np.zeros(nc, np.int64))
     if len(lay):
-        pair = np.unique(mem_cl * 8 + lay)
-        nlay = np.bincount(pair // 8, minlength=nc)[:nc]
+        pair = np.unique(mem_cl * 8 + np.asarray(lay, np.int64))
+        layers = Jagged(pair % 8, np.bincount(pair // 8, minlength=nc)[:nc])
     pos = np.asarray(C["cl_pos"], float).reshape(nc, 3)
     spe = np.asarray(C["cl_sidepe"], float).reshape(nc, 2)
     st = np.asarray(C["cl_sidetime"], float).reshape(nc, 2)
     sh = np.asarray(C["cl_sidehits"], np.int64).reshape(nc, 2)
-    return {"run": np.full(nc, run, np.int64), "subrun": sub, "event": evt, "idx": _rank_in_event(ev),
+    return _typed("clusters", {
+            "run": np.full(nc, run, np.int64), "subrun": sub, "event": evt, "idx": _rank_in_event(ev),
             "sectortype": np.asarray(C["cl_type"], np.int64), "start": np.asarray(C["cl_start"], float),
             "end": np.asarray(C["cl_end"], float), "pe": np.asarray(C["cl_pe"], float),
-            "slope": np.asarray(C["cl_slope"], float), "layers": nlay, "pulses": Jagged(mem_idx, counts),
+            "slope": np.asarray(C["cl_slope"], float), "layers": layers, "pulses": Jagged(mem_idx, counts),
             "poscalc": np.asarray(C["cl_poscalc"], bool), "avgtime": np.asarray(C["cl_avgtime"], float),
             "x": pos[:, 0], "y": pos[:, 1], "z": pos[:, 2], "sidepe0": spe[:, 0], "sidepe1": spe[:, 1],
-            "sidetime0": st[:, 0], "sidetime1": st[:, 1], "sidehits0": sh[:, 0], "sidehits1": sh[:, 1]}
+            "sidetime0": st[:, 0], "sidetime1": st[:, 1], "sidehits0": sh[:, 0], "sidehits1": sh[:, 1]})
 
 
 def digis_tree(D, cond, run=0):
@@ -199,11 +226,12 @@ def digis_tree(D, cond, run=0):
     sub, evt = _key_parts(ev)
     adc = np.asarray(D["adc"])
     n = len(lab)
-    return {"run": np.full(n, run, np.int64), "subrun": sub, "event": evt, "idx": _rank_in_event(ev),
-            "bar": ch // 4, "sipm": ch % 4, "starttdc": np.asarray(D["starttdc"], np.int64),
-            "nzs": np.zeros(n, bool), "roc": np.asarray(cond.roc, np.int64)[ch],
-            "feb": np.asarray(cond.feb, np.int64)[ch], "febch": ch % CHANNELS_PER_FEB,
-            "adc": Jagged(adc.reshape(-1).astype(np.int16), np.full(n, adc.shape[1] if adc.ndim == 2 else 0))}
+    return _typed("digis", {
+        "run": np.full(n, run, np.int64), "subrun": sub, "event": evt, "idx": _rank_in_event(ev),
+        "bar": ch // 4, "sipm": ch % 4, "starttdc": np.asarray(D["starttdc"], np.int64),
+        "nzs": np.zeros(n, bool), "roc": np.asarray(cond.roc, np.int64)[ch],
+        "feb": np.asarray(cond.feb, np.int64)[ch], "febch": ch % CHANNELS_PER_FEB,
+        "adc": Jagged(adc.reshape(-1).astype(np.int16), np.full(n, adc.shape[1] if adc.ndim == 2 else 0))})
 
 
 def calib_tree(cond, run=0):
@@ -211,11 +239,12 @@ def calib_tree(cond, run=0):
     from .detector import CHANNELS_PER_FEB
     n = len(cond.pedestal)
     ch = np.arange(n, dtype=np.int64)
-    return {"run": np.full(n, run, np.int64), "channel": ch, "pedestal": np.asarray(cond.pedestal, float),
-            "pulsearea": np.asarray(cond.pulsearea, float), "pulseheight": np.asarray(cond.pulseheight, float),
-            "timeoffset": np.asarray(cond.timeoffset, float), "status": np.asarray(cond.status, np.int64),
-            "roc": np.asarray(cond.roc, np.int64), "feb": np.asarray(cond.feb, np.int64),
-            "febch": ch % CHANNELS_PER_FEB}
+    return _typed("calib", {
+        "run": np.full(n, run, np.int64), "channel": ch, "pedestal": np.asarray(cond.pedestal, float),
+        "pulsearea": np.asarray(cond.pulsearea, float), "pulseheight": np.asarray(cond.pulseheight, float),
+        "timeoffset": np.asarray(cond.timeoffset, float), "status": np.asarray(cond.status, np.int64),
+        "roc": np.asarray(cond.roc, np.int64), "feb": np.asarray(cond.feb, np.int64),
+        "febch": ch % CHANNELS_PER_FEB})
 
 
 def events_tree(event_keys, pulses, clusters, digis=None, run=0):
@@ -230,9 +259,10 @@ def events_tree(event_keys, pulses, clusters, digis=None, run=0):
         return np.searchsorted(k, ev, side="right") - np.searchsorted(k, ev, side="left")
 
     n = len(ev)
-    return {"run": np.full(n, run, np.int64), "subrun": sub, "event": evt, "spill": np.zeros(n, np.int64),
-            "evlen": np.zeros(n, np.int64), "pbtime": np.zeros(n, float), "ndigi": count(digis),
-            "npulse": count(pulses), "ncluster": count(clusters)}
+    return _typed("events", {
+        "run": np.full(n, run, np.int64), "subrun": sub, "event": evt, "spill": np.zeros(n, np.int64),
+        "evlen": np.zeros(n, float), "pbtime": np.zeros(n, float), "ndigi": count(digis),
+        "npulse": count(pulses), "ncluster": count(clusters)})
 
 
 def write_reco(path, trees):

@@ -1,5 +1,6 @@
 """Stage-boundary tables in the reference's dump layouts (crvhip/dumpio.py): column sets of the dumpers, round trips
-through npz and parquet, and the derived columns (row index in the event, member pulses, layer count)."""
+through npz and parquet, the derived columns (row index in the event, member pulses, member layers) and the
+dumper's branch types."""
 import numpy as np
 import pytest
 
@@ -74,7 +75,8 @@ def test_reco_trees_round_trip_and_derived_columns(tmp_path, suffix):
     c = trees["clusters"]
     # bars 10, 26, 42 lie in layers 0, 1, 2 of module 0; bars 10 and 11 both in layer 0
     assert [int(G.layer[b]) for b in (10, 26, 42, 11)] == [0, 1, 2, 0]
-    np.testing.assert_array_equal(c["layers"], [3, 1])
+    np.testing.assert_array_equal(c["layers"].values, [0, 1, 2, 0])
+    np.testing.assert_array_equal(c["layers"].counts, [3, 1])
     np.testing.assert_array_equal(c["pulses"].counts, [3, 2])
     np.testing.assert_array_equal(trees["events"]["npulse"], [3, 2])
     np.testing.assert_array_equal(trees["events"]["ncluster"], [1, 1])
@@ -89,6 +91,25 @@ def test_reco_trees_round_trip_and_derived_columns(tmp_path, suffix):
                 np.testing.assert_array_equal(back[col].counts, v.counts, err_msg=f"{name}.{col}")
             else:
                 np.testing.assert_array_equal(back[col], v, err_msg=f"{name}.{col}")
+
+
+def test_columns_have_the_dumpers_branch_types(tmp_path):
+    # reference CrvRecoTableDumper_module.cc:59-92: unsigned ids, int counters, float PE and positions, double times
+    P, C = _tables()
+    trees = IO.reco_trees({"pulses": P, "clusters": C}, DT.nominal_conditions(), DT.sector_geom(), run=3)
+    trees["calib"] = IO.calib_tree(DT.nominal_conditions(), run=3)
+    path = str(tmp_path / "reco")
+    IO.write_reco(path, trees)
+    want = {("pulses", "event"): np.uint32, ("pulses", "pe"): np.float32, ("pulses", "time"): np.float64,
+            ("pulses", "flags"): np.int32, ("pulses", "wfidx"): np.int32, ("clusters", "layers"): np.int32,
+            ("clusters", "pe"): np.float32, ("clusters", "start"): np.float64, ("clusters", "sidehits0"): np.int32,
+            ("events", "evlen"): np.float64, ("events", "npulse"): np.int32, ("calib", "pedestal"): np.float32}
+    for (tree, col), dt in want.items():
+        back = IO.read_reco(path, tree)[col]
+        got = back.values.dtype if isinstance(back, IO.Jagged) else back.dtype
+        assert got == dt, f"{tree}.{col}: {got}"
+    for tree, cols in trees.items():
+        assert set(IO.DTYPES[tree]) == set(IO.TREES[tree]) == set(cols)
 
 
 def test_write_reco_refuses_other_columns(tmp_path):

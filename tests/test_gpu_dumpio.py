@@ -27,14 +27,18 @@ def test_step_file_to_crvdump(tmp_path, suffix):
     P, C = IO.read_reco(reco_path, "pulses"), IO.read_reco(reco_path, "clusters")
     assert len(P["idx"]) == len(out["pulses"]["ev"]) > 1000 and len(C["idx"]) == len(out["clusters"]["cl_event"])
     np.testing.assert_array_equal(P["time"], out["pulses"]["time"])
-    np.testing.assert_array_equal(C["pe"], out["clusters"]["cl_pe"])
+    np.testing.assert_array_equal(C["pe"], out["clusters"]["cl_pe"].astype(np.float32))  # a float branch
     # a cluster's members, looked up by (event, idx) in the pulse tree, are the chain's member pulses
-    pkey = (P["subrun"] << 20 | P["event"]).astype(np.int64)
-    ckey = np.repeat((C["subrun"] << 20 | C["event"]).astype(np.int64), C["pulses"].counts)
+    pkey = P["subrun"].astype(np.int64) << 20 | P["event"]
+    ckey = np.repeat(C["subrun"].astype(np.int64) << 20 | C["event"], C["pulses"].counts)
     rows = np.searchsorted(pkey, ckey) + C["pulses"].values
     np.testing.assert_array_equal(pkey[rows], ckey)
     np.testing.assert_array_equal(P["idx"][rows], C["pulses"].values)
-    assert ((C["layers"] >= 3) & (C["layers"] <= 4)).all()  # kept hits pass a three-of-four-layer combination
+    nlay = C["layers"].counts  # kept hits pass a three-of-four-layer combination
+    assert ((nlay >= 3) & (nlay <= 4)).all()
+    lay = C["layers"].values
+    first = np.repeat(np.r_[0, np.cumsum(nlay)[:-1]], nlay)
+    assert ((lay >= 0) & (lay < 4)).all() and (np.diff(lay)[np.diff(first) == 0] > 0).all()  # sorted, distinct
     E = IO.read_reco(reco_path, "events")
     assert E["npulse"].sum() == len(P["idx"]) and E["ncluster"].sum() == len(C["idx"])
     assert (E["run"] == 12).all()
