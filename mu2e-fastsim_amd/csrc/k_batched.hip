@@ -117,9 +117,10 @@ struct BStep {  // the steps of the batch (device SoA)
   const double *edep, *z, *t;
 };
 
-// key bits of a work-queue order (crv_batch_opts.sort_tasks 1, 2, 3): photon count, its octave (a bucket of at most
-// 32), its quarter octave (at most 128)
-__host__ __device__ constexpr int sort_key_bits(int mode) { return mode == 1 ? 32 : mode == 2 ? 6 : 8; }
+// key bits of a work-queue order (crv_batch_opts.sort_tasks 1, 2, 3): above the bucket, one bit that puts the SiPMs
+// of counters with several steps first; the bucket is the photon count (31 bits), its octave (at most 32) or its
+// quarter octave (at most 128)
+__host__ __device__ constexpr int sort_key_bits(int mode) { return mode == 1 ? 32 : mode == 2 ? 7 : 9; }
 
 // photoelectrons per (step, SiPM), and per SiPM in total
 __global__ void kb_count(crv_params p, Tables t, Derived dv, BStep st, const int32_t *__restrict__ Tdev,
@@ -164,10 +165,13 @@ __global__ void kb_count(crv_params p, Tables t, Derived dv, BStep st, const int
   sipm_nph[g] = total;
   // longest first, ties in SiPM order (a stable radix sort of the complement): mode 1 by photon count, modes 2 and 3
   // by its octave or quarter octave (fewer key bits, and the SiPMs of one bucket keep their order, so that
-  // neighbouring SiPMs, which share steps, stay neighbours in the queue)
-  const uint32_t v = (uint32_t)(total < 0xFFFFFFFFll ? total : 0xFFFFFFFFll);
+  // neighbouring SiPMs, which share steps, stay neighbours in the queue). Above that, the SiPMs of counters with
+  // several steps come first: kb_sipm merges their photon streams in memory and those with one step in registers, so
+  // a wavefront of one kind runs one of the two codes instead of both.
+  const uint32_t single = b - a > 1 ? 0u : 1u;
+  const uint32_t v = (uint32_t)(total < 0x7FFFFFFFll ? total : 0x7FFFFFFFll);
   if (sort_mode == 1) {
-    sort_key[g] = 0xFFFFFFFFu - v;
+    sort_key[g] = (single << 31) | (0x7FFFFFFFu - v);
   } else {
     const int mb = sort_mode == 3 ? 2 : 0;  // bits of the count below its leading one that the bucket keeps
     uint32_t bucket = 0;                    // 0 for no photons, else 1 + (octave << mb) + those bits
@@ -176,7 +180,8 @@ __global__ void kb_count(crv_params p, Tables t, Derived dv, BStep st, const int
       const uint32_t m = (e >= mb ? v >> (e - mb) : v << (mb - e)) & ((1u << mb) - 1u);
       bucket = 1u + ((uint32_t)e << mb) + m;
     }
-    sort_key[g] = ((1u << sort_key_bits(sort_mode)) - 1u) - bucket;
+    const int kb = sort_key_bits(sort_mode) - 1;
+    sort_key[g] = (single << kb) | (((1u << kb) - 1u) - bucket);
   }
   sort_idx[g] = (int32_t)g;
 }
