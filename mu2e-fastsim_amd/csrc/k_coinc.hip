@@ -1881,27 +1881,30 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
           S.cdt[ci] = best_dt;
         }
         __syncthreads();
-        for (int c0 = 0; c0 < nfilt; c0 += BT) {  // rank in (cluster, bar, sipm, dt, candidate) order
+        for (int c0 = 0; c0 < nfilt; c0 += BT) {  // the chosen candidates, in candidate order (S.tmp)
           const int ci = c0 + tid;
           const bool chosen = ci < nfilt && S.chk[ci] >= 0;
-          if (chosen) {
-            const int k = S.chk[ci], h = S.filt[ci], b = S.bar[h], sp = S.sipm[h];
-            const double dt = S.cdt[ci];
-            int rank = 0;
-            for (int cj = 0; cj < nfilt; ++cj) {
-              const int kj = S.chk[cj];
-              if (kj < 0 || cj == ci) continue;
-              const int hj = S.filt[cj], bj = S.bar[hj], sj = S.sipm[hj];
-              const double dj = S.cdt[cj];
-              rank += kj < k || (kj == k && (bj < b || (bj == b && (sj < sp || (sj == sp && (dj < dt ||
-                      (dj == dt && cj < ci)))))));
-            }
-            S.sk[rank] = k;
-            S.sc[rank] = ci;
-          }
           int tot;
-          bscan(L, chosen, tot);
+          const int r = bscan(L, chosen, tot);
+          if (chosen) S.tmp[nchosen + r] = ci;
           nchosen += tot;
+        }
+        __syncthreads();
+        for (int u = tid; u < nchosen; u += BT) {  // rank in (cluster, bar, sipm, dt, candidate) order
+          const int ci = S.tmp[u];
+          const int k = S.chk[ci], h = S.filt[ci], b = S.bar[h], sp = S.sipm[h];
+          const double dt = S.cdt[ci];
+          int rank = 0;
+          for (int v = 0; v < nchosen; ++v) {
+            const int cj = S.tmp[v];
+            if (cj == ci) continue;
+            const int kj = S.chk[cj], hj = S.filt[cj], bj = S.bar[hj], sj = S.sipm[hj];
+            const double dj = S.cdt[cj];
+            rank += kj < k || (kj == k && (bj < b || (bj == b && (sj < sp || (sj == sp && (dj < dt ||
+                    (dj == dt && cj < ci)))))));
+          }
+          S.sk[rank] = k;
+          S.sc[rank] = ci;
         }
         __syncthreads();
       }
