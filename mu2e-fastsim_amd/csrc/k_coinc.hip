@@ -816,29 +816,96 @@ __device__ __forceinline__ double block_max(BlockLds &L, double v) {
   return v;
 }
 
+// the cell grid of findClusters over times [lo, hi] and x [xlo, xhi] (largest link distance mdm): cells at least one
+// link window (max_dt, plus dt_opp across sides when `opposite`) wide in time and the largest link distance wide in
+// x, so that linked positions lie in neighbouring cells; at most maxb - 2 cells
+struct CellGrid {
+  double lo, width, xlo, xw;
+  bool fin, xfin;
+  int nbt, nbx;
+  __device__ int cell(double t, double x) const {
+    const double ft = fin ? (t - lo) / width : 0.0;
+    const int bt = ft >= 0.0 ? min((int)fmin(ft, (double)(nbt - 1)), nbt - 1) : 0;  // NaN -> bin 0
+    const double fx = xfin ? (x - xlo) / xw : 0.0;
+    const int bx = fx >= 0.0 ? min((int)fmin(fx, (double)(nbx - 1)), nbx - 1) : 0;
+    return bt * nbx + bx;
+  }
+};
+
+__device__ CellGrid cell_grid(double lo, double hi, double xlo, double xhi, double mdm, double max_dt, bool opposite,
+                              double dt_opp, int maxb) {
+  CellGrid g;
+  const double mt = max_dt + (opposite ? dt_opp : 0.0);
+  const double xspan = xhi - xlo;
+  g.xfin = xspan >= 0.0 && xspan < 1e300 && mdm >= 0.0 && mdm < 1e300;
+  const double xw0 = g.xfin ? fmax(mdm, 1e-9) * (1.0 + 1e-12) + 1e-9 : 1.0;
+  g.nbx = g.xfin ? max(1, min((int)(xspan / xw0) + 1, 32)) : 1;
+  g.xw = g.xfin ? fmax(xw0, xspan / (double)max(g.nbx - 1, 1) * (1.0 + 1e-12)) : 1.0;
+  g.xlo = xlo;
+  const int tcap = (maxb - 2) / g.nbx;
+  const double span = hi - lo;
+  g.fin = span >= 0.0 && span < 1e300;  // else one time bin
+  g.width = g.fin ? fmax(mt, span / (double)max(tcap - 1, 1)) * (1.0 + 1e-12) + 1e-9 : 1.0;
+  g.nbt = g.fin ? max(1, min((int)(span / g.width) + 1, tcap)) : 1;
+  g.lo = lo;
+  return g;
+}
+
+// the clusters of list[0..m) in seed order, from each position's label (its component's lowest position) and pass:
+// member offsets to off[], members (pulses) to out[] by (pass, position); ITM: m ints of scratch
+__device__ int components_out(BlockLds &L, const BS &S, const int32_t *list, int m, const int32_t *LAB,
+                              const int32_t *PAS, int32_t *ITM, int32_t *out, int32_t *off) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < m; i += BT) S.tmp[i] = 0;
+  __syncthreads();
+  for (int i = tid; i < m; i += BT) atomicAdd(&S.tmp[LAB[i]], 1);
+  __syncthreads();
+  int base = 0, cbase = 0;
+  for (int c = 0; c < m; c += BT) {  // member offsets and cluster numbers over seeds in position order
+    const int i = c + tid;
+    const int cnt = i < m ? S.tmp[i] : 0;
+    int tot, ctot;
+    const int r = bscan(L, cnt, tot);
+    const int cr = bscan(L, cnt > 0, ctot);
+    if (cnt > 0) {
+      off[cbase + cr] = base + r;
+      S.clid[i] = cbase + cr;
+      S.tmp[i] = base + r;  // cursor
+    }
+    base += tot;
+    cbase += ctot;
+  }
+  __syncthreads();
+  if (tid == 0) off[cbase] = base;
+  for (int i = tid; i < m; i += BT) ITM[atomicAdd(&S.tmp[LAB[i]], 1)] = i;  // positions, unordered per cluster
+  __syncthreads();
+  for (int k = tid; k < base; k += BT) {  // rank by (pass, position) within the cluster
+    const int i = ITM[k], c = S.clid[LAB[i]], pi = PAS[i];
+    const int o = off[c], e = off[c + 1];
+    int r = 0;
+    for (int q = o; q < e; ++q) {
+      const int j = ITM[q], pj = PAS[j];
+      r += pj < pi || (pj == pi && j < i);
+    }
+    out[o + r] = list[i];
+  }
+  __syncthreads();
+  return cbase;
+}
+
 // greedy findClusters of the pulses list[0..m) (reco/coinc.py:125-152): members (pulses) to out[] in join order,
 // cluster starts to off[]; returns the number of clusters. Clusters are the link graph's connected components in
 // the order of their first position; a member's join order is (pass, position) with pass(seed) = 1 and
-// pass(p) = min over linked q of pass(q) + [q > p]. The positions' times, x, link distances and sides are staged in
-// LDS (LM) or in the scratch arrays px / pt / pmd / pside; link candidates come from the neighbouring cells of a
-// (time, x) grid whose cells are at least one link window and one largest link distance wide.
-template <bool LM>
-__device__ int block_clusters_t(BlockLds &L, const BS &S, const int32_t *list, int m, const double *md,
-                                double max_dt, bool opposite, double dt_opp, int32_t *out, int32_t *off) {
+// pass(p) = min over linked q of pass(q) + [q > p]. Both are least fixpoints, reached in any order of visits. Link
+// candidates come from the neighbouring cells of a (time, x) grid (CellGrid).
+//
+// Up to CM positions: times, x, link distances and sides staged in LDS in list order.
+__device__ int block_clusters_lds(BlockLds &L, const BS &S, const int32_t *list, int m, const double *md,
+                                  double max_dt, bool opposite, double dt_opp, int32_t *out, int32_t *off) {
   const int tid = threadIdx.x;
-  double *X, *T, *MD;
-  uint8_t *SD;
-  int32_t *LAB, *PAS, *TSO, *TBI, *BINS;
-  int maxb;
-  if constexpr (LM) {
-    X = L.u.cl.x, T = L.u.cl.t, MD = L.u.cl.md, SD = L.u.cl.side;
-    LAB = L.u.cl.label, PAS = L.u.cl.pass, TSO = L.u.cl.tsort, TBI = L.u.cl.tbin, BINS = L.u.cl.bins;
-    maxb = CB;
-  } else {
-    X = S.px, T = S.pt, MD = S.pmd, SD = S.pside;
-    LAB = S.label, PAS = S.pass, TSO = S.tsort, TBI = S.tbin, BINS = L.u.hist;
-    maxb = BH;
-  }
+  double *X = L.u.cl.x, *T = L.u.cl.t, *MD = L.u.cl.md;
+  uint8_t *SD = L.u.cl.side;
+  int32_t *LAB = L.u.cl.label, *PAS = L.u.cl.pass, *TSO = L.u.cl.tsort, *TBI = L.u.cl.tbin, *BINS = L.u.cl.bins;
   double lo = INFINITY, hi = -INFINITY;
   for (int i = tid; i < m; i += BT) {
     const int h = list[i];
@@ -861,28 +928,12 @@ __device__ int block_clusters_t(BlockLds &L, const BS &S, const int32_t *list, i
   xlo = block_min(L, xlo);
   xhi = block_max(L, xhi);
   mdm = block_max(L, mdm);
-  // cells of (time, x): at least one link window wide in time and the largest link distance wide in x, so linked
-  // positions lie in neighbouring cells; at most maxb - 2 cells
-  const double mt = max_dt + (opposite ? dt_opp : 0.0);
-  const double xspan = xhi - xlo;
-  const bool xfin = xspan >= 0.0 && xspan < 1e300 && mdm >= 0.0 && mdm < 1e300;
-  const double xw0 = xfin ? fmax(mdm, 1e-9) * (1.0 + 1e-12) + 1e-9 : 1.0;
-  const int nbx = xfin ? max(1, min((int)(xspan / xw0) + 1, 32)) : 1;
-  const double xw = xfin ? fmax(xw0, xspan / (double)max(nbx - 1, 1) * (1.0 + 1e-12)) : 1.0;
-  const int tcap = (maxb - 2) / nbx;
-  const double span = hi - lo;
-  const bool fin = span >= 0.0 && span < 1e300;  // else one time bin
-  const double width = fin ? fmax(mt, span / (double)max(tcap - 1, 1)) * (1.0 + 1e-12) + 1e-9 : 1.0;
-  const int nbt = fin ? max(1, min((int)(span / width) + 1, tcap)) : 1;
-  const int nb = nbt * nbx;
+  const CellGrid G = cell_grid(lo, hi, xlo, xhi, mdm, max_dt, opposite, dt_opp, CB);
+  const int nbt = G.nbt, nbx = G.nbx, nb = nbt * nbx;
   for (int b = tid; b < nb + 1; b += BT) BINS[b] = 0;
   __syncthreads();
   for (int i = tid; i < m; i += BT) {
-    const double ft = fin ? (T[i] - lo) / width : 0.0;
-    const int bt = ft >= 0.0 ? min((int)fmin(ft, (double)(nbt - 1)), nbt - 1) : 0;  // NaN -> bin 0
-    const double fx = xfin ? (X[i] - xlo) / xw : 0.0;
-    const int bx = fx >= 0.0 ? min((int)fmin(fx, (double)(nbx - 1)), nbx - 1) : 0;
-    TBI[i] = bt * nbx + bx;
+    TBI[i] = G.cell(T[i], X[i]);
     atomicAdd(&BINS[TBI[i]], 1);
   }
   __syncthreads();
@@ -953,43 +1004,129 @@ __device__ int block_clusters_t(BlockLds &L, const BS &S, const int32_t *list, i
     __syncthreads();
     if (!__syncthreads_or(changed)) break;
   }
-  // clusters in seed order; members by (pass, position)
-  for (int i = tid; i < m; i += BT) S.tmp[i] = 0;
+  return components_out(L, S, list, m, LAB, PAS, TBI, out, off);  // the bins are no longer needed
+}
+
+// Longer lists: the positions' data gathered in cell order into global scratch (px / pt / pmd / pside; the
+// cell-order labels in tbin, passes in clid, cells in items; the inverse order in tmp), so that the candidates of
+// neighbouring positions are neighbouring memory that the wavefront reads once; the fixpoints come out in list order
+// in label / pass.
+__device__ int block_clusters_cells(BlockLds &L, const BS &S, const int32_t *list, int m, const double *md,
+                                    double max_dt, bool opposite, double dt_opp, int32_t *out, int32_t *off) {
+  const int tid = threadIdx.x;
+  double *X = S.px, *T = S.pt, *MD = S.pmd;  // cell order
+  uint8_t *SD = S.pside;
+  int32_t *TSO = S.tsort;  // cell order -> list position
+  int32_t *TBI = S.tbin;   // list position -> cell, then the cell-order labels
+  int32_t *CEL = S.items;  // cell order -> cell
+  int32_t *RNK = S.tmp;    // list position -> cell order
+  int32_t *LABC = S.tbin, *PASC = S.clid;
+  int32_t *BINS = L.u.hist;
+  double lo = INFINITY, hi = -INFINITY, xlo = INFINITY, xhi = -INFINITY, mdm = -INFINITY;
+  for (int i = tid; i < m; i += BT) {
+    const int h = list[i];
+    const double tt = S.t[h], xx = S.x[h];
+    lo = fmin(lo, tt);
+    hi = fmax(hi, tt);
+    xlo = fmin(xlo, xx);
+    xhi = fmax(xhi, xx);
+    mdm = fmax(mdm, md[h]);
+  }
+  lo = block_min(L, lo);
+  hi = block_max(L, hi);
+  xlo = block_min(L, xlo);
+  xhi = block_max(L, xhi);
+  mdm = block_max(L, mdm);
+  const CellGrid G = cell_grid(lo, hi, xlo, xhi, mdm, max_dt, opposite, dt_opp, BH);
+  const int nbt = G.nbt, nbx = G.nbx, nb = nbt * nbx;
+  for (int b = tid; b < nb + 1; b += BT) BINS[b] = 0;
   __syncthreads();
-  for (int i = tid; i < m; i += BT) atomicAdd(&S.tmp[LAB[i]], 1);
-  __syncthreads();
-  int base = 0, cbase = 0;
-  for (int c = 0; c < m; c += BT) {  // member offsets and cluster numbers over seeds in position order
-    const int i = c + tid;
-    const int cnt = i < m ? S.tmp[i] : 0;
-    int tot, ctot;
-    const int r = bscan(L, cnt, tot);
-    const int cr = bscan(L, cnt > 0, ctot);
-    if (cnt > 0) {
-      off[cbase + cr] = base + r;
-      S.clid[i] = cbase + cr;
-      S.tmp[i] = base + r;  // cursor
-    }
-    base += tot;
-    cbase += ctot;
+  for (int i = tid; i < m; i += BT) {
+    const int h = list[i];
+    TBI[i] = G.cell(S.t[h], S.x[h]);
+    atomicAdd(&BINS[TBI[i]], 1);
   }
   __syncthreads();
-  if (tid == 0) off[cbase] = base;
-  int32_t *ITM = LM ? TBI : S.items;  // the bins are no longer needed
-  for (int i = tid; i < m; i += BT) ITM[atomicAdd(&S.tmp[LAB[i]], 1)] = i;  // positions, unordered per cluster
+  bscan_arr(L, BINS, nb + 1);
+  for (int i = tid; i < m; i += BT) TSO[atomicAdd(&BINS[TBI[i]], 1)] = i;  // BINS[b] ends as the end of cell b
   __syncthreads();
-  for (int k = tid; k < base; k += BT) {  // rank by (pass, position) within the cluster
-    const int i = ITM[k], c = S.clid[LAB[i]], pi = PAS[i];
-    const int o = off[c], e = off[c + 1];
-    int r = 0;
-    for (int q = o; q < e; ++q) {
-      const int j = ITM[q], pj = PAS[j];
-      r += pj < pi || (pj == pi && j < i);
-    }
-    out[o + r] = list[i];
+  for (int k = tid; k < m; k += BT) {
+    const int i = TSO[k], h = list[i];
+    X[k] = S.x[h];
+    T[k] = S.t[h];
+    MD[k] = md[h];
+    SD[k] = S.side[h];
+    CEL[k] = TBI[i];
+    RNK[i] = k;
   }
   __syncthreads();
-  return cbase;
+  for (int k = tid; k < m; k += BT) {
+    LABC[k] = TSO[k];
+    PASC[k] = INF_PASS;
+  }
+  __syncthreads();
+  auto link = [&](int a, int b) {
+    const double mdd = fmax(MD[a], MD[b]);
+    const double w = max_dt + ((opposite && SD[a] != SD[b]) ? dt_opp : 0.0);
+    return fabs(X[a] - X[b]) <= mdd && fabs(T[a] - T[b]) < w;
+  };
+  for (;;) {  // labels: every thread leaves together (the vote is a barrier with a block-wide OR)
+    int changed = 0;
+    for (int k = tid; k < m; k += BT) {
+      const int bt = CEL[k] / nbx, bx = CEL[k] - bt * nbx;
+      int best = LABC[k];
+      for (int r = max(bt - 1, 0); r <= min(bt + 1, nbt - 1); ++r) {
+        const int c0 = r * nbx + max(bx - 1, 0), c1 = r * nbx + min(bx + 1, nbx - 1);
+        for (int k2 = c0 ? BINS[c0 - 1] : 0; k2 < BINS[c1]; ++k2) {
+          const int lj = LABC[k2];
+          if (lj < best && k2 != k && link(k, k2)) best = lj;
+        }
+      }
+      if (best < LABC[k]) {
+        atomicMin(&LABC[k], best);
+        changed = 1;
+      }
+    }
+    __syncthreads();
+    for (int k = tid; k < m; k += BT) {  // pointer jumping over positions' labels
+      int l = LABC[k];
+      while (LABC[RNK[l]] < l) l = LABC[RNK[l]];
+      LABC[k] = l;
+    }
+    __syncthreads();
+    if (!__syncthreads_or(changed)) break;
+  }
+  for (int k = tid; k < m; k += BT)
+    if (LABC[k] == TSO[k]) PASC[k] = 1;
+  __syncthreads();
+  for (;;) {  // passes
+    int changed = 0;
+    for (int k = tid; k < m; k += BT) {
+      const int i = TSO[k];
+      if (LABC[k] == i) continue;
+      const int bt = CEL[k] / nbx, bx = CEL[k] - bt * nbx;
+      int best = PASC[k];
+      for (int r = max(bt - 1, 0); r <= min(bt + 1, nbt - 1); ++r) {
+        const int c0 = r * nbx + max(bx - 1, 0), c1 = r * nbx + min(bx + 1, nbx - 1);
+        for (int k2 = c0 ? BINS[c0 - 1] : 0; k2 < BINS[c1]; ++k2) {
+          const int cand = PASC[k2] + (TSO[k2] > i ? 1 : 0);
+          if (cand < best && k2 != k && link(k, k2)) best = cand;
+        }
+      }
+      if (best < PASC[k]) {
+        PASC[k] = best;
+        changed = 1;
+      }
+    }
+    __syncthreads();
+    if (!__syncthreads_or(changed)) break;
+  }
+  for (int k = tid; k < m; k += BT) {  // back to list order
+    S.label[TSO[k]] = LABC[k];
+    S.pass[TSO[k]] = PASC[k];
+  }
+  __syncthreads();
+  return components_out(L, S, list, m, S.label, S.pass, S.items, out, off);
 }
 
 __device__ int block_clusters(BlockLds &L, const BS &S, const int32_t *list, int m, const double *md, double max_dt,
@@ -999,8 +1136,8 @@ __device__ int block_clusters(BlockLds &L, const BS &S, const int32_t *list, int
     __syncthreads();
     return 0;
   }
-  return m <= CM ? block_clusters_t<true>(L, S, list, m, md, max_dt, opposite, dt_opp, out, off)
-                 : block_clusters_t<false>(L, S, list, m, md, max_dt, opposite, dt_opp, out, off);
+  return m <= CM ? block_clusters_lds(L, S, list, m, md, max_dt, opposite, dt_opp, out, off)
+                 : block_clusters_cells(L, S, list, m, md, max_dt, opposite, dt_opp, out, off);
 }
 
 // checkCoincidence of a cluster's members mem[0..n) for one thread; by-layer lists in byl[0..n) with their widened
