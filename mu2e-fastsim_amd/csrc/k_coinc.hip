@@ -1010,7 +1010,9 @@ __device__ int block_clusters_lds(BlockLds &L, const BS &S, const int32_t *list,
 // Longer lists: the positions' data gathered in cell order into global scratch (px / pt / pmd / pside; the
 // cell-order labels in tbin, passes in clid, cells in items; the inverse order in tmp), so that the candidates of
 // neighbouring positions are neighbouring memory that the wavefront reads once; the fixpoints come out in list order
-// in label / pass.
+// in label / pass. Each position's links are found once: a bit per candidate of its three rows of cells, in row
+// order, kept as a 64-bit mask in cdt (free while clusters are formed); the rounds then visit linked candidates
+// only. A position with more than 63 candidates has bit 63 set and tests its links in every round.
 __device__ int block_clusters_cells(BlockLds &L, const BS &S, const int32_t *list, int m, const double *md,
                                     double max_dt, bool opposite, double dt_opp, int32_t *out, int32_t *off) {
   const int tid = threadIdx.x;
@@ -1070,18 +1072,47 @@ __device__ int block_clusters_cells(BlockLds &L, const BS &S, const int32_t *lis
     const double w = max_dt + ((opposite && SD[a] != SD[b]) ? dt_opp : 0.0);
     return fabs(X[a] - X[b]) <= mdd && fabs(T[a] - T[b]) < w;
   };
+  constexpr uint64_t OVER = 1ull << 63;
+  uint64_t *ADJ = reinterpret_cast<uint64_t *>(S.cdt);
+  // fn(k2) for every candidate k2 of position k linked to it (cell-order indices)
+  auto linked = [&](int k, uint64_t adj, auto &&fn) {
+    const int bt = CEL[k] / nbx, bx = CEL[k] - bt * nbx;
+    int q = 0;
+    for (int r = max(bt - 1, 0); r <= min(bt + 1, nbt - 1); ++r) {
+      const int c0 = r * nbx + max(bx - 1, 0), c1 = r * nbx + min(bx + 1, nbx - 1);
+      const int s0 = c0 ? BINS[c0 - 1] : 0, s1 = BINS[c1];
+      if (adj & OVER) {
+        for (int k2 = s0; k2 < s1; ++k2)
+          if (k2 != k && link(k, k2)) fn(k2);
+      } else {
+        for (uint64_t b = q < 63 ? (adj >> q) & ((s1 - s0 >= 64 ? ~0ull : (1ull << (s1 - s0)) - 1ull)) : 0ull; b;
+             b &= b - 1)
+          fn(s0 + __ffsll((long long)b) - 1);
+      }
+      q += s1 - s0;
+    }
+  };
+  for (int k = tid; k < m; k += BT) {
+    const int bt = CEL[k] / nbx, bx = CEL[k] - bt * nbx;
+    uint64_t adj = 0;
+    int q = 0;
+    for (int r = max(bt - 1, 0); r <= min(bt + 1, nbt - 1); ++r) {
+      const int c0 = r * nbx + max(bx - 1, 0), c1 = r * nbx + min(bx + 1, nbx - 1);
+      for (int k2 = c0 ? BINS[c0 - 1] : 0; k2 < BINS[c1] && !(adj & OVER); ++k2, ++q) {
+        if (q >= 63) adj = OVER;
+        else if (k2 != k && link(k, k2)) adj |= 1ull << q;
+      }
+    }
+    ADJ[k] = adj;
+  }
+  __syncthreads();
   for (;;) {  // labels: every thread leaves together (the vote is a barrier with a block-wide OR)
     int changed = 0;
     for (int k = tid; k < m; k += BT) {
-      const int bt = CEL[k] / nbx, bx = CEL[k] - bt * nbx;
+      const uint64_t adj = ADJ[k];
+      if (!adj) continue;
       int best = LABC[k];
-      for (int r = max(bt - 1, 0); r <= min(bt + 1, nbt - 1); ++r) {
-        const int c0 = r * nbx + max(bx - 1, 0), c1 = r * nbx + min(bx + 1, nbx - 1);
-        for (int k2 = c0 ? BINS[c0 - 1] : 0; k2 < BINS[c1]; ++k2) {
-          const int lj = LABC[k2];
-          if (lj < best && k2 != k && link(k, k2)) best = lj;
-        }
-      }
+      linked(k, adj, [&](int k2) { best = min(best, LABC[k2]); });
       if (best < LABC[k]) {
         atomicMin(&LABC[k], best);
         changed = 1;
@@ -1104,15 +1135,8 @@ __device__ int block_clusters_cells(BlockLds &L, const BS &S, const int32_t *lis
     for (int k = tid; k < m; k += BT) {
       const int i = TSO[k];
       if (LABC[k] == i) continue;
-      const int bt = CEL[k] / nbx, bx = CEL[k] - bt * nbx;
       int best = PASC[k];
-      for (int r = max(bt - 1, 0); r <= min(bt + 1, nbt - 1); ++r) {
-        const int c0 = r * nbx + max(bx - 1, 0), c1 = r * nbx + min(bx + 1, nbx - 1);
-        for (int k2 = c0 ? BINS[c0 - 1] : 0; k2 < BINS[c1]; ++k2) {
-          const int cand = PASC[k2] + (TSO[k2] > i ? 1 : 0);
-          if (cand < best && k2 != k && link(k, k2)) best = cand;
-        }
-      }
+      linked(k, ADJ[k], [&](int k2) { best = min(best, PASC[k2] + (TSO[k2] > i ? 1 : 0)); });
       if (best < PASC[k]) {
         PASC[k] = best;
         changed = 1;
