@@ -385,6 +385,16 @@ __device__ __forceinline__ bool dead_px(const Tables &t, int pix) {
   return (w >> (pix & 31)) & 1u;
 }
 
+// heads visited in increasing stream index: (h1, i1) the earliest (first index on ties), (h2, i2) the earliest of the
+// others
+__device__ __forceinline__ void earliest2(double h, int k, double &h1, int &i1, double &h2, int &i2) {
+  const bool lt1 = h < h1, lt2 = h < h2;
+  h2 = lt1 ? h1 : lt2 ? h : h2;
+  i2 = lt1 ? i1 : lt2 ? k : i2;
+  h1 = lt1 ? h : h1;
+  i1 = lt1 ? k : i1;
+}
+
 // append a firing to the SiPM's log and to the per-pixel last-fire times
 // (an even entry waits in registers, pt / pv, and goes out with the next one as one 16-byte chunk)
 __device__ __forceinline__ void log_fire(const Out &o, int64_t lb, int64_t &nlog, float &pt, float &pv, float *lastt,
@@ -521,12 +531,16 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           }
         };
         float pend_t = 0.f, pend_v = 0.f;  // the log's pending even entry
-        // photon streams: in registers for a counter with one step, in global memory otherwise
+        // photon streams: the current stream in registers; with several steps the other streams in global memory
+        // (gS, gTb, gHead, gRJ) and the earliest of their heads (th, from stream kth) in registers. Photons come from
+        // the current stream while its head is the earliest (on a tie, from the lower step); otherwise its state goes
+        // back to memory and a scan of the heads makes the earliest the current stream. Streams of separate steps
+        // seldom overlap in time, so a SiPM scans about once per step instead of once per photon.
         float S1 = 0.f;
-        double head1 = 0.0, tb1 = 0.0;
-        int r1 = 0, N1 = 0;
+        double head1 = 0.0, tb1 = 0.0, th = INFINITY;
+        int r1 = 0, N1 = 0, cur = -1, kth = 0;
         uint32_t j1 = 0;
-        U4 dq1{0u, 0u, 0u, 0u};  // the delay draw holding the stream's next four exponentials
+        U4 dq1{0u, 0u, 0u, 0u};  // the delay draw holding the current stream's next four exponentials
         for (int k = a; k < b; ++k) {
           const int64_t gi = (int64_t)k * 4 + s;
           const int N = nph[gi];
@@ -543,6 +557,7 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
           const U4 d0 = dr(KIND_DELAY, ev, (uint32_t)ch, j, 0u);
           const float S = -__logf(u32f(d0.x)) / (float)N;
           if (nstr == 1) {
+            cur = k;
             dq1 = d0;
             N1 = N;
             j1 = j;
@@ -559,89 +574,60 @@ __global__ __launch_bounds__(64) void kb_sipm(crv_params p, Tables t, Derived dv
         bool bad = false;
         for (;;) {
           // next photon: the earliest stream head (lowest step first on ties)
-          int kb = -1;
-          double tph = 0.0;
+          bool scan = false;
           if (nstr == 1) {
-            if (r1 < N1) {
-              kb = a;
-              tph = head1;
+            if (r1 >= N1) break;  // its one stream is exhausted
+          } else {
+            scan = !(r1 < N1 && (head1 < th || (head1 == th && cur < kth)));
+          }
+          if (scan) {
+            if (cur >= 0) {  // the current stream back to memory
+              const int64_t gi = (int64_t)cur * 4 + s;
+              o.gS[gi] = S1;
+              o.gHead[gi] = r1 < N1 ? head1 : INFINITY;
+              o.gRJ[gi] = (uint64_t)j1 << 32 | (uint32_t)r1;
             }
-          } else {  // one load per stream, four streams' loads in flight at a time
-            tph = INFINITY;
+            // the earliest head (first index on ties) and the earliest of the others; one load per stream, eight in
+            // flight at a time
+            double h1 = INFINITY, h2 = INFINITY;
+            int i1 = -1, i2 = 0;
             int k = a;
             for (; k + 8 <= b; k += 8) {
               double h[8];
   #pragma unroll
               for (int q = 0; q < 8; ++q) h[q] = o.gHead[(int64_t)(k + q) * 4 + s];
   #pragma unroll
-              for (int q = 0; q < 8; ++q)
-                if (h[q] < tph) {
-                  kb = k + q;
-                  tph = h[q];
-                }
+              for (int q = 0; q < 8; ++q) earliest2(h[q], k + q, h1, i1, h2, i2);
             }
-            for (; k + 4 <= b; k += 4) {
-              const double h0 = o.gHead[(int64_t)k * 4 + s], h1 = o.gHead[(int64_t)(k + 1) * 4 + s];
-              const double h2 = o.gHead[(int64_t)(k + 2) * 4 + s], h3 = o.gHead[(int64_t)(k + 3) * 4 + s];
-              if (h0 < tph) {
-                kb = k;
-                tph = h0;
-              }
-              if (h1 < tph) {
-                kb = k + 1;
-                tph = h1;
-              }
-              if (h2 < tph) {
-                kb = k + 2;
-                tph = h2;
-              }
-              if (h3 < tph) {
-                kb = k + 3;
-                tph = h3;
-              }
-            }
-            for (; k < b; ++k) {
-              const double h = o.gHead[(int64_t)k * 4 + s];
-              if (h < tph) {
-                kb = k;
-                tph = h;
-              }
-            }
+            for (; k < b; ++k) earliest2(o.gHead[(int64_t)k * 4 + s], k, h1, i1, h2, i2);
+            if (i1 < 0) break;  // every stream is exhausted
+            th = h2;
+            kth = i2;
+            cur = i1;
+            const int64_t gi = (int64_t)cur * 4 + s;
+            const uint64_t rj = o.gRJ[gi];  // one load for the step ordinal and the rank
+            j1 = (uint32_t)(rj >> 32);
+            r1 = (int)(uint32_t)rj;
+            N1 = nph[gi];
+            S1 = o.gS[gi];
+            tb1 = o.gTb[gi];
+            head1 = h1;
+            dq1 = dr(KIND_DELAY, ev, (uint32_t)ch, j1, (uint32_t)(r1 >> 2));
           }
-          if (kb < 0 || tph > p.photon_window_hi) break;  // later photons of every stream are later still
+          const double tph = head1;
+          if (tph > p.photon_window_hi) break;  // later photons of every stream are later still
           const float toff = (float)(tph - tbase);
           // advance the stream
-          uint32_t j;
-          int r;
-          if (nstr == 1) {
-            j = j1;
-            r = r1;
-            if (r1 + 1 < N1) {
-              const int rn = r1 + 1;
-              if ((rn & 3) == 0) dq1 = dr(KIND_DELAY, ev, (uint32_t)ch, j1, (uint32_t)(rn >> 2));
-              const uint32_t x = (rn & 3) == 0 ? dq1.x : (rn & 3) == 1 ? dq1.y : (rn & 3) == 2 ? dq1.z : dq1.w;
-              S1 += (-__logf(u32f(x))) / (float)(N1 - rn);
-              head1 = tb1 + (double)(tauf * S1);
-            }
-            r1 += 1;
-          } else {
-            const int64_t gi = (int64_t)kb * 4 + s;
-            const uint64_t rj = o.gRJ[gi];  // one load for the step ordinal and the rank
-            j = (uint32_t)(rj >> 32);
-            r = (int)(uint32_t)rj;
-            const int N = nph[gi];
-            if (r + 1 < N) {
-              const int rn = r + 1;
-              const U4 uu = dr(KIND_DELAY, ev, (uint32_t)ch, j, (uint32_t)(rn >> 2));
-              const uint32_t x = (rn & 3) == 0 ? uu.x : (rn & 3) == 1 ? uu.y : (rn & 3) == 2 ? uu.z : uu.w;
-              const float S = o.gS[gi] + (-__logf(u32f(x))) / (float)(N - rn);
-              o.gS[gi] = S;
-              o.gHead[gi] = o.gTb[gi] + (double)(tauf * S);
-            } else {
-              o.gHead[gi] = INFINITY;
-            }
-            o.gRJ[gi] = rj + 1u;
+          const uint32_t j = j1;
+          const int r = r1;
+          if (r1 + 1 < N1) {
+            const int rn = r1 + 1;
+            if ((rn & 3) == 0) dq1 = dr(KIND_DELAY, ev, (uint32_t)ch, j1, (uint32_t)(rn >> 2));
+            const uint32_t x = (rn & 3) == 0 ? dq1.x : (rn & 3) == 1 ? dq1.y : (rn & 3) == 2 ? dq1.z : dq1.w;
+            S1 += (-__logf(u32f(x))) / (float)(N1 - rn);
+            head1 = tb1 + (double)(tauf * S1);
           }
+          r1 += 1;
           // pixel of the photon (photon-map CDF), firing with the recovered overvoltage
           const U4 u = dr(KIND_PIXEL, ev, (uint32_t)ch, j, pack_photon((uint32_t)r, 0u, 0u));
           const double xc = unif(u.x) * t.cdf_total;
