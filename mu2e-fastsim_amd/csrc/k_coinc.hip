@@ -1007,12 +1007,36 @@ __device__ int block_clusters_lds(BlockLds &L, const BS &S, const int32_t *list,
   return components_out(L, S, list, m, LAB, PAS, TBI, out, off);  // the bins are no longer needed
 }
 
+// union-find over cell-order positions (par in LDS): a root hooks under the root of the lower list position, so the
+// root of every component ends as its lowest list position
+__device__ __forceinline__ int uf_find(const volatile int *par, int x) {
+  int y;
+  while ((y = par[x]) != x) x = y;
+  return x;
+}
+__device__ void uf_unite(int *par, const int32_t *tso, int a, int b) {
+  const volatile int *vp = par;
+  for (;;) {
+    a = uf_find(vp, a);
+    b = uf_find(vp, b);
+    if (a == b) return;
+    if (tso[a] < tso[b]) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(par + a, a, b) == a) return;  // else a was hooked meanwhile: again from its new root
+  }
+}
+
 // Longer lists: the positions' data gathered in cell order into global scratch (px / pt / pmd / pside; the
 // cell-order labels in tbin, passes in clid, cells in items; the inverse order in tmp), so that the candidates of
 // neighbouring positions are neighbouring memory that the wavefront reads once; the fixpoints come out in list order
 // in label / pass. Each position's links are found once: a bit per candidate of its three rows of cells, in row
 // order, kept as a 64-bit mask in cdt (free while clusters are formed); the rounds then visit linked candidates
-// only. A position with more than 63 candidates has bit 63 set and tests its links in every round.
+// only. A position with more than 63 candidates has bit 63 set and tests its links in every round. While the bins
+// leave room for two ints per position in the LDS histogram (up to ~5,600 positions), the labels come from a
+// union-find over the links (one pass) and the pass rounds run on LDS; longer lists propagate labels in rounds.
 __device__ int block_clusters_cells(BlockLds &L, const BS &S, const int32_t *list, int m, const double *md,
                                     double max_dt, bool opposite, double dt_opp, int32_t *out, int32_t *off) {
   const int tid = threadIdx.x;
@@ -1039,7 +1063,8 @@ __device__ int block_clusters_cells(BlockLds &L, const BS &S, const int32_t *lis
   xlo = block_min(L, xlo);
   xhi = block_max(L, xhi);
   mdm = block_max(L, mdm);
-  const CellGrid G = cell_grid(lo, hi, xlo, xhi, mdm, max_dt, opposite, dt_opp, BH);
+  const bool lds_rounds = 2 * m <= BH - CB;  // parents and passes beside at most BH - 2 m bins
+  const CellGrid G = cell_grid(lo, hi, xlo, xhi, mdm, max_dt, opposite, dt_opp, lds_rounds ? BH - 2 * m : BH);
   const int nbt = G.nbt, nbx = G.nbx, nb = nbt * nbx;
   for (int b = tid; b < nb + 1; b += BT) BINS[b] = 0;
   __syncthreads();
@@ -1106,6 +1131,45 @@ __device__ int block_clusters_cells(BlockLds &L, const BS &S, const int32_t *lis
     ADJ[k] = adj;
   }
   __syncthreads();
+  if (lds_rounds) {
+    int *PAR = BINS + nb + 1, *PASL = PAR + m;
+    for (int k = tid; k < m; k += BT) PAR[k] = k;
+    __syncthreads();
+    for (int k = tid; k < m; k += BT) {
+      const uint64_t adj = ADJ[k];
+      if (adj) linked(k, adj, [&](int k2) {
+          if (k2 > k) uf_unite(PAR, TSO, k, k2);  // each link once
+        });
+    }
+    __syncthreads();
+    for (int k = tid; k < m; k += BT) {
+      const int r = uf_find(PAR, k);
+      LABC[k] = TSO[r];
+      PASL[k] = r == k ? 1 : INF_PASS;  // the seed of its component
+    }
+    __syncthreads();
+    for (;;) {  // passes
+      int changed = 0;
+      for (int k = tid; k < m; k += BT) {
+        if (PASL[k] == 1) continue;  // a seed, or already in the first pass: no lower pass exists
+        const int i = TSO[k];
+        int best = PASL[k];
+        linked(k, ADJ[k], [&](int k2) { best = min(best, PASL[k2] + (TSO[k2] > i ? 1 : 0)); });
+        if (best < PASL[k]) {
+          PASL[k] = best;
+          changed = 1;
+        }
+      }
+      __syncthreads();
+      if (!__syncthreads_or(changed)) break;
+    }
+    for (int k = tid; k < m; k += BT) {  // back to list order
+      S.label[TSO[k]] = LABC[k];
+      S.pass[TSO[k]] = PASL[k];
+    }
+    __syncthreads();
+    return components_out(L, S, list, m, S.label, S.pass, S.items, out, off);
+  }
   for (;;) {  // labels: every thread leaves together (the vote is a barrier with a block-wide OR)
     int changed = 0;
     for (int k = tid; k < m; k += BT) {
