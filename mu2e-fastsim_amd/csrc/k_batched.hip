@@ -1017,8 +1017,11 @@ __device__ __forceinline__ void wave_close(const crv_params &p, const Tables &t,
   }
   while (w.E < length + 2) wave_step(p, L, o, w, true, length);
   if (w.nd == 0 || w.rec < 0) return;  // no digis (the slot stays reserved), or the record pool is full
-  const int pedw = adc_f(p, 0.f);
-  for (int J = w.E; J < W; ++J) put_adc(o, L, w, J, (uint32_t)pedw);
+  const int pedw = adc_f(p, 0.f);  // the rest of the record: pedestal words, whole 8-sample chunks at a time
+  int J = w.E;
+  for (; J < W && ((W & 7) != 0 || (J & 7) != 0); ++J) put_adc(o, L, w, J, (uint32_t)pedw);
+  const uint32_t pp = ((uint32_t)pedw & 0xFFFFu) * 0x10001u;
+  for (; J < W; J += 8) *(uint4 *)(o.rec_adc + (int64_t)w.rec * W + J) = make_uint4(pp, pp, pp, pp);
   o.rec_label[w.rec] = L.evkey * CRV_CHAN_MULT + L.ch;
   o.rec_tick[w.rec] = w.kf;
   ++L.used;
