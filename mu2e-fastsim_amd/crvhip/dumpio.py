@@ -21,8 +21,9 @@ pulse tree; `flags` bit 0 is the failed fit (the bit the reference decodes, reco
 sorted distinct layers of the cluster's member pulses (reference reco/coinc.py:391, read back as a jagged int column by
 CrvCoincidenceClustersFromTable_module.cc:69); `wfidx` holds the pulse's reco-waveform number in the event (this chain
 keeps no art digi collection to point into); spill, evlen and pbtime are 0 (no onspill folding). Column types are the
-dumper's branch types (CrvRecoTableDumper_module.cc:59-92): unsigned run, subrun, event; int counters and indices;
-float PE, height, slope, positions and calibration constants; double times; short ADC words.
+dumper's branch types (CrvRecoTableDumper_module.cc:59-92; its reader, CrvRecoTable.hh:33-37, keys rows by UInt_t
+run, subrun, event): unsigned run, subrun, event; int counters and indices; float PE, height, slope, positions and
+calibration constants; double times; short ADC words.
 
 Storage: a path ending in `.npz` is one archive with keys `<tree>/<column>`, a jagged column stored flat with its
 per-row counts under `<tree>/n<column>` (the counted-array convention the reference's table reader accepts,
