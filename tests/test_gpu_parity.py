@@ -172,6 +172,28 @@ def test_clusters_match_on_dense_events():
     _cluster_compare(out, ref)
 
 
+def test_clusters_match_on_merged_dense_events():
+    """Two events of twenty showers each (1,400-1,900 hits on a side): lists longer than the block finder's LDS stage,
+    clustered in cell order with a union-find over the links, dense components included; all must equal the
+    oracle."""
+    cfg = CF.ChainConfig()
+    cond = DT.nominal_conditions()
+    G = DT.sector_geom()
+    ev, bar, sipm, time, pe, _ = _dense_pulses(np.random.default_rng(33), 40)
+    ev = ev // 20
+    o = np.lexsort((time, sipm, bar, ev))
+    ev, bar, sipm, time, pe = ev[o], bar[o], sipm[o], time[o], pe[o]
+    key = (ev * DT.N_COUNTERS + bar) * 4 + sipm
+    start = np.r_[0, np.flatnonzero(np.diff(key)) + 1]
+    seq = np.arange(len(key)) - np.repeat(start, np.diff(np.r_[start, len(key)]))  # rank on its SiPM
+    for side in (0, 1):
+        assert (np.bincount(ev[sipm % 2 == side]) > 1300).all()
+    out = CH.find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G)
+    ref = OC.find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G)
+    assert len(ref["cl_event"]) >= 30
+    _cluster_compare(out, ref)
+
+
 def test_clusters_match_reference_golden(golden):
     cfg = CF.ChainConfig()
     cond = DT.nominal_conditions()
