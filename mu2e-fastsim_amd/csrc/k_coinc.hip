@@ -985,7 +985,7 @@ __device__ int block_clusters_lds(BlockLds &L, const BS &S, const int32_t *list,
   for (;;) {
     int changed = 0;
     for (int i = tid; i < m; i += BT) {
-      if (LAB[i] == i) continue;
+      if (PAS[i] == 1) continue;  // a seed or a first-pass member: no lower pass exists
       const int bt = TBI[i] / nbx, bx = TBI[i] - bt * nbx;
       int best = PAS[i];
       for (int r = max(bt - 1, 0); r <= min(bt + 1, nbt - 1); ++r) {
