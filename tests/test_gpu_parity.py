@@ -194,6 +194,28 @@ def test_clusters_match_on_merged_dense_events():
     _cluster_compare(out, ref)
 
 
+def test_clusters_match_on_eighty_merged_showers():
+    """One event of eighty showers (6,400 hits on a side): longer than the block finder's union-find fits beside the
+    cell bins in LDS, so its labels propagate in rounds over the cell-order lists; must equal the oracle (the oracle
+    takes about a minute here)."""
+    cfg = CF.ChainConfig()
+    cond = DT.nominal_conditions()
+    G = DT.sector_geom()
+    ev, bar, sipm, time, pe, _ = _dense_pulses(np.random.default_rng(35), 80)
+    ev = np.zeros_like(ev)
+    o = np.lexsort((time, sipm, bar))
+    bar, sipm, time, pe = bar[o], sipm[o], time[o], pe[o]
+    key = bar * 4 + sipm
+    start = np.r_[0, np.flatnonzero(np.diff(key)) + 1]
+    seq = np.arange(len(key)) - np.repeat(start, np.diff(np.r_[start, len(key)]))
+    for side in (0, 1):
+        assert (sipm % 2 == side).sum() > 6000
+    out = CH.find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G)
+    ref = OC.find_clusters(ev, bar, sipm, time, pe, seq, cfg, cond, G)
+    assert len(ref["cl_event"]) >= 50
+    _cluster_compare(out, ref)
+
+
 def test_clusters_match_reference_golden(golden):
     cfg = CF.ChainConfig()
     cond = DT.nominal_conditions()
