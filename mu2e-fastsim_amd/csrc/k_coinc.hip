@@ -611,29 +611,33 @@ __global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64
           }
         }
         __syncthreads();
-        // order (cluster, bar, sipm, dt, candidate): rank of each chosen candidate among the chosen
-        for (int base = 0; base < nfilt; base += 64) {
+        for (int base = 0; base < nfilt; base += 64) {  // the chosen candidates in candidate order (items is free)
           const int ci = base + lane;
           const bool chosen = ci < nfilt && S.chk[ci] >= 0;
-          if (chosen) {
-            const int k = S.chk[ci], h = S.filt[ci];
-            const int b = S.bar[h], sp = S.sipm[h];
-            const double dt = S.cdt[ci];
-            int rank = 0;
-            for (int cj = 0; cj < nfilt; ++cj) {
-              const int kj = S.chk[cj];
-              if (kj < 0 || cj == ci) continue;
-              const int hj = S.filt[cj];
-              const int bj = S.bar[hj], sj = S.sipm[hj];
-              const double dj = S.cdt[cj];
-              const bool less = kj < k || (kj == k && (bj < b || (bj == b && (sj < sp || (sj == sp && (dj < dt ||
-                                (dj == dt && cj < ci)))))));
-              rank += less;
-            }
-            S.sk[rank] = (int16_t)k;
-            S.sc[rank] = (uint8_t)ci;
+          const uint64_t bm = __ballot(chosen);
+          if (chosen) S.items[nchosen + __popcll(bm & lanes_below())] = (uint8_t)ci;
+          nchosen += __popcll(bm);
+        }
+        __syncthreads();
+        // order (cluster, bar, sipm, dt, candidate): rank of each chosen candidate among the chosen
+        for (int u = lane; u < nchosen; u += 64) {
+          const int ci = S.items[u];
+          const int k = S.chk[ci], h = S.filt[ci];
+          const int b = S.bar[h], sp = S.sipm[h];
+          const double dt = S.cdt[ci];
+          int rank = 0;
+          for (int v = 0; v < nchosen; ++v) {
+            const int cj = S.items[v];
+            if (cj == ci) continue;
+            const int kj = S.chk[cj], hj = S.filt[cj];
+            const int bj = S.bar[hj], sj = S.sipm[hj];
+            const double dj = S.cdt[cj];
+            const bool less = kj < k || (kj == k && (bj < b || (bj == b && (sj < sp || (sj == sp && (dj < dt ||
+                              (dj == dt && cj < ci)))))));
+            rank += less;
           }
-          nchosen += __popcll(__ballot(chosen));
+          S.sk[rank] = (int16_t)k;
+          S.sc[rank] = (uint8_t)ci;
         }
         __syncthreads();
       }
