@@ -3,7 +3,7 @@
 Imports, read-only, the reference's numpy-only modules from /root/reference/crv/crvfastsim (response/sipm.py,
 reco/waveform.py, reco/pulsefit.py, reco/coinc.py, reco/synth.py) and records their outputs on small inputs as .npz
 fixtures next to this script. The fixtures hold data only (inputs and outputs); the tests compare the oracle with
-them, and the GPU box never sees the reference.
+them, and nothing that runs the GPU tests reads the reference.
 
   python tests/golden/make_golden.py            (reference_vectors.npz)
   python tests/golden/make_golden.py noise      (sipm_noise_vectors.npz: dark noise and trap afterpulses)
