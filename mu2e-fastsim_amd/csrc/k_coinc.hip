@@ -452,20 +452,21 @@ __device__ int wave_check(const crv_params &p, const Tables &t, WS<WMAX> &S, con
 }
 
 template <int WMAX>
-__global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, int64_t nev, const int64_t *__restrict__ eoff,
+__global__ __launch_bounds__(64) void k_coinc_wave(crv_params p, Tables t, const int32_t *__restrict__ list,
+                                                   const int *__restrict__ lcount, const int64_t *__restrict__ eoff,
                                                    const int64_t *__restrict__ evk, const double *__restrict__ ptime,
                                                    const double *__restrict__ ppe, Ev e, int32_t *__restrict__ ncl,
                                                    int32_t *__restrict__ nmem, ClSlots slots,
                                                    int32_t *__restrict__ cl_moff, int32_t *__restrict__ mem,
-                                                   int64_t min_n, int *__restrict__ flags,
-                                                   const int32_t *__restrict__ nev_dev = nullptr) {
+                                                   int *__restrict__ flags) {
   constexpr int WNW = WMAX / 64;
   __shared__ WS<WMAX> S;
   const int lane = threadIdx.x;
-  if (nev_dev) nev = min(nev, (int64_t)*nev_dev);  // fixed-capacity calls: the events of the call
-  for (int64_t g = blockIdx.x; g < nev; g += gridDim.x) {
+  const int64_t nlist = *lcount;  // the events of this size class (k_block_keys' list)
+  for (int64_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const int64_t g = list[li];
     const int64_t a = eoff[g], n64 = eoff[g + 1] - a;
-    if (n64 > WMAX || n64 < min_n) continue;  // another size class takes it
+    if (n64 > WMAX) continue;  // not of this class (never listed so)
     const int n = (int)n64;
     const int64_t evkey = evk[g];
     int32_t *cl_moff_e = cl_moff + a + g;
@@ -2209,13 +2210,27 @@ __global__ __launch_bounds__(BT) void k_coinc_block(crv_params p, Tables t, int6
 
 // the block finder's events: each event of at least min_n pulses appended (in no particular order) with its pulse
 // count, at most 65535, as the queue's sort key
+// and the wavefront finders' events: class c (0: <= 64 pulses, 1: <= 128, 2: <= 256) listed at lists + c * nev, in no
+// particular order, their numbers in count[1 + c] (one atomic per wavefront and class)
 __global__ void k_block_keys(int64_t nev, const int64_t *__restrict__ eoff, int64_t min_n, uint32_t *__restrict__ key,
-                             int32_t *__restrict__ idx, int *__restrict__ count,
+                             int32_t *__restrict__ idx, int *__restrict__ count, int32_t *__restrict__ lists,
                              const int32_t *__restrict__ nev_dev = nullptr) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nev || (nev_dev && g >= *nev_dev)) return;
   const int64_t n = eoff[g + 1] - eoff[g];
-  if (n < min_n) return;
+  const int cls = n <= 64 ? 0 : n <= 128 ? 1 : n < min_n ? 2 : 3;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const uint64_t bm = __ballot(cls == c);
+    if (!bm) continue;
+    const int leader = __ffsll((long long)bm) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&count[1 + c], __popcll(bm));
+    base = __shfl(base, leader);
+    if (cls == c) lists[c * nev + base + __popcll(bm & lanes_below())] = (int32_t)g;
+  }
+  if (cls < 3) return;
   const int k = atomicAdd(count, 1);
   key[k] = (uint32_t)(n < 65535 ? n : 65535);
   idx[k] = (int32_t)g;
@@ -2318,7 +2333,6 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     if (c->t.nbars + 1 > BH || 8 * c->t.ncit + 1 > BH)
       return set_error(c, CRV_E_ARG, "clusters: geometry too large for the block finder's bar / column buckets");
     const ClSlots slots{cl_ev, cl_type, cl_pc, cl_sh, cl_d};
-    const int grid = (int)std::min<int64_t>(nev, 16384);
     BS b{};
     double **bd[] = {&b.t, &b.pe, &b.x, &b.y, &b.maxd, &b.wm, &b.cmaxd, &b.wq, &b.cdt, &b.px, &b.pt, &b.pmd};
     for (int i = 0; i < 12; ++i) *bd[i] = (double *)ws(c, i < 9 ? 130 + i : 171 + i, ns * 8);
@@ -2334,14 +2348,17 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     // sorted by decreasing pulse count (16-bit keys) on the block finder's side stream
     uint32_t *qk = (uint32_t *)ws(c, 186, nev * 4 + 4), *qk2 = (uint32_t *)ws(c, 187, nev * 4 + 4);
     int32_t *qi = (int32_t *)ws(c, 188, nev * 4 + 4), *qo = (int32_t *)ws(c, 189, nev * 4 + 4);
-    int *qctr = (int *)ws(c, 190, 64);  // queue position, then the number of block-finder events
-    if (!qk || !qk2 || !qi || !qo || !qctr) return ws_fail(c);
-    CRV_HIP(c, hipMemsetAsync(qctr, 0, 8, s));
-    k_block_keys<<<nblocks(nev, 256), 256, 0, s>>>(nev, eoff, (int64_t)257, qk, qi, qctr + 1);
+    // queue position, the number of block-finder events, the numbers of the three wavefront classes' events
+    int *qctr = (int *)ws(c, 190, 64);
+    int32_t *wl = (int32_t *)ws(c, 192, nev * 12 + 16);  // the wavefront classes' event lists
+    if (!qk || !qk2 || !qi || !qo || !qctr || !wl) return ws_fail(c);
+    CRV_HIP(c, hipMemsetAsync(qctr, 0, 20, s));
+    k_block_keys<<<nblocks(nev, 256), 256, 0, s>>>(nev, eoff, (int64_t)257, qk, qi, qctr + 1, wl);
     CRV_HIP(c, hipGetLastError());
-    int nq = 0;
-    CRV_HIP(c, hipMemcpyAsync(&nq, qctr + 1, 4, hipMemcpyDeviceToHost, s));
+    int hc[4] = {0, 0, 0, 0};
+    CRV_HIP(c, hipMemcpyAsync(hc, qctr + 1, sizeof(hc), hipMemcpyDeviceToHost, s));
     CRV_HIP(c, hipStreamSynchronize(s));
+    const int nq = hc[0];
     size_t qtb = 0;
     void *qtmp = nullptr;
     if (nq > 0) {
@@ -2351,15 +2368,24 @@ int clusters_stage(Ctx *c, const int64_t *ev, const int32_t *bar, const int32_t 
     CRV_HIP(c, hipEventRecord(c->fork[0], s));
     CRV_HIP(c, hipStreamWaitEvent(c->aux[0], c->fork[0], 0));
     CRV_HIP(c, hipStreamWaitEvent(c->aux[1], c->fork[0], 0));
-    k_coinc_wave<64><<<grid, 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff, mem,
-                                         (int64_t)0, c->flags);
-    const hipError_t e0 = hipGetLastError();
-    k_coinc_wave<128><<<grid, 64, 0, c->aux[0]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff,
-                                                  mem, (int64_t)65, c->flags);
-    const hipError_t e1 = hipGetLastError();
-    k_coinc_wave<256><<<std::min(grid, 4096), 64, 0, c->aux[1]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem,
-                                                                  slots, cl_moff, mem, (int64_t)129, c->flags);
-    const hipError_t e2 = hipGetLastError();
+    hipError_t e0 = hipSuccess, e1 = hipSuccess, e2 = hipSuccess;
+    if (hc[1]) {
+      k_coinc_wave<64><<<std::min(hc[1], 16384), 64, 0, s>>>(c->p, c->t, wl, qctr + 2, eoff, evk, time, pe, e, ncl,
+                                                             nmem, slots, cl_moff, mem, c->flags);
+      e0 = hipGetLastError();
+    }
+    if (hc[2]) {
+      k_coinc_wave<128><<<std::min(hc[2], 16384), 64, 0, c->aux[0]>>>(c->p, c->t, wl + nev, qctr + 3, eoff, evk, time,
+                                                                      pe, e, ncl, nmem, slots, cl_moff, mem,
+                                                                      c->flags);
+      e1 = hipGetLastError();
+    }
+    if (hc[3]) {
+      k_coinc_wave<256><<<std::min(hc[3], 4096), 64, 0, c->aux[1]>>>(c->p, c->t, wl + 2 * nev, qctr + 4, eoff, evk,
+                                                                     time, pe, e, ncl, nmem, slots, cl_moff, mem,
+                                                                     c->flags);
+      e2 = hipGetLastError();
+    }
     hipError_t e3 = hipSuccess;
     if (nq > 0) {
       e3 = hipcub::DeviceRadixSort::SortPairsDescending(qtmp, qtb, qk, qk2, qi, qo, nq, 0, 16, c->aux[1]);
@@ -2478,8 +2504,9 @@ int clusters_fixed(Ctx *c, Plan &pl, Arena &ar, const int64_t *ev, const int32_t
   uint32_t *qk = (uint32_t *)A(nev * 4 + 4), *qk2 = (uint32_t *)A(nev * 4 + 4);
   int32_t *qi = (int32_t *)A(nev * 4 + 4), *qo = (int32_t *)A(nev * 4 + 4);
   int *qctr = (int *)A(64);
+  int32_t *wl = (int32_t *)A(nev * 12 + 16);
   bool ok = evk && cnt && nruns && nev_dev && eoff && ncl && nmem && cl_ev && cl_type && cl_pc && cl_sh && cl_d &&
-            cl_moff && mem && clo && mo && qk && qk2 && qi && qo && qctr;
+            cl_moff && mem && clo && mo && qk && qk2 && qi && qo && qctr && wl;
   for (auto q : bd) ok = ok && *q;
   for (auto q : bi) ok = ok && *q;
   for (auto q : bu) ok = ok && *q;
@@ -2500,7 +2527,7 @@ int clusters_fixed(Ctx *c, Plan &pl, Arena &ar, const int64_t *ev, const int32_t
   k_zero_i32<<<nblocks(nev + 1, 256), 256, 0, s>>>((int32_t *)qk, nev + 1);
   k_zero_i32<<<nblocks(nev + 1, 256), 256, 0, s>>>(ncl, nev + 1);   // events past the call's find nothing
   k_zero_i32<<<nblocks(nev + 1, 256), 256, 0, s>>>(nmem, nev + 1);
-  k_block_keys<<<nblocks(nev, 256), 256, 0, s>>>(nev, eoff, (int64_t)257, qk, qi, qctr + 1, nev_dev);
+  k_block_keys<<<nblocks(nev, 256), 256, 0, s>>>(nev, eoff, (int64_t)257, qk, qi, qctr + 1, wl, nev_dev);
   CRV_HIP(c, hipGetLastError());
   size_t qtb = 0;
   CRV_HIP(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, qtb, qk, qk2, qi, qo, (int)nev, 0, 16, c->aux[1]));
@@ -2509,14 +2536,14 @@ int clusters_fixed(Ctx *c, Plan &pl, Arena &ar, const int64_t *ev, const int32_t
   CRV_HIP(c, hipEventRecord(c->fork[0], s));
   CRV_HIP(c, hipStreamWaitEvent(c->aux[0], c->fork[0], 0));
   CRV_HIP(c, hipStreamWaitEvent(c->aux[1], c->fork[0], 0));
-  k_coinc_wave<64><<<grid, 64, 0, s>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff, mem,
-                                       (int64_t)0, c->flags, nev_dev);
+  k_coinc_wave<64><<<grid, 64, 0, s>>>(c->p, c->t, wl, qctr + 2, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff,
+                                       mem, c->flags);
   const hipError_t e0 = hipGetLastError();
-  k_coinc_wave<128><<<grid, 64, 0, c->aux[0]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem, slots, cl_moff,
-                                                mem, (int64_t)65, c->flags, nev_dev);
+  k_coinc_wave<128><<<grid, 64, 0, c->aux[0]>>>(c->p, c->t, wl + nev, qctr + 3, eoff, evk, time, pe, e, ncl, nmem,
+                                                slots, cl_moff, mem, c->flags);
   const hipError_t e1 = hipGetLastError();
-  k_coinc_wave<256><<<std::min(grid, 4096), 64, 0, c->aux[1]>>>(c->p, c->t, nev, eoff, evk, time, pe, e, ncl, nmem,
-                                                                slots, cl_moff, mem, (int64_t)129, c->flags, nev_dev);
+  k_coinc_wave<256><<<std::min(grid, 4096), 64, 0, c->aux[1]>>>(c->p, c->t, wl + 2 * nev, qctr + 4, eoff, evk, time,
+                                                                pe, e, ncl, nmem, slots, cl_moff, mem, c->flags);
   const hipError_t e2 = hipGetLastError();
   hipError_t e3 = hipcub::DeviceRadixSort::SortPairsDescending(qtmp, qtb, qk, qk2, qi, qo, (int)nev, 0, 16, c->aux[1]);
   if (e3 == hipSuccess) {
