@@ -288,14 +288,14 @@ def kernels_fp32(avg, cnt, fires):
 # HBM bytes per event actually moved by a kernel on the cosmic workload: rocprofv3 FETCH_SIZE + WRITE_SIZE passes
 # over 1M events, each scaled by the factor that makes a plain 1 GiB device copy read and write 1 GiB
 # (profiles/chain_fp32_cosmic1m_traffic.txt); used for roofline.traffic of the same workload only
-PMC_BYTES_PER_EVENT = {"cosmic": {"kb_sipm": 90926.8, "kb_wrec": 35396.3, "kb_fit": 24969.6}}
+PMC_BYTES_PER_EVENT = {"cosmic": {"kb_sipm": 90919.7, "kb_wrec": 35395.0, "kb_fit": 24742.5}}
 # VALU issue and active lanes of the chain's kernels on the cosmic workload, from rocprofv3 SQ counters over 1M events
 # (profiles/chain_fp32_cosmic1m_valu.txt): busy = 4 SQ_ACTIVE_INST_VALU / 1024 SIMDs / GRBM_GUI_ACTIVE cycles per XCD,
 # lanes = SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)
-PMC_VALU = {"cosmic": {"kb_sipm": {"busy": 0.794, "lanes": 0.346, "valu_per_event": 35907},
-                       "kb_wrec": {"busy": 0.794, "lanes": 0.427, "valu_per_event": 10792},
-                       "kb_fit": {"busy": 1.084, "lanes": 0.664, "valu_per_event": 10208},
-                       "k_coinc_wave<64>": {"busy": 0.557, "lanes": 0.538, "valu_per_event": 7183}}}
+PMC_VALU = {"cosmic": {"kb_sipm": {"busy": 0.795, "lanes": 0.346, "valu_per_event": 35907},
+                       "kb_wrec": {"busy": 0.792, "lanes": 0.427, "valu_per_event": 10794},
+                       "kb_fit": {"busy": 1.088, "lanes": 0.664, "valu_per_event": 10209},
+                       "k_coinc_wave<64>": {"busy": 0.569, "lanes": 0.539, "valu_per_event": 7175}}}
 
 
 def roofline(kernels, workload=None, events=0, per_call=1):
